@@ -1,0 +1,188 @@
+"""Throughput benchmark of the backward-LPDM step engine on one MI355X.
+
+Workload ("plain-gpu-1M-24h"): one release of 2^20 particles, 24 h backward at
+dt = 60 s (1,440 steps), full physics (Hanna three-regime turbulence with the
+Richardson closure aloft, meander on, per-column Emanuel convection once per
+met hour), synthetic met on a 240 x 160 grid at 0.25 deg with 60 AGL levels and
+25 hourly snapshots, footprint 24 time-ago bins x 1 layer x 160 x 240 in float32.
+
+A "step" is one model time step of that run over all 2^20 particles: the fused
+step kernel, the deterministic footprint accumulation and, on a bracket change,
+the hourly resample, window build and convection. The met source hours are
+resident in HBM before the timed region starts. ``value`` counts active
+particle-steps per second of wall time over the timed steps.
+
+Prints one JSON line. ``--gpus N`` with N > 1 exits: this engine runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+METRIC = "particle-steps/s, batched vs plain at stated error; p50/p99 footprint latency"
+WORKLOAD = "plain-gpu-1M-24h"
+PEAK_HBM_GBS = 8000.0
+# Algorithmic HBM bytes per active particle-step of the fused step kernel
+# (float32 structure-of-arrays state): read lon lat z u' v' w' u_m v_m, weight,
+# r_off, we_off, rel_idx, alive (49 B); write lon lat z u' v' w' u_m v_m, alive
+# (33 B); emit a 4 B cell key and a 4 B value (8 B). Noise is generated in the
+# kernel (0 B). Total 90 B.
+BYTES_PER_PARTICLE_STEP = 90
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1440)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--met", choices=("f32", "bf16"), default="f32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-particles", type=int, default=32768)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    return ap.parse_args()
+
+
+def cpu_baseline(cfg, n_particles: int, n_steps: int) -> dict:
+    """The oracle's float32 plain method (the reference's CPU path, restated)
+    on a bounded sample of the same workload: the first ``n_steps`` steps of
+    the run with ``n_particles`` particles of the same release on the same
+    grid. Times the step loop only."""
+
+    import numpy as np
+
+    from oracle import lpdm as O
+    from oracle import run as ORUN
+
+    rel = cfg["releases"][0]
+    sample = dict(cfg)
+    sample["releases"] = [type(rel)(**{**rel.__dict__, "n_particles": n_particles})]
+    sample["convection"] = False
+    res = ORUN.run(sample, dty=np.float32, params=O.HannaParams(), max_steps=n_steps)
+    return {
+        "value": res.particle_steps / res.step_seconds,
+        "unit": "particle-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"first {n_steps} steps of {WORKLOAD} with {n_particles} particles "
+                   f"(float32 numpy restatement of the reference's plain method, "
+                   f"{res.particle_steps} particle-steps in {res.step_seconds:.2f} s, one core)"),
+    }
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard by release with no collective in the step)", file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    from glide_amd import _lib as L
+    from glide_amd.runner import BatchEngine
+    from glide_amd.synthetic import SyntheticMet, workload_config
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    torch.cuda.set_device(0)
+    cfg = workload_config(WORKLOAD)
+    met = SyntheticMet(cfg["spec"])
+    met_dtype = L.GLIDE_MET_BF16 if args.met == "bf16" else L.GLIDE_MET_F32
+    eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=met_dtype)
+    n_sched = len(eng.sched)
+    K = max(1, min(args.steps, n_sched))
+    W = max(0, min(args.warmup, n_sched))
+
+    # warmup, then back to the release state
+    eng.run(W)
+    torch.cuda.synchronize()
+    eng.reset()
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    sptr = int(stream.cuda_stream)
+    ev = eng.time_step_kernel(K, stream)  # HIP events around each step-kernel launch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        eng.run_step(sptr)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    esc = eng.escaped.cpu().numpy()
+    active = []
+    cum_esc = 0
+    for i in range(K):
+        active.append(eng.active_count(eng.sched[i]) - cum_esc)
+        cum_esc += int(esc[i])
+    psteps = int(sum(active))
+    step_ms = np.array([a.elapsed_time(b) for a, b in ev])
+    act = np.array(active, dtype=np.float64)
+    busy = act > 0
+    avg_ms = float(step_ms[busy].mean()) if busy.any() else float(step_ms.mean())
+    bytes_launch = BYTES_PER_PARTICLE_STEP * float(act[busy].mean()) if busy.any() else 0.0
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 5),
+        "traffic": None,
+        "kernel": "glide::step_kernel<false>" if met_dtype == L.GLIDE_MET_F32 else "glide::step_kernel<true>",
+        "kernel_avg_us": round(avg_ms * 1e3, 3),
+        "bytes_per_particle_step": BYTES_PER_PARTICLE_STEP,
+    }
+    traffic_file = ROOT / "profiles" / f"traffic_{WORKLOAD}_{args.met}.json"
+    if traffic_file.exists():
+        try:
+            roofline["traffic"] = json.loads(traffic_file.read_text()).get("bytes_per_launch")
+        except (OSError, ValueError):
+            roofline["traffic"] = None
+    out = {
+        "metric": METRIC,
+        "value": psteps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": elapsed * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if met_dtype == L.GLIDE_MET_F32 else "bf16-met/f32",
+        "data": "synthetic (seeded met cube and release; no network)",
+        "config": {
+            "workload": WORKLOAD,
+            "particles": eng.n,
+            "dt_s": int(cfg["dt"]),
+            "steps_in_run": n_sched,
+            "grid": f"{cfg['spec'].nx}x{cfg['spec'].ny}x{cfg['spec'].za}",
+            "met_hours": len(eng.met.src),
+            "met_storage": args.met,
+            "convection": "per-column emanuel",
+            "meander": True,
+            "max_substeps": eng.h.max_substeps,
+            "footprint": f"1x{cfg['grid']['n_t']}x1x{cfg['grid']['n_y']}x{cfg['grid']['n_x']} f32",
+        },
+        "active_particle_steps": psteps,
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_particles, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,159 @@
+"""ctypes binding of ``libglide_hip.so`` (the C ABI in ``include/glide.h``).
+
+The library is mandatory: :func:`lib` raises ``RuntimeError`` when it is not
+built or cannot be loaded. There is no fallback path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libglide_hip.so"
+
+GLIDE_OK = 0
+GLIDE_ERR_INVALID = -1
+GLIDE_ERR_MEMORY = -2
+GLIDE_ERR_DEVICE = -3
+
+GLIDE_MET_F32 = 0
+GLIDE_MET_BF16 = 1
+
+F_MEANDER = 0x1
+F_TL_FLOORS = 0x2
+F_SL_OVERRIDE = 0x4
+F_EMIT = 0x8
+F_MASK_INPUT = 0x10
+F_NO_TURB = 0x20
+
+vp = C.c_void_p
+
+
+class GlideGrid(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("reserved", C.c_int32),
+        ("lon0", C.c_double), ("lon1", C.c_double), ("lat0", C.c_double), ("lat1", C.c_double),
+        ("levels", vp),
+    ]
+
+
+class GlideWindow(C.Structure):
+    _fields_ = [("wind", vp), ("aux", vp), ("surf", vp), ("dtype", C.c_int32), ("reserved", C.c_int32)]
+
+
+class GlideParticles(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64), ("pos_stride", C.c_int64),
+        ("lon", vp), ("lat", vp), ("z", vp),
+        ("up", vp), ("vp", vp), ("wp", vp), ("um", vp), ("vm", vp),
+        ("weight", vp), ("r_off", vp), ("we_off", vp), ("rel_idx", vp), ("uid", vp),
+        ("alive", vp), ("active", vp),
+    ]
+
+
+class GlideFootprintGrid(C.Structure):
+    _fields_ = [
+        ("lon_min", C.c_double), ("lon_max", C.c_double), ("lat_min", C.c_double),
+        ("lat_max", C.c_double), ("z_edges", vp),
+        ("n_r", C.c_int32), ("n_t", C.c_int32), ("n_z", C.c_int32), ("n_y", C.c_int32),
+        ("n_x", C.c_int32), ("reserved", C.c_int32),
+        ("dom_lon_min", C.c_double), ("dom_lon_max", C.c_double), ("dom_lat_min", C.c_double),
+        ("dom_lat_max", C.c_double), ("dom_alt_max", C.c_double),
+    ]
+
+
+class GlideStepParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("alpha", C.c_double), ("cursor_off", C.c_double),
+        ("length_s", C.c_double), ("substep_c", C.c_double), ("z_ubl_m", C.c_double),
+        ("meander_timescale_s", C.c_double), ("seed", C.c_uint64),
+        ("step_index", C.c_int32), ("max_substeps", C.c_int32),
+        ("flags", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class GlideEmit(C.Structure):
+    _fields_ = [("keys", vp), ("vals", vp), ("escaped", vp)]
+
+
+P = C.POINTER
+
+# name -> (restype, argtypes); every symbol include/glide.h declares.
+SIGNATURES = {
+    "glide_abi_version": (C.c_int, []),
+    "glide_last_error": (C.c_char_p, []),
+    "glide_step": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), P(GlideWindow),
+                             P(GlideFootprintGrid), P(GlideEmit), vp]),
+    "glide_footprint_keys": (C.c_int, [C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp, vp, C.c_double,
+                                       P(GlideFootprintGrid), vp, vp, vp]),
+    "glide_footprint_workspace_bytes": (C.c_size_t, [C.c_int64]),
+    "glide_footprint_accumulate": (C.c_int, [vp, vp, C.c_int64, C.c_uint32, C.c_uint32, vp, vp, vp, vp,
+                                             C.c_size_t, vp]),
+    "glide_resample_hour": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp, C.c_int32,
+                                      vp, vp, C.c_double, C.c_double, vp, C.c_int32, C.c_int32, C.c_int32,
+                                      vp, vp, vp]),
+    "glide_build_window": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                     C.c_int32, C.c_int32, vp, vp, vp, vp]),
+    "glide_convect_columns": (C.c_int, [vp, vp, vp, C.c_double, vp, vp, C.c_int32, C.c_int32, C.c_int32,
+                                        C.c_double, C.c_double, C.c_double, C.c_double, vp, vp, vp, vp, vp]),
+    "glide_convect": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), vp, vp, vp, vp, vp,
+                                P(GlideFootprintGrid), P(GlideEmit), vp]),
+    "glide_graph_begin": (C.c_int, [vp]),
+    "glide_graph_end": (C.c_int, [vp, P(vp)]),
+    "glide_graph_launch": (C.c_int, [vp, vp]),
+    "glide_graph_destroy": (C.c_int, [vp]),
+}
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (once). Raises RuntimeError if it is missing."""
+
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = Path(os.environ.get("GLIDE_HIP_LIB", LIB_PATH))
+    if not path.exists():
+        raise RuntimeError(
+            f"{path} is not built; run `python -m glide_amd.build` (the HIP library is required)"
+        )
+    h = C.CDLL(str(path))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(h, name)
+        fn.restype = res
+        fn.argtypes = args
+    if h.glide_abi_version() != 1:
+        raise RuntimeError("libglide_hip.so ABI version mismatch")
+    _LIB = h
+    return h
+
+
+def check(rc: int) -> None:
+    """Map a glide status to the reference's exception types."""
+
+    if rc == GLIDE_OK:
+        return
+    msg = (lib().glide_last_error() or b"").decode(errors="replace")
+    if rc == GLIDE_ERR_INVALID:
+        raise ValueError(msg)
+    if rc == GLIDE_ERR_MEMORY:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None for None)."""
+
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_ptr(stream=None) -> int:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)

@@ -1,0 +1,349 @@
+// Per-column reduced-Emanuel convection, backward (transpose) sampling.
+//
+// The reference builds one non-divergent mass-flux matrix for the bbox-mean
+// column (emanuel.py:429-506) and samples each active particle's destination
+// from the renormalised transposed column of its host level
+// (emanuel.py:720-805). Here every met column carries its own cloud base/top
+// and cloud-base flux; the matrix is structured (rank-1 updraft + sub-diagonal
+// subsidence + diagonal closure), so each particle evaluates the entries of its
+// column on the fly in O(nz) instead of materialising Z x Z per column.
+#include "footprint_key.h"
+
+namespace glide {
+
+constexpr int kConvLevels = 128;
+
+struct ColumnArgs {
+    const float* conv;  // [4][ny][nx]: cape, lcl_h, top_h, dtv
+    const float* t_s;
+    const float* t_e;  // [nz][ny][nx]
+    float alpha;
+    const double* p_hpa;
+    const float* lev;
+    int nz, ny, nx;
+    float closure_c, trigger_dtv, min_cape, min_depth;
+    int32_t* i_lcl;
+    int32_t* i_lnb;
+    float* m_b;
+};
+
+// Column descriptors (emanuel.py:629-675 applied per column).
+__global__ void convect_columns_kernel(const ColumnArgs a) {
+    const int64_t ncol = static_cast<int64_t>(a.ny) * a.nx;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    const float cape = a.conv[c];
+    const float lcl_h = a.conv[ncol + c];
+    const float top_h = a.conv[2 * ncol + c];
+    const float dtv = a.conv[3 * ncol + c];
+    int lcl = -1, lnb = a.nz - 1;
+    for (int k = 0; k < a.nz; ++k) {
+        if (a.lev[k] >= lcl_h) {
+            lcl = k;
+            break;
+        }
+    }
+    for (int k = 0; k < a.nz; ++k) {
+        if (a.lev[k] >= top_h) {
+            lnb = k;
+            break;
+        }
+    }
+    bool fires = lcl >= 0 && lnb > lcl && cape >= a.min_cape && dtv >= a.trigger_dtv;
+    if (fires && (a.lev[lnb] - a.lev[lcl]) < a.min_depth) fires = false;
+    float mb = 0.0f;
+    if (fires) {
+        const float ts = a.t_s[static_cast<int64_t>(lcl) * ncol + c];
+        const float te = a.t_e[static_cast<int64_t>(lcl) * ncol + c];
+        const float t_lcl = ts * (1.0f - a.alpha) + te * a.alpha;
+        const float p_lcl = static_cast<float>(a.p_hpa[lcl]) * 100.0f;
+        const float rho = p_lcl / (287.05f * fmaxf(t_lcl, 150.0f));
+        const float wb = fminf(sqrtf(2.0f * fmaxf(cape, 0.0f)), 5.0f);
+        mb = a.closure_c * rho * wb * 3600.0f;
+    }
+    a.i_lcl[c] = fires ? lcl : -1;
+    a.i_lnb[c] = fires ? lnb : -1;
+    a.m_b[c] = mb;
+}
+
+// Layer masses |dp|/g with half layers at the ends (emanuel.py:263-279).
+__global__ void layer_mass_kernel(const double* p_hpa, int nz, float* lm) {
+    const int k = threadIdx.x;
+    if (k >= nz) return;
+    auto p = [&](int i) { return static_cast<float>(p_hpa[i]) * 100.0f; };
+    float dp;
+    if (k == 0) {
+        dp = fabsf(p(0) - p(1)) * 0.5f;
+    } else if (k == nz - 1) {
+        dp = fabsf(p(nz - 2) - p(nz - 1)) * 0.5f;
+    } else {
+        dp = 0.5f * fabsf(p(k - 1) - p(k + 1));
+    }
+    lm[k] = dp / 9.80665f;
+}
+
+// Structured column of one cloud: e (entrainment, BL), d (detrainment, cloud),
+// phi (subsidence), CFL scale.
+struct Cloud {
+    int lcl, lnb;
+    float mb, bl_total, w_total, scale;
+};
+
+__device__ __forceinline__ float cloud_w(const Cloud& c, int k) {
+    if (k < c.lcl || k > c.lnb) return 0.0f;
+    return 1.0f - static_cast<float>(k - c.lcl) / static_cast<float>(c.lnb - c.lcl);
+}
+__device__ __forceinline__ float cloud_e(const Cloud& c, const float* m, int i) {
+    return i < c.lcl ? c.mb * m[i] / c.bl_total : 0.0f;
+}
+__device__ __forceinline__ float cloud_d(const Cloud& c, int k) { return c.mb * cloud_w(c, k) / c.w_total; }
+
+__device__ Cloud make_cloud(int lcl, int lnb, float mb, const float* m, int nz) {
+    Cloud c{lcl, lnb, mb, 0.0f, 0.0f, 1.0f};
+    float bl = 0.0f;
+    for (int i = 0; i < lcl; ++i) bl += m[i];
+    c.bl_total = fmaxf(bl, 1e-12f);
+    float wt = 0.0f;
+    for (int k = lcl; k <= lnb; ++k) wt += cloud_w(c, k);
+    c.w_total = fmaxf(wt, 1e-12f);
+    float sum_d = 0.0f;
+    for (int k = 0; k < nz; ++k) sum_d += cloud_d(c, k);
+    // CFL cap on the per-layer outflux fraction (emanuel.py:500-504); row i of
+    // the unscaled matrix sums to e_i * sum(d) / mb + phi_{i-1}.
+    float ce = 0.0f, cd = 0.0f, phi_prev = 0.0f, mx = 0.0f;
+    for (int i = 0; i < nz; ++i) {
+        const float row = cloud_e(c, m, i) * sum_d / mb + phi_prev;
+        mx = fmaxf(mx, row / fmaxf(m[i], 1e-12f));
+        ce += cloud_e(c, m, i);
+        cd += cloud_d(c, i);
+        phi_prev = (i < nz - 1) ? fmaxf(ce - cd, 0.0f) : 0.0f;
+    }
+    if (mx > 0.9f) c.scale = 0.9f / mx;
+    return c;
+}
+
+struct ConvectArgs {
+    GlideStepParams p;
+    GlideParticles s;
+    float lon0, lat0, sx, sy;
+    int nx, ny, nz;
+    const float* levels;
+    const int32_t* i_lcl;
+    const int32_t* i_lnb;
+    const float* m_b;
+    const float* lm;
+    FpDev fp;
+    GlideEmit e;
+};
+
+__global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
+    __shared__ float lev[kConvLevels];
+    __shared__ float lm[kConvLevels];
+    for (int k = threadIdx.x; k < a.nz; k += blockDim.x) {
+        lev[k] = a.levels[k];
+        lm[k] = a.lm[k];
+    }
+    __syncthreads();
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.s.n) return;
+    const GlideParticles& s = a.s;
+    const GlideStepParams& p = a.p;
+    const bool emit = (p.flags & GLIDE_F_EMIT) != 0;
+    const float co = static_cast<float>(p.cursor_off);
+    bool active;
+    if (p.flags & GLIDE_F_MASK_INPUT) {
+        active = s.active[i] != 0;
+    } else {
+        const float ro = s.r_off[i];
+        active = (ro >= co) && (ro - static_cast<float>(p.length_s) <= co) && (s.alive[i] != 0);
+    }
+    if (!active) {
+        if (emit) {
+            a.e.keys[i] = a.fp.n_cells;
+            a.e.vals[i] = 0.0f;
+        }
+        return;
+    }
+    const int64_t ps = s.pos_stride;
+    const float lon = s.lon[i * ps];
+    const float lat = s.lat[i * ps];
+    float z = s.z[i * ps];
+    const int nz = a.nz;
+    // Nearest column.
+    const int cx = static_cast<int>(floorf(frac_index(lon, a.lon0, a.sx, a.nx) + 0.5f));
+    const int cy = static_cast<int>(floorf(frac_index(lat, a.lat0, a.sy, a.ny) + 0.5f));
+    const int64_t col = static_cast<int64_t>(min(cy, a.ny - 1)) * a.nx + min(cx, a.nx - 1);
+    const int lcl = a.i_lcl[col];
+    const int lnb = a.i_lnb[col];
+    const float mb = a.m_b[col];
+    if (lcl >= 0 && lnb > lcl && mb > 0.0f) {
+        // Host = nearest level (emanuel.py:765-771).
+        int cnt = 0;
+        for (int k = 0; k < nz; ++k) cnt += (lev[k] < z) ? 1 : 0;
+        const int up = min(max(cnt, 1), nz - 1);
+        const int lo = up - 1;
+        const int host = ((z - lev[lo]) < (lev[up] - z)) ? lo : up;
+        const Cloud c = make_cloud(lcl, lnb, mb, lm, nz);
+        // Column `host` of fmassfrac, over m[host] (emanuel.py:756-779).
+        float sum_d = 0.0f;
+        for (int k = 0; k < nz; ++k) sum_d += cloud_d(c, k);
+        const float d_h = cloud_d(c, host);
+        float ce = 0.0f, cd = 0.0f;
+        for (int k = 0; k <= host && k < nz; ++k) {
+            ce += cloud_e(c, lm, k);
+            cd += cloud_d(c, k);
+        }
+        const float phi_h = (host < nz - 1) ? fmaxf(ce - cd, 0.0f) : 0.0f;  // f[host+1][host]
+        float phi_hm1 = 0.0f;  // f[host][host-1]
+        if (host >= 1) {
+            const float ce1 = ce - cloud_e(c, lm, host), cd1 = cd - cloud_d(c, host);
+            phi_hm1 = fmaxf(ce1 - cd1, 0.0f);
+        }
+        const float row_h = c.scale * (cloud_e(c, lm, host) * sum_d / mb + phi_hm1);
+        const float lmh = fmaxf(lm[host], 1e-6f);
+        auto colv = [&](int j) {
+            float f = c.scale * (cloud_e(c, lm, j) * d_h / mb + ((j == host + 1) ? phi_h : 0.0f));
+            if (j == host) f += fmaxf(lmh - row_h, 0.0f);
+            return fmaxf(f / lmh, 0.0f);
+        };
+        float tot = 0.0f;
+        for (int j = 0; j < nz; ++j) tot += colv(j);
+        tot = fmaxf(tot, 1e-12f);
+        const uint32_t uid = s.uid ? s.uid[i] : static_cast<uint32_t>(i);
+        const U4 r = philox4x32_10(uid, static_cast<uint32_t>(p.step_index), kStreamConvection, 0u,
+                                   static_cast<uint32_t>(p.seed), static_cast<uint32_t>(p.seed >> 32));
+        const float u = u01(r.x);
+        float cum = 0.0f;
+        int dest = nz - 1;
+        for (int j = 0; j < nz; ++j) {
+            cum += colv(j) / tot;
+            if (cum >= u) {
+                dest = j;
+                break;
+            }
+        }
+        if (dest != host) {
+            const float zl = lev[dest > 0 ? dest - 1 : 0];
+            const float zh = lev[dest];
+            z = zl + u01(r.y) * (zh - zl);
+            s.z[i * ps] = z;
+        }
+    }
+    if (emit) {
+        const int64_t tb = time_bin(s.we_off[i], co, a.fp.n_t);
+        const int64_t rb = s.rel_idx ? s.rel_idx[i] : 0;
+        a.e.keys[i] = footprint_key(a.fp, true, lon, lat, z, tb, rb);
+        a.e.vals[i] = s.weight[i] * static_cast<float>(p.dt);
+        const bool esc = !within_domain(a.fp, lon, lat, z);
+        if (esc) s.alive[i] = 0;
+        count_escapes(esc, a.e.escaped);
+    }
+}
+
+// Mass conservation per column in float64: materialise f[i][j] and report
+// max_i |row_i - col_i| / m_i (non-divergence, emanuel.py:437-446).
+__global__ void colsum_check_kernel(const int32_t* i_lcl, const int32_t* i_lnb, const float* m_b,
+                                    const float* lm_g, int nz, int64_t ncol, double* err) {
+    const int64_t col = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    const int lcl = i_lcl[col], lnb = i_lnb[col];
+    const float mb = m_b[col];
+    double worst = 0.0;
+    if (lcl >= 0 && lnb > lcl && mb > 0.0f) {
+        float lm[kConvLevels];
+        for (int k = 0; k < nz; ++k) lm[k] = lm_g[k];
+        const Cloud c = make_cloud(lcl, lnb, mb, lm, nz);
+        float phi[kConvLevels];
+        float ce = 0.0f, cd = 0.0f;
+        for (int k = 0; k < nz; ++k) {
+            ce += cloud_e(c, lm, k);
+            cd += cloud_d(c, k);
+            phi[k] = (k < nz - 1) ? fmaxf(ce - cd, 0.0f) : 0.0f;
+        }
+        for (int i = 0; i < nz; ++i) {
+            double row = 0.0, cols = 0.0;
+            for (int j = 0; j < nz; ++j) {
+                const float fij = c.scale * (cloud_e(c, lm, i) * cloud_d(c, j) / mb + ((i == j + 1) ? phi[j] : 0.0f));
+                const float fji = c.scale * (cloud_e(c, lm, j) * cloud_d(c, i) / mb + ((j == i + 1) ? phi[i] : 0.0f));
+                row += static_cast<double>(fij);
+                cols += static_cast<double>(fji);
+            }
+            const double e = fabs(row - cols) / fmax(static_cast<double>(lm[i]), 1e-12);
+            worst = e > worst ? e : worst;
+        }
+    }
+    err[col] = worst;
+}
+
+}  // namespace glide
+
+using namespace glide;
+
+extern "C" int glide_convect_columns(const float* conv, const float* t_s, const float* t_e, double alpha,
+                                     const double* p_level_hpa, const float* levels, int32_t nz, int32_t ny,
+                                     int32_t nx, double closure_c, double trigger_dtv_k, double min_cape_j_kg,
+                                     double min_cloud_depth_m, int32_t* i_lcl, int32_t* i_lnb, float* m_b,
+                                     float* layer_mass, void* stream) {
+    GLIDE_REQUIRE(conv && t_s && t_e && p_level_hpa && levels && i_lcl && i_lnb && m_b && layer_mass,
+                  "glide_convect_columns: NULL argument");
+    GLIDE_REQUIRE(nz >= 2 && nz <= kConvLevels && ny >= 1 && nx >= 1, "bad convection dimensions");
+    GLIDE_REQUIRE(closure_c > 0, "closure_c must be > 0");
+    ColumnArgs a{conv, t_s, t_e, static_cast<float>(alpha), p_level_hpa, levels, nz, ny, nx,
+                 static_cast<float>(closure_c), static_cast<float>(trigger_dtv_k),
+                 static_cast<float>(min_cape_j_kg), static_cast<float>(min_cloud_depth_m), i_lcl, i_lnb, m_b};
+    const int64_t ncol = static_cast<int64_t>(ny) * nx;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(convect_columns_kernel, dim3(static_cast<unsigned>((ncol + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(layer_mass_kernel, dim3(1), dim3(kConvLevels), 0, st, p_level_hpa, nz, layer_mass);
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}
+
+extern "C" int glide_convect(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
+                             const int32_t* i_lcl, const int32_t* i_lnb, const float* m_b,
+                             const float* layer_mass, double* colsum_err, const GlideFootprintGrid* fg,
+                             const GlideEmit* e, void* stream) {
+    GLIDE_REQUIRE(p && s && g && i_lcl && i_lnb && m_b && layer_mass, "glide_convect: NULL argument");
+    GLIDE_REQUIRE(g->nz >= 2 && g->nz <= kConvLevels && g->nx >= 2 && g->ny >= 2, "bad grid");
+    GLIDE_REQUIRE(g->lon1 != g->lon0 && g->lat1 != g->lat0, "Grid dimension has 0 span (first == last)");
+    GLIDE_REQUIRE(s->pos_stride >= 1 && s->n >= 0, "bad particles");
+    if (p->flags & GLIDE_F_MASK_INPUT) {
+        GLIDE_REQUIRE(s->active, "active mask is NULL");
+    } else {
+        GLIDE_REQUIRE(s->r_off && s->alive, "r_off/alive are NULL (no explicit mask)");
+    }
+    ConvectArgs a{};
+    a.p = *p;
+    a.s = *s;
+    a.nx = g->nx;
+    a.ny = g->ny;
+    a.nz = g->nz;
+    a.lon0 = static_cast<float>(g->lon0);
+    a.lat0 = static_cast<float>(g->lat0);
+    a.sx = static_cast<float>((g->nx - 1) / (g->lon1 - g->lon0));
+    a.sy = static_cast<float>((g->ny - 1) / (g->lat1 - g->lat0));
+    a.levels = g->levels;
+    a.i_lcl = i_lcl;
+    a.i_lnb = i_lnb;
+    a.m_b = m_b;
+    a.lm = layer_mass;
+    if (p->flags & GLIDE_F_EMIT) {
+        GLIDE_REQUIRE(fg && e && e->keys && e->vals && e->escaped, "emit outputs are NULL");
+        GLIDE_REQUIRE(s->we_off && s->weight && s->alive, "we_off/weight/alive needed to emit");
+        const uint64_t cells = static_cast<uint64_t>(fg->n_r) * fg->n_t * fg->n_z * fg->n_y * fg->n_x;
+        GLIDE_REQUIRE(cells > 0 && cells < 0xffffffffull, "bad footprint size");
+        a.fp = make_fp(*fg);
+        a.e = *e;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (s->n > 0) {
+        hipLaunchKernelGGL(convect_kernel, dim3(static_cast<unsigned>((s->n + 255) / 256)), dim3(256), 0, st, a);
+    }
+    if (colsum_err != nullptr) {
+        const int64_t ncol = static_cast<int64_t>(g->ny) * g->nx;
+        hipLaunchKernelGGL(colsum_check_kernel, dim3(static_cast<unsigned>((ncol + 127) / 128)), dim3(128), 0, st,
+                           i_lcl, i_lnb, m_b, layer_mass, g->nz, ncol, colsum_err);
+    }
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}

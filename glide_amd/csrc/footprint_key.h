@@ -1,0 +1,93 @@
+// Footprint cell key and the escape predicate, bit-exact with the reference's
+// float32 index math (footprint_gridder.py:156-196, main.py:690-711).
+#pragma once
+
+#include "common.h"
+
+namespace glide {
+
+struct FpDev {
+    float lon_min, lat_min, span_x, span_y;  // float32 values, as torch sees them
+    float fnx, fny;
+    int n_r, n_t, n_z, n_y, n_x;
+    uint32_t n_cells;
+    const float* z_edges;
+    float dom_lon_min, dom_lon_max, dom_lat_min, dom_lat_max, dom_alt_max;
+};
+
+inline FpDev make_fp(const GlideFootprintGrid& g) {
+    FpDev d;
+    d.lon_min = static_cast<float>(g.lon_min);
+    d.lat_min = static_cast<float>(g.lat_min);
+    const double sx = g.lon_max - g.lon_min;
+    const double sy = g.lat_max - g.lat_min;
+    d.span_x = static_cast<float>(sx > 1e-6 ? sx : 1e-6);
+    d.span_y = static_cast<float>(sy > 1e-6 ? sy : 1e-6);
+    d.fnx = static_cast<float>(g.n_x);
+    d.fny = static_cast<float>(g.n_y);
+    d.n_r = g.n_r;
+    d.n_t = g.n_t;
+    d.n_z = g.n_z;
+    d.n_y = g.n_y;
+    d.n_x = g.n_x;
+    d.n_cells = static_cast<uint32_t>(static_cast<uint64_t>(g.n_r) * g.n_t * g.n_z * g.n_y * g.n_x);
+    d.z_edges = g.z_edges;
+    d.dom_lon_min = static_cast<float>(g.dom_lon_min);
+    d.dom_lon_max = static_cast<float>(g.dom_lon_max);
+    d.dom_lat_min = static_cast<float>(g.dom_lat_min);
+    d.dom_lat_max = static_cast<float>(g.dom_lat_max);
+    d.dom_alt_max = static_cast<float>(g.dom_alt_max);
+    return d;
+}
+
+// floor(((c - c_min) / span) * n) as an index, or -1 when outside [0, n).
+__device__ __forceinline__ int bin_index(float c, float c_min, float span, float fn, int n) {
+#pragma clang fp contract(off)
+    const float f = floorf(((c - c_min) / span) * fn);
+    if (!(f >= 0.0f) || f >= static_cast<float>(n)) return -1;
+    return static_cast<int>(f);
+}
+
+// Key of one particle: flat cell index, or n_cells when the particle is
+// inactive or outside any dimension (the reference's w_eff = 0 case).
+__device__ __forceinline__ uint32_t footprint_key(const FpDev& g, bool active, float lon, float lat,
+                                                  float z, int64_t t_idx, int64_t r_idx) {
+    if (!active) return g.n_cells;
+    if (r_idx < 0 || r_idx >= g.n_r || t_idx < 0 || t_idx >= g.n_t) return g.n_cells;
+    const int xi = bin_index(lon, g.lon_min, g.span_x, g.fnx, g.n_x);
+    const int yi = bin_index(lat, g.lat_min, g.span_y, g.fny, g.n_y);
+    if (xi < 0 || yi < 0) return g.n_cells;
+    // bucketize(z, edges, right=False) - 1: bins (e_b, e_{b+1}].
+    int cnt = 0;
+    for (int k = 0; k <= g.n_z; ++k) cnt += (g.z_edges[k] < z) ? 1 : 0;
+    const int zi = cnt - 1;
+    if (zi < 0 || zi >= g.n_z) return g.n_cells;
+    const uint64_t key =
+        ((((static_cast<uint64_t>(r_idx) * g.n_t + t_idx) * g.n_z + zi) * g.n_y + yi) * g.n_x + xi);
+    return static_cast<uint32_t>(key);
+}
+
+// Time-ago bin (main.py:1459-1465) from the pre-step cursor.
+__device__ __forceinline__ int64_t time_bin(float we_off, float cursor_off, int n_t) {
+#pragma clang fp contract(off)
+    const float el = fmaxf(we_off - cursor_off, 0.0f);
+    int64_t t = static_cast<int64_t>(floorf(el / 3600.0f));
+    if (n_t == 1 && t > 0) t = 0;
+    return t;
+}
+
+__device__ __forceinline__ bool within_domain(const FpDev& g, float lon, float lat, float z) {
+    return lon >= g.dom_lon_min && lon <= g.dom_lon_max && lat >= g.dom_lat_min &&
+           lat <= g.dom_lat_max && z <= g.dom_alt_max;
+}
+
+// Wave-aggregated add of per-lane escape flags into *ctr.
+__device__ __forceinline__ void count_escapes(bool esc, unsigned long long* ctr) {
+    const unsigned long long m = __ballot(esc);
+    if (m == 0ull) return;
+    const unsigned long long act = __ballot(1);
+    const int leader = __ffsll(static_cast<long long>(act)) - 1;
+    if (static_cast<int>(__lane_id()) == leader) atomicAdd(ctr, static_cast<unsigned long long>(__popcll(m)));
+}
+
+}  // namespace glide

@@ -1,0 +1,388 @@
+// Fused per-particle backward step (one thread per particle).
+//
+// Restates the reference's static step core in one pass over the particle:
+// RK2 advection through the time-interpolated trilinear winds
+// (hanna.py:855-896), surface bilinear samples and stability (hanna.py:955-967),
+// sigma/T_L at z_eval and the +-1 m probes (hanna.py:983-991, 1500-1596), the
+// density drift (hanna.py:993-1006), the substepped exact-OU update with
+// clipping, displacement and smooth-wall reflection (hanna.py:1224-1368),
+// meander (hanna.py:1028-1054) and the active-mask gate (hanna.py:1056-1068).
+// With GLIDE_F_EMIT the same pass writes the footprint key/value of the new
+// position and applies the escape kill (main.py:1459-1477).
+//
+// Inactive particles are not touched at all (equivalent to the reference's
+// torch.where gate). Each active particle runs only its own k substeps: with
+// the counter-based noise a sub_dt = 0 substep is the identity, so this equals
+// the fixed max_substeps loop (hanna.py:1245-1254).
+#include "footprint_key.h"
+#include "physics.h"
+
+namespace glide {
+
+struct StepArgs {
+    GlideStepParams p;
+    GlideParticles s;
+    float lon0, lat0, sx, sy;  // met grid: ix = (lon - lon0) * sx
+    float blon0, blat0, bsx, bsy;  // surface bilinear mapping (same grid)
+    int nx, ny, nz;
+    const float* levels;
+    const void* wind;
+    const void* aux;
+    const void* surf;
+    FpDev fp;
+    GlideEmit e;
+};
+
+constexpr int kMaxLevels = 128;
+constexpr int kStepBlock = 256;
+
+template <bool BF16>
+__global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
+    __shared__ float lev[kMaxLevels];
+    for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
+    __syncthreads();
+
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.s.n) return;
+    const GlideParticles& s = a.s;
+    const GlideStepParams& p = a.p;
+    const bool emit = (p.flags & GLIDE_F_EMIT) != 0;
+    const float co = static_cast<float>(p.cursor_off);
+
+    bool active;
+    if (p.flags & GLIDE_F_MASK_INPUT) {
+        active = s.active[i] != 0;
+    } else {
+        const float ro = s.r_off[i];
+        active = (ro >= co) && (ro - static_cast<float>(p.length_s) <= co) && (s.alive[i] != 0);
+    }
+    if (!active) {
+        if (emit) {
+            a.e.keys[i] = a.fp.n_cells;
+            a.e.vals[i] = 0.0f;
+        }
+        return;
+    }
+
+    const int64_t ps = s.pos_stride;
+    float lon = s.lon[i * ps];
+    float lat = s.lat[i * ps];
+    float z = s.z[i * ps];
+    const float dt = static_cast<float>(p.dt);
+    const float al = static_cast<float>(p.alpha);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+
+    // ---- RK2 advection (backward) ----
+    auto wind_at = [&](float lo, float la, float zz, float& dlo, float& dla, float& dzz) {
+        const float ix = frac_index(lo, a.lon0, a.sx, nx);
+        const float iy = frac_index(la, a.lat0, a.sy, ny);
+        const float iz = level_index(lev, nz, zz);
+        const Corner3 c = corners3(ix, iy, iz, nx, ny, nz);
+        float v[6];
+        sample_rec<BF16, 6>(a.wind, c, v);
+        const float u = v[0] * (1.0f - al) + v[3] * al;
+        const float vv = v[1] * (1.0f - al) + v[4] * al;
+        const float w = v[2] * (1.0f - al) + v[5] * al;
+        dlo = u / m_per_deg_lon(la);
+        dla = vv / kMPerDegLat;
+        dzz = w;
+    };
+    {
+        float d0, d1, d2;
+        wind_at(lon, lat, z, d0, d1, d2);
+        const float h = static_cast<float>(0.5 * p.dt);
+        float e0, e1, e2;
+        wind_at(lon - h * d0, lat - h * d1, z - h * d2, e0, e1, e2);
+        lon = lon - dt * e0;
+        lat = lat - dt * e1;
+        z = z - dt * e2;
+    }
+
+    float up = s.up[i], vp = s.vp[i], wp = s.wp[i];
+    const bool meander = (p.flags & GLIDE_F_MEANDER) != 0;
+    float um = meander ? s.um[i] : 0.0f, vm = meander ? s.vm[i] : 0.0f;
+
+    if (!(p.flags & GLIDE_F_NO_TURB)) {
+        // ---- surface fields at the post-advection position ----
+        Surface sf;
+        float sp, shf, tsf;
+        {
+            const float ix = frac_index(lon, a.blon0, a.bsx, nx);
+            const float iy = frac_index(lat, a.blat0, a.bsy, ny);
+            const float fx0 = floorf(ix), fy0 = floorf(iy);
+            const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
+            const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1);
+            const float fx = ix - fx0, fy = iy - fy0;
+            const float w00 = (1.0f - fx) * (1.0f - fy), w01 = fx * (1.0f - fy);
+            const float w10 = (1.0f - fx) * fy, w11 = fx * fy;
+            float acc[5] = {0, 0, 0, 0, 0};
+            const int64_t cells[4] = {static_cast<int64_t>(y0) * nx + x0, static_cast<int64_t>(y0) * nx + x1,
+                                      static_cast<int64_t>(y1) * nx + x0, static_cast<int64_t>(y1) * nx + x1};
+            const float ws[4] = {w00, w01, w10, w11};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float r[10];
+                load_surf<BF16>(a.surf, cells[q], r);
+#pragma unroll
+                for (int ch = 0; ch < 5; ++ch) acc[ch] += ws[q] * torch_lerp(r[ch], r[5 + ch], al);
+            }
+            sf.blh = fmaxf(acc[0], kBlhMin);
+            sp = acc[1];
+            sf.ustar = fmaxf(acc[2], kUstarMin);
+            shf = acc[3];
+            tsf = acc[4];
+        }
+        float L, wstar;
+        stability(sf.blh, sp, sf.ustar, shf, tsf, L, wstar);
+        sf.L = L;
+        sf.wstar = wstar;
+        sf.hol = sf.blh / L;
+        sf.fcor = 2.0f * kOmega * sinf(lat * kDeg2Rad);
+
+        const bool tl_floors = (p.flags & GLIDE_F_TL_FLOORS) != 0;
+        const bool sl_over = (p.flags & GLIDE_F_SL_OVERRIDE) != 0;
+        const float zu = static_cast<float>(p.z_ubl_m);
+        const float z_eval = fmaxf(z, zu);
+        const float ix = frac_index(lon, a.lon0, a.sx, nx);
+        const float iy = frac_index(lat, a.lat0, a.sy, ny);
+
+        // aux record at z_eval: FT pair, density pair, meander pair.
+        float ax[6];
+        {
+            const Corner3 c = corners3(ix, iy, level_index(lev, nz, z_eval), nx, ny, nz);
+            sample_rec<BF16, 6>(a.aux, c, ax);
+        }
+
+        // Column turbulence at z_eval (hanna.py:1500-1596).
+        SigTL t = in_bl(z_eval, sf);
+        if (sl_over && z_eval < 0.1f * sf.blh) {
+            const float z_sl = 0.1f * sf.blh;
+            const SigTL tc = in_bl(fmaxf(z_eval, z_sl), sf);
+            t.su = tc.su;
+            t.sv = tc.sv;
+            t.sw = surface_layer_sw(z_eval, sf.ustar, sf.L);
+            const float zt = fmaxf(z_eval, kZMin);
+            t.tu = kKarman * zt / fmaxf(t.su, kSigmaMin);
+            t.tv = kKarman * zt / fmaxf(t.sv, kSigmaMin);
+            t.tw = kKarman * zt / fmaxf(t.sw, kSigmaMin);
+        }
+        if (tl_floors) {
+            t.tu = fmaxf(t.tu, 10.0f);
+            t.tv = fmaxf(t.tv, 10.0f);
+            t.tw = fmaxf(t.tw, 30.0f);
+        }
+        if (z_eval > sf.blh) {
+            t.su = t.sv = t.sw = ax[0];
+            t.tu = t.tv = t.tw = ax[1];
+        }
+        t.su = fmaxf(t.su, kSigmaMin);
+        t.sv = fmaxf(t.sv, kSigmaMin);
+        t.sw = fmaxf(t.sw, kSigmaMin);
+        t.tu = fmaxf(t.tu, kTLMin);
+        t.tv = fmaxf(t.tv, kTLMin);
+        t.tw = fmaxf(t.tw, kTLMin);
+
+        // sigma_w at the +-1 m probes.
+        auto sw_probe = [&](float zq) {
+            float sw = in_bl_sw(zq, sf);
+            if (sl_over && zq < 0.1f * sf.blh) sw = surface_layer_sw(zq, sf.ustar, sf.L);
+            if (zq > sf.blh) {
+                const Corner3 c = corners3(ix, iy, level_index(lev, nz, zq), nx, ny, nz);
+                float f0, f1;
+                sample_rec2<BF16>(a.aux, c, f0, f1);
+                sw = f0;
+            }
+            return fmaxf(sw, kSigmaMin);
+        };
+        const float z_hi = z_eval + 1.0f;
+        const float z_lo = fmaxf(z_eval - 1.0f, zu);
+        const float sw_hi = sw_probe(z_hi);
+        const float sw_lo = sw_probe(z_lo);
+        const float dsig2 = (sw_hi * sw_hi - sw_lo * sw_lo) / fmaxf(z_hi - z_lo, 1.0f);
+        const float sw_sq = fmaxf(t.sw * t.sw, kSigmaMin * kSigmaMin);
+
+        const float rho = fmaxf(ax[2], 1e-3f);
+        const float dens_b = -(sw_sq * ax[3] / rho);
+        const float half_b = -0.5f * dsig2;
+
+        // Substep schedule (hanna.py:1162-1186).
+        const float ratio = dt / (static_cast<float>(p.substep_c) * fmaxf(t.tw, 1e-3f));
+        int k = p.max_substeps;
+        if (ratio < static_cast<float>(p.max_substeps)) k = max(static_cast<int>(ceilf(ratio)), 1);
+        const float sdt = dt / static_cast<float>(k);
+
+        const float su2 = t.su * t.su, sv2 = t.sv * t.sv, sw2 = t.sw * t.sw;
+        const float umax = 4.0f * t.su, vmax = 4.0f * t.sv, wmax = 4.0f * t.sw;
+        const uint32_t uid = s.uid ? s.uid[i] : static_cast<uint32_t>(i);
+        const uint32_t k0 = static_cast<uint32_t>(p.seed), k1 = static_cast<uint32_t>(p.seed >> 32);
+        const uint32_t step = static_cast<uint32_t>(p.step_index);
+        for (int q = 0; q < k; ++q) {
+            const U4 r = philox4x32_10(uid, step, static_cast<uint32_t>(q), 0u, k0, k1);
+            float n0, n1, n2, n3;
+            box_muller(u01(r.x), u01(r.y), n0, n1);
+            box_muller(u01(r.z), u01(r.w), n2, n3);
+            (void)n3;
+            const float factor = 1.0f + wp * wp / sw_sq;
+            const float drift = factor * half_b + dens_b;
+            float un = ou(up, t.tu, su2, sdt, n0, 0.0f);
+            float vn = ou(vp, t.tv, sv2, sdt, n1, 0.0f);
+            float wn = ou(wp, t.tw, sw2, sdt, n2, drift * sdt);
+            wn = fminf(fmaxf(wn, -wmax), wmax);
+            un = fminf(fmaxf(un, -umax), umax);
+            vn = fminf(fmaxf(vn, -vmax), vmax);
+            z = z + (-wn) * sdt;
+            const float ml = m_per_deg_lon(lat);
+            lon = lon + (-un) * sdt / ml;
+            lat = lat + (-vn) * sdt / kMPerDegLat;
+            if (z < 0.0f) {
+                z = 0.0f - z;
+                wn = -wn;
+            }
+            up = un;
+            vp = vn;
+            wp = wn;
+        }
+
+        if (meander) {
+            const U4 r = philox4x32_10(uid, step, kStreamMeander, 0u, k0, k1);
+            float n0, n1;
+            box_muller(u01(r.x), u01(r.y), n0, n1);
+            const float tm = static_cast<float>(p.meander_timescale_s);
+            um = ou(um, tm, ax[4] * ax[4], dt, n0, 0.0f);
+            vm = ou(vm, tm, ax[5] * ax[5], dt, n1, 0.0f);
+            const float ml = m_per_deg_lon(lat);
+            lon = lon + (-um) * dt / ml;
+            lat = lat + (-vm) * dt / kMPerDegLat;
+        }
+    }
+
+    s.lon[i * ps] = lon;
+    s.lat[i * ps] = lat;
+    s.z[i * ps] = z;
+    s.up[i] = up;
+    s.vp[i] = vp;
+    s.wp[i] = wp;
+    if (meander) {
+        s.um[i] = um;
+        s.vm[i] = vm;
+    }
+
+    if (emit) {
+        const int64_t tb = time_bin(s.we_off[i], co, a.fp.n_t);
+        const int64_t rb = s.rel_idx ? s.rel_idx[i] : 0;
+        a.e.keys[i] = footprint_key(a.fp, true, lon, lat, z, tb, rb);
+        a.e.vals[i] = s.weight[i] * dt;
+        const bool esc = !within_domain(a.fp, lon, lat, z);
+        if (esc) s.alive[i] = 0;
+        count_escapes(esc, a.e.escaped);
+    }
+}
+
+__global__ void footprint_key_kernel(int64_t n, int64_t ps, const float* lon, const float* lat,
+                                     const float* z, const uint8_t* active, const float* weight,
+                                     const int64_t* t_idx, const int64_t* rel_idx, float dt,
+                                     FpDev g, uint32_t* keys, float* vals) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool act = active[i] != 0;
+    const uint32_t k = footprint_key(g, act, lon[i * ps], lat[i * ps], z[i * ps], t_idx[i], rel_idx[i]);
+    keys[i] = k;
+    vals[i] = (k == g.n_cells) ? 0.0f : weight[i] * dt;
+}
+
+static int fill_grid(StepArgs& a, const GlideGrid& g, const GlideWindow& w) {
+    GLIDE_REQUIRE(g.nx >= 2 && g.ny >= 2 && g.nz >= 2, "met grid needs >= 2 points per axis");
+    GLIDE_REQUIRE(g.nz <= kMaxLevels, "at most %d met levels supported (got %d)", kMaxLevels, g.nz);
+    GLIDE_REQUIRE(g.lon1 != g.lon0 && g.lat1 != g.lat0, "Grid dimension has 0 span (first == last)");
+    GLIDE_REQUIRE(g.levels != nullptr, "levels pointer is NULL");
+    GLIDE_REQUIRE(w.wind && w.aux && w.surf, "met window pointers are NULL");
+    a.nx = g.nx;
+    a.ny = g.ny;
+    a.nz = g.nz;
+    a.lon0 = static_cast<float>(g.lon0);
+    a.lat0 = static_cast<float>(g.lat0);
+    a.sx = static_cast<float>((g.nx - 1) / (g.lon1 - g.lon0));
+    a.sy = static_cast<float>((g.ny - 1) / (g.lat1 - g.lat0));
+    a.blon0 = a.lon0;
+    a.blat0 = a.lat0;
+    a.bsx = a.sx;
+    a.bsy = a.sy;
+    a.levels = g.levels;
+    a.wind = w.wind;
+    a.aux = w.aux;
+    a.surf = w.surf;
+    return GLIDE_OK;
+}
+
+}  // namespace glide
+
+using namespace glide;
+
+extern "C" int glide_step(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
+                          const GlideWindow* w, const GlideFootprintGrid* fg, const GlideEmit* e,
+                          void* stream) {
+    GLIDE_REQUIRE(p && s && g && w, "glide_step: NULL argument");
+    GLIDE_REQUIRE(s->n >= 0, "n must be >= 0");
+    GLIDE_REQUIRE(s->lon && s->lat && s->z && s->up && s->vp && s->wp, "particle arrays are NULL");
+    GLIDE_REQUIRE(s->pos_stride >= 1, "pos_stride must be >= 1");
+    GLIDE_REQUIRE(p->max_substeps >= 1, "max_substeps must be >= 1");
+    GLIDE_REQUIRE(p->substep_c > 0, "substep_c must be > 0");
+    GLIDE_REQUIRE(p->dt >= 0, "dt must be >= 0");
+    GLIDE_REQUIRE(w->dtype == GLIDE_MET_F32 || w->dtype == GLIDE_MET_BF16, "unknown met dtype");
+    if (p->flags & GLIDE_F_MEANDER) GLIDE_REQUIRE(s->um && s->vm, "meander state arrays are NULL");
+    if (p->flags & GLIDE_F_MASK_INPUT) {
+        GLIDE_REQUIRE(s->active, "active mask is NULL");
+    } else {
+        GLIDE_REQUIRE(s->r_off && s->alive, "r_off/alive are NULL (no explicit mask)");
+    }
+    StepArgs a{};
+    a.p = *p;
+    a.s = *s;
+    int rc = fill_grid(a, *g, *w);
+    if (rc) return rc;
+    if (p->flags & GLIDE_F_EMIT) {
+        GLIDE_REQUIRE(fg && e && e->keys && e->vals && e->escaped, "emit outputs are NULL");
+        GLIDE_REQUIRE(s->we_off && s->weight && s->alive, "we_off/weight/alive needed to emit");
+        GLIDE_REQUIRE(fg->n_r >= 1 && fg->n_t >= 1 && fg->n_z >= 1 && fg->n_y >= 1 && fg->n_x >= 1,
+                      "All footprint dimensions must be > 0");
+        const uint64_t cells = static_cast<uint64_t>(fg->n_r) * fg->n_t * fg->n_z * fg->n_y * fg->n_x;
+        GLIDE_REQUIRE(cells < 0xffffffffull, "footprint has too many cells for 32-bit keys");
+        GLIDE_REQUIRE(fg->n_z <= 64, "at most 64 footprint z bins");
+        a.fp = make_fp(*fg);
+        a.e = *e;
+    }
+    if (s->n == 0) return GLIDE_OK;
+    const int64_t blocks = (s->n + kStepBlock - 1) / kStepBlock;
+    GLIDE_REQUIRE(blocks < (1ll << 31), "too many particles");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (w->dtype == GLIDE_MET_BF16) {
+        hipLaunchKernelGGL(step_kernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(kStepBlock), 0, st, a);
+    } else {
+        hipLaunchKernelGGL(step_kernel<false>, dim3(static_cast<unsigned>(blocks)), dim3(kStepBlock), 0, st, a);
+    }
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}
+
+extern "C" int glide_footprint_keys(int64_t n, int64_t pos_stride, const float* lon, const float* lat,
+                                    const float* z, const uint8_t* active, const float* weight,
+                                    const int64_t* t_idx, const int64_t* rel_idx, double dt,
+                                    const GlideFootprintGrid* fg, uint32_t* keys, float* vals,
+                                    void* stream) {
+    GLIDE_REQUIRE(fg, "NULL footprint grid");
+    GLIDE_REQUIRE(n >= 0 && pos_stride >= 1, "bad n / pos_stride");
+    GLIDE_REQUIRE(fg->n_r >= 1 && fg->n_t >= 1 && fg->n_z >= 1 && fg->n_y >= 1 && fg->n_x >= 1,
+                  "All footprint dimensions must be > 0");
+    const uint64_t cells = static_cast<uint64_t>(fg->n_r) * fg->n_t * fg->n_z * fg->n_y * fg->n_x;
+    GLIDE_REQUIRE(cells < 0xffffffffull, "footprint has too many cells for 32-bit keys");
+    GLIDE_REQUIRE(fg->n_z <= 64, "at most 64 footprint z bins");
+    if (n == 0) return GLIDE_OK;
+    GLIDE_REQUIRE(lon && lat && z && active && weight && t_idx && rel_idx && keys && vals && fg->z_edges,
+                  "NULL array");
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(footprint_key_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), n, pos_stride, lon, lat, z, active, weight,
+                       t_idx, rel_idx, static_cast<float>(dt), make_fp(*fg), keys, vals);
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}

@@ -1,0 +1,328 @@
+"""Device-resident driver for one batch of releases.
+
+Mirrors the reference's static per-step body (main.py:1383-1477, S2 order)
+with everything on the GPU: the met source hours stay in HBM, the hourly
+resample and window build run as column kernels on a bracket change, and each
+step is one fused step kernel (which also emits the footprint keys and applies
+the escape kill) followed by the deterministic footprint accumulation.
+Convection runs once per bracket, between the step and the gridding, as in the
+reference; on that step the convection kernel emits the keys instead.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from glide_amd import _lib as L
+from glide_amd.schedule import cursor_schedule, release_arrays
+from glide_amd.synthetic import SyntheticMet
+
+
+@dataclass
+class HannaOptions:
+    """HannaScheme constructor options (reference hanna.py:440-520); the
+    shipped configs use max_substeps=6 and meander on."""
+
+    meander_enabled: bool = True
+    meander_coefficient: float = 0.16
+    meander_stencil_radius: int = 1
+    meander_timescale_s: float = 1800.0
+    substep_c: float = 0.5
+    max_substeps: int = 6
+    z_ubl_m: float = 2.0
+    flexpart_tl_floors: bool = True
+    surface_layer_override: bool = False
+
+    def flags(self) -> int:
+        f = 0
+        if self.meander_enabled:
+            f |= L.F_MEANDER
+        if self.flexpart_tl_floors:
+            f |= L.F_TL_FLOORS
+        if self.surface_layer_override:
+            f |= L.F_SL_OVERRIDE
+        return f
+
+
+@dataclass
+class ConvectionOptions:
+    """EmanuelReducedConvection constructor options (emanuel.py:526-559)."""
+
+    closure_c: float = 0.03
+    trigger_dtv_k: float = 0.9
+    min_cape_j_kg: float = 50.0
+    min_cloud_depth_m: float = 500.0
+
+
+def _dev(a, dtype=None, device="cuda"):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(device)
+
+
+class DeviceMet:
+    """Met source hours resident in HBM, the resampled ring and one packed
+    window, for a :class:`SyntheticMet` grid."""
+
+    def __init__(self, met: SyntheticMet, hours, *, device="cuda", met_dtype: int = L.GLIDE_MET_F32,
+                 meander_coeff: float = 0.16, meander_radius: int = 1):
+        self.met = met
+        spec = met.spec
+        self.device = device
+        self.nx, self.ny, self.nz = spec.nx, spec.ny, spec.za
+        self.met_dtype = met_dtype
+        self.meander_coeff = meander_coeff
+        self.meander_radius = meander_radius
+        lev = spec.agl_levels
+        self.levels_np = lev
+        self.levels_d = _dev(lev, torch.float64, device)
+        self.levels_f = _dev(lev.astype(np.float32), None, device)
+        self.lon = met.lon
+        self.lat = met.lat
+        self.terrain = _dev(met.terrain, None, device)
+        mlon = np.clip(111320.0 * np.cos(np.deg2rad(self.lat)), 1.0, None)
+        self.mlon = _dev(mlon, torch.float64, device)
+        self.src = {}
+        for h in hours:
+            sh = met.hour(h)
+            self.src[h] = dict(
+                src3=_dev(sh.src3, None, device), src2=_dev(sh.src2, None, device),
+                height=_dev(sh.height_agl, None, device),
+                p_hpa=_dev(sh.p_level_pa / 100.0, torch.float64, device),
+                conv=_dev(sh.conv, None, device),
+            )
+        self.zp = next(iter(self.src.values()))["src3"].shape[1]
+        ncell = self.nz * self.ny * self.nx
+        esize = 2 if met_dtype == L.GLIDE_MET_BF16 else 4
+        self.wind = torch.empty(ncell * 8 * esize, dtype=torch.uint8, device=device)
+        self.aux = torch.empty(ncell * 8 * esize, dtype=torch.uint8, device=device)
+        self.surf = torch.empty(self.ny * self.nx * 12 * esize, dtype=torch.uint8, device=device)
+        self.ring = {}
+        self.window_hour = None
+        self.grid = L.GlideGrid(nx=self.nx, ny=self.ny, nz=self.nz, reserved=0,
+                                lon0=float(self.lon[0]), lon1=float(self.lon[-1]),
+                                lat0=float(self.lat[0]), lat1=float(self.lat[-1]),
+                                levels=self.levels_f.data_ptr())
+        self.window = L.GlideWindow(wind=self.wind.data_ptr(), aux=self.aux.data_ptr(),
+                                    surf=self.surf.data_ptr(), dtype=met_dtype, reserved=0)
+
+    def resampled(self, h: int, stream: int):
+        """Resampled hour ``h`` (out3 [5, Za, Y, X], surface [4, Y, X], p_hpa [Za])."""
+
+        if h in self.ring:
+            return self.ring[h]
+        if h not in self.src:
+            raise ValueError(f"met hour {h} not resident (have {sorted(self.src)})")
+        s = self.src[h]
+        out3 = torch.empty((5, self.nz, self.ny, self.nx), dtype=torch.float32, device=self.device)
+        p_out = torch.empty(self.nz, dtype=torch.float64, device=self.device)
+        dlat = float(self.lat[1] - self.lat[0])
+        dlon = float(self.lon[1] - self.lon[0])
+        L.check(L.lib().glide_resample_hour(
+            s["src3"].data_ptr(), 5, self.zp, self.ny, self.nx, s["height"].data_ptr(),
+            self.levels_d.data_ptr(), self.nz, self.terrain.data_ptr(), self.mlon.data_ptr(),
+            dlat, dlon, s["p_hpa"].data_ptr(), 0, 1, 2, out3.data_ptr(), p_out.data_ptr(), stream))
+        # the window walks backward in time: keep only the hours next to h
+        for old in [k for k in self.ring if abs(k - h) > 2]:
+            del self.ring[old]
+        self.ring[h] = (out3, s["src2"], p_out)
+        return self.ring[h]
+
+    def build_window(self, h: int, stream: int) -> None:
+        if self.window_hour == h:
+            return
+        s3, s2, p = self.resampled(h, stream)
+        e3, e2, _ = self.resampled(h + 1, stream)
+        L.check(L.lib().glide_build_window(
+            s3.data_ptr(), e3.data_ptr(), s2.data_ptr(), e2.data_ptr(), p.data_ptr(),
+            self.levels_f.data_ptr(), self.nz, self.ny, self.nx, self.meander_coeff,
+            self.meander_radius, self.met_dtype, self.wind.data_ptr(), self.aux.data_ptr(),
+            self.surf.data_ptr(), stream))
+        self.window_hour = h
+
+
+class BatchEngine:
+    """One batch of releases on the device, stepped in the reference's order."""
+
+    def __init__(self, cfg: dict, *, device="cuda", met: SyntheticMet | None = None,
+                 met_dtype: int = L.GLIDE_MET_F32, hanna: HannaOptions | None = None,
+                 convection: ConvectionOptions | None = None, compensated: bool = False,
+                 device_met: DeviceMet | None = None):
+        self.cfg = cfg
+        self.device = device
+        self.h = hanna or HannaOptions()
+        self.conv_opts = convection if convection is not None else (
+            ConvectionOptions() if cfg.get("convection") else None)
+        rels = cfg["releases"]
+        self.length_s = int(rels[0].length_s)
+        self.sched = cursor_schedule(rels, int(cfg["dt"]), self.length_s)
+        hours = sorted({s.bracket_hour for s in self.sched} | {s.bracket_hour + 1 for s in self.sched})
+        self.met = device_met or DeviceMet(met or SyntheticMet(cfg["spec"]), hours, device=device,
+                                           met_dtype=met_dtype,
+                                           meander_coeff=self.h.meander_coefficient,
+                                           meander_radius=self.h.meander_stencil_radius)
+        self.seed = int(cfg["seed"])
+        self.ra = release_arrays(rels)
+        self.n = int(self.ra.lon.size)
+        g = cfg["grid"]
+        dom = cfg["domain"]
+        self.grid_cfg = g
+        self.z_edges = _dev(np.asarray(g["z_edges"], np.float32), None, device)
+        self.fg = L.GlideFootprintGrid(
+            lon_min=g["lon_min"], lon_max=g["lon_max"], lat_min=g["lat_min"], lat_max=g["lat_max"],
+            z_edges=self.z_edges.data_ptr(), n_r=g["n_r"], n_t=g["n_t"], n_z=g["n_z"],
+            n_y=g["n_y"], n_x=g["n_x"], reserved=0,
+            dom_lon_min=dom["lon_min"], dom_lon_max=dom["lon_max"], dom_lat_min=dom["lat_min"],
+            dom_lat_max=dom["lat_max"], dom_alt_max=dom["alt_max"])
+        self.n_cells = g["n_r"] * g["n_t"] * g["n_z"] * g["n_y"] * g["n_x"]
+        self.cells_per_release = self.n_cells // g["n_r"]
+        self.compensated = compensated
+        self._alloc()
+        self.reset()
+
+    def _alloc(self):
+        d = self.device
+        n = self.n
+        f32 = torch.float32
+        self.lon = torch.empty(n, dtype=f32, device=d)
+        self.lat = torch.empty(n, dtype=f32, device=d)
+        self.z = torch.empty(n, dtype=f32, device=d)
+        self.up = torch.empty(n, dtype=f32, device=d)
+        self.vp = torch.empty(n, dtype=f32, device=d)
+        self.wp = torch.empty(n, dtype=f32, device=d)
+        self.um = torch.empty(n, dtype=f32, device=d)
+        self.vm = torch.empty(n, dtype=f32, device=d)
+        self.weight = _dev(self.ra.weight, None, d)
+        self.r_off = _dev(self.ra.r_off, None, d)
+        self.we_off = _dev(self.ra.we_off, None, d)
+        self.rel_idx = _dev(self.ra.rel_idx, None, d)
+        self.uid = _dev(self.ra.uid.view(np.int32), None, d)
+        self.alive = torch.empty(n, dtype=torch.uint8, device=d)
+        self.keys = torch.empty(n, dtype=torch.int32, device=d)
+        self.vals = torch.empty(n, dtype=f32, device=d)
+        self.escaped = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)
+        self.footprint = torch.zeros(self.n_cells, dtype=f32, device=d)
+        self.comp = torch.zeros(self.n_cells, dtype=f32, device=d) if self.compensated else None
+        self.release_total = torch.zeros(self.grid_cfg["n_r"], dtype=torch.float64, device=d)
+        ws = L.lib().glide_footprint_workspace_bytes(n)
+        self.workspace = torch.empty(ws, dtype=torch.uint8, device=d)
+        nc = self.met.ny * self.met.nx
+        self.i_lcl = torch.empty(nc, dtype=torch.int32, device=d)
+        self.i_lnb = torch.empty(nc, dtype=torch.int32, device=d)
+        self.m_b = torch.empty(nc, dtype=f32, device=d)
+        self.layer_mass = torch.empty(self.met.nz, dtype=f32, device=d)
+        self.colsum_err = torch.zeros(nc, dtype=torch.float64, device=d)
+        self.colsum_err_max = 0.0
+        self.parts = L.GlideParticles(
+            n=n, pos_stride=1, lon=self.lon.data_ptr(), lat=self.lat.data_ptr(), z=self.z.data_ptr(),
+            up=self.up.data_ptr(), vp=self.vp.data_ptr(), wp=self.wp.data_ptr(),
+            um=self.um.data_ptr(), vm=self.vm.data_ptr(), weight=self.weight.data_ptr(),
+            r_off=self.r_off.data_ptr(), we_off=self.we_off.data_ptr(),
+            rel_idx=self.rel_idx.data_ptr(), uid=self.uid.data_ptr(), alive=self.alive.data_ptr(),
+            active=None)
+
+    def reset(self):
+        """Back to the release state (positions at the sources, state zero)."""
+
+        self.lon.copy_(torch.from_numpy(self.ra.lon))
+        self.lat.copy_(torch.from_numpy(self.ra.lat))
+        self.z.copy_(torch.from_numpy(self.ra.z))
+        for t in (self.up, self.vp, self.wp, self.um, self.vm):
+            t.zero_()
+        self.alive.fill_(1)
+        self.escaped.zero_()
+        self.footprint.zero_()
+        if self.comp is not None:
+            self.comp.zero_()
+        self.release_total.zero_()
+        self.step_i = 0
+        self.last_bracket = None
+        self.step_events = None
+
+    def step_params(self, st, flags: int) -> L.GlideStepParams:
+        h = self.h
+        return L.GlideStepParams(
+            dt=float(st.delta_s), alpha=float(st.alpha), cursor_off=float(st.cursor_off),
+            length_s=float(self.length_s), substep_c=h.substep_c, z_ubl_m=h.z_ubl_m,
+            meander_timescale_s=h.meander_timescale_s, seed=self.seed, step_index=st.index,
+            max_substeps=h.max_substeps, flags=flags, reserved=0)
+
+    def emit(self, st) -> L.GlideEmit:
+        esc = self.escaped.data_ptr() + 8 * st.index
+        return L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(), escaped=esc)
+
+    def run_step(self, stream: int | None = None) -> None:
+        """Advance one step of the schedule."""
+
+        lib = L.lib()
+        if stream is None:
+            stream = L.stream_ptr()
+        st = self.sched[self.step_i]
+        new_bracket = st.bracket_hour != self.last_bracket
+        self.met.build_window(st.bracket_hour, stream)
+        convect = new_bracket and self.conv_opts is not None
+        flags = self.h.flags() | (0 if convect else L.F_EMIT)
+        p = self.step_params(st, flags)
+        em = self.emit(st)
+        ev = self.step_events[self.step_i] if self.step_events is not None else None
+        if ev is not None:
+            ev[0].record(self._torch_stream)
+        L.check(lib.glide_step(C.byref(p), C.byref(self.parts), C.byref(self.met.grid),
+                               C.byref(self.met.window), C.byref(self.fg), C.byref(em), stream))
+        if ev is not None:
+            ev[1].record(self._torch_stream)
+        if convect:
+            s3, _, phpa = self.met.resampled(st.bracket_hour, stream)
+            e3, _, _ = self.met.resampled(st.bracket_hour + 1, stream)
+            co = self.conv_opts
+            conv = self.met.src[st.bracket_hour]["conv"]
+            nlev = self.met.nz * self.met.ny * self.met.nx
+            L.check(lib.glide_convect_columns(
+                conv.data_ptr(), s3.data_ptr() + 3 * nlev * 4, e3.data_ptr() + 3 * nlev * 4,
+                float(st.alpha), phpa.data_ptr(), self.met.levels_f.data_ptr(), self.met.nz,
+                self.met.ny, self.met.nx, co.closure_c, co.trigger_dtv_k, co.min_cape_j_kg,
+                co.min_cloud_depth_m, self.i_lcl.data_ptr(), self.i_lnb.data_ptr(),
+                self.m_b.data_ptr(), self.layer_mass.data_ptr(), stream))
+            pc = self.step_params(st, self.h.flags() | L.F_EMIT)
+            L.check(lib.glide_convect(C.byref(pc), C.byref(self.parts), C.byref(self.met.grid),
+                                      self.i_lcl.data_ptr(), self.i_lnb.data_ptr(), self.m_b.data_ptr(),
+                                      self.layer_mass.data_ptr(), self.colsum_err.data_ptr(),
+                                      C.byref(self.fg), C.byref(em), stream))
+        L.check(lib.glide_footprint_accumulate(
+            self.keys.data_ptr(), self.vals.data_ptr(), self.n, self.n_cells, self.cells_per_release,
+            self.footprint.data_ptr(), self.comp.data_ptr() if self.comp is not None else None,
+            self.release_total.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(), stream))
+        self.last_bracket = st.bracket_hour
+        self.step_i += 1
+
+    def time_step_kernel(self, n_steps: int, stream) -> list:
+        """Record HIP events around the next ``n_steps`` step-kernel launches on
+        ``stream`` (a torch stream, the one the kernels are launched on)."""
+
+        self._torch_stream = stream
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(n_steps)]
+        self.step_events = {self.step_i + i: e for i, e in enumerate(evs)}
+        return evs
+
+    def run(self, n_steps: int | None = None) -> None:
+        end = len(self.sched) if n_steps is None else min(len(self.sched), self.step_i + n_steps)
+        while self.step_i < end:
+            self.run_step()
+
+    def active_count(self, st) -> int:
+        co = np.float32(st.cursor_off)
+        act = (self.ra.r_off >= co) & (self.ra.r_off - np.float32(self.length_s) <= co)
+        return int(act.sum())
+
+    def state_numpy(self) -> dict:
+        torch.cuda.synchronize()
+        out = {k: getattr(self, k).cpu().numpy() for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm")}
+        out["alive"] = self.alive.cpu().numpy().astype(bool)
+        out["uid"] = self.ra.uid
+        return out

@@ -1,0 +1,95 @@
+"""Batch schedule: release arrays and the cursor loop's per-step scalars.
+
+Restates reference main.py:1338-1389 (cursor bounds and delta_s),
+main.py:892-938 (bracket and alpha) and release_generator.py:240-322 (per-
+release particle arrays). Times are integer seconds after the met epoch.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass(frozen=True)
+class StepInfo:
+    index: int
+    cursor_s: int  # pre-step cursor (s after the met epoch)
+    delta_s: int
+    cursor_off: float  # cursor - batch start
+    bracket_hour: int  # hour_floor(cursor) as an hour index
+    alpha: float
+
+
+def cursor_schedule(releases, dt: int, length_s: int) -> list[StepInfo]:
+    """Steps of one batch: from the latest window end down to the earliest
+    window end minus the backward length, ``while t_cursor > terminus``."""
+
+    if dt <= 0:
+        raise ValueError("dt_seconds must be > 0")
+    ends = [r.release_time_s + r.duration_s for r in releases]
+    start = max(ends)
+    term = min(ends) - int(length_s)
+    batch_start = min(r.release_time_s for r in releases)
+    out = []
+    t = start
+    i = 0
+    while t > term:
+        d = min(dt, t - term)
+        bh = t // 3600
+        alpha = max(0.0, min(1.0, ((t - 0.5 * d) - 3600.0 * bh) / 3600.0))
+        out.append(StepInfo(i, int(t), int(d), float(t - batch_start), int(bh), alpha))
+        t -= d
+        i += 1
+    return out
+
+
+@dataclass
+class ReleaseArrays:
+    lon: np.ndarray
+    lat: np.ndarray
+    z: np.ndarray
+    weight: np.ndarray
+    r_off: np.ndarray
+    we_off: np.ndarray
+    rel_idx: np.ndarray
+    uid: np.ndarray
+    batch_start_s: int
+
+
+def release_offsets(seed: int, n: int, duration_s: float) -> np.ndarray:
+    """U[0, duration) float32 offsets from a CPU torch generator seeded with the
+    release seed (release_generator.py:286-293)."""
+
+    import torch
+
+    g = torch.Generator(device="cpu")
+    g.manual_seed(int(seed))
+    return torch.empty(n, dtype=torch.float32).uniform_(0.0, float(duration_s), generator=g).numpy()
+
+
+def release_arrays(releases, uid_base: int = 0) -> ReleaseArrays:
+    """Particle arrays for a batch of point releases (weight 1/N per release,
+    offsets relative to the earliest release in the batch)."""
+
+    if not releases:
+        raise ValueError("ReleaseBatch must contain at least one release")
+    batch_start = min(r.release_time_s for r in releases)
+    cols = {k: [] for k in ("lon", "lat", "z", "weight", "r_off", "we_off", "rel_idx")}
+    for r in releases:
+        n = int(r.n_particles)
+        if n <= 0:
+            raise ValueError(f"release {r.release_idx} has n_particles={n}; must be > 0")
+        off = float(r.release_time_s - batch_start)
+        cols["lon"].append(np.full(n, r.lon, np.float32))
+        cols["lat"].append(np.full(n, r.lat, np.float32))
+        cols["z"].append(np.full(n, r.alt_agl_m, np.float32))
+        cols["weight"].append(np.full(n, 1.0 / n, np.float32))
+        cols["r_off"].append(release_offsets(r.seed, n, r.duration_s) + np.float32(off))
+        cols["we_off"].append(np.full(n, off + r.duration_s, np.float32))
+        cols["rel_idx"].append(np.full(n, r.release_idx, np.int32))
+    a = {k: np.concatenate(v) for k, v in cols.items()}
+    n_tot = a["lon"].size
+    return ReleaseArrays(uid=np.arange(uid_base, uid_base + n_tot, dtype=np.uint32),
+                         batch_start_s=batch_start, **a)

@@ -1,0 +1,269 @@
+/*
+ * glide.h -- C ABI of the MI355X backward-LPDM step engine (libglide_hip.so).
+ *
+ * Plain C: pointers, sizes and POD structs only. Every pointer named "device"
+ * is HIP device memory (for example a torch tensor's data_ptr() on ROCm); the
+ * library never frees caller memory. Streams are hipStream_t passed as void*
+ * (NULL = the default stream). Every entry point returns GLIDE_OK (0) or a
+ * negative status; glide_last_error() gives the message of the last failure on
+ * the calling thread. Status mapping used by the Python layer:
+ *   GLIDE_ERR_INVALID -> ValueError, GLIDE_ERR_MEMORY -> MemoryError,
+ *   GLIDE_ERR_DEVICE  -> RuntimeError.
+ *
+ * Reference interfaces replaced (mrghg/glide, paths under src/lpdm/):
+ *   glide_step                 HannaScheme.step static branch + the driver's
+ *                              active mask / time bin / escape kill
+ *                              (turbulence/hanna.py:580-633, 855-1068;
+ *                               main.py:1396-1401, 1459-1477)
+ *   glide_footprint_keys       FootprintGridder.accumulate index math
+ *                              (footprint_gridder.py:156-196)
+ *   glide_footprint_accumulate FootprintGridder.accumulate scatter_add_
+ *                              (footprint_gridder.py:197)
+ *   glide_resample_hour        ArcoEra5ZarrReader._resample_hour_to_agl
+ *                              (met_reader.py:641-693; vertical_grid.py:166-316)
+ *   glide_build_window         HannaScheme window stacks + raw met staging
+ *                              (hanna.py:747-853, 1694-1830)
+ *   glide_convect_columns      EmanuelReducedConvection.maybe_convect trigger +
+ *                              closure (convection/emanuel.py:629-679)
+ *   glide_convect              EmanuelReducedConvection matrix build +
+ *                              _redistribute_particles, backward
+ *                              (convection/emanuel.py:429-506, 720-805)
+ *   glide_graph_*              HannaScheme._maybe_graph_compile CUDA-graph
+ *                              capture of the step core (hanna.py:1117-1160)
+ */
+#ifndef GLIDE_H
+#define GLIDE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GLIDE_ABI_VERSION 1
+
+#define GLIDE_OK 0
+#define GLIDE_ERR_INVALID (-1)
+#define GLIDE_ERR_MEMORY (-2)
+#define GLIDE_ERR_DEVICE (-3)
+
+/* Met storage precision of a packed window. */
+#define GLIDE_MET_F32 0
+#define GLIDE_MET_BF16 1
+
+/* GlideStepParams.flags */
+#define GLIDE_F_MEANDER 0x1        /* Maryon meander OU (hanna.py:1028-1054) */
+#define GLIDE_F_TL_FLOORS 0x2      /* FLEXPART T_L floors 10/10/30 s (hanna.py:1560-1563) */
+#define GLIDE_F_SL_OVERRIDE 0x4    /* surface-layer override (hanna.py:1542-1554) */
+#define GLIDE_F_EMIT 0x8           /* write footprint keys/values + escape kill */
+#define GLIDE_F_MASK_INPUT 0x10    /* active mask given explicitly (GlideParticles.active) */
+#define GLIDE_F_NO_TURB 0x20       /* skip the turbulence body (advection only) */
+
+/*
+ * Met grid and the per-level AGL ladder (metadata.lon/lat/level,
+ * met_reader.py:85-99). lon0/lon1/lat0/lat1 are the first and last grid
+ * coordinates (latitude may descend). levels: device float[nz], ascending.
+ */
+typedef struct GlideGrid {
+    int32_t nx, ny, nz, reserved;
+    double lon0, lon1, lat0, lat1;
+    const float* levels;
+} GlideGrid;
+
+/*
+ * One packed met window on the device (built by glide_build_window).
+ *   wind: [nz][ny][nx] records of 8: u_s v_s w_s u_e v_e w_e 0 0
+ *   aux : [nz][ny][nx] records of 8: sigma_w_ft T_Lw_ft rho drho_dz sigma_mu sigma_mv 0 0
+ *   surf: [ny][nx] records of 12: blh sp ustar shf t0 (hour start), same (hour end), 0 0
+ * Records hold float (GLIDE_MET_F32) or bfloat16 (GLIDE_MET_BF16) values.
+ */
+typedef struct GlideWindow {
+    const void* wind;
+    const void* aux;
+    const void* surf;
+    int32_t dtype, reserved;
+} GlideWindow;
+
+/*
+ * Particle state, structure of arrays (device). lon/lat/z may be strided
+ * (pos_stride = 4 for the reference's (N, 4) [lon, lat, alt, weight] tensor,
+ * 1 for separate arrays); all other arrays are contiguous.
+ *   uid:     Philox counter word 0 (stable particle id).
+ *   r_off:   release-time offset (s after batch start), we_off: window end
+ *            offset, rel_idx: release index (release_generator.py:202-237).
+ *   alive:   liveness (escape kill), active: explicit mask when
+ *            GLIDE_F_MASK_INPUT is set (else active is computed from r_off,
+ *            length and alive, main.py:1396-1401).
+ * um/vm may be NULL when meander is off.
+ */
+typedef struct GlideParticles {
+    int64_t n;
+    int64_t pos_stride;
+    float* lon;
+    float* lat;
+    float* z;
+    float* up;
+    float* vp;
+    float* wp;
+    float* um;
+    float* vm;
+    const float* weight;
+    const float* r_off;
+    const float* we_off;
+    const int32_t* rel_idx;
+    const uint32_t* uid;
+    uint8_t* alive;
+    const uint8_t* active;
+} GlideParticles;
+
+/*
+ * Footprint geometry (FootprintGridder.__init__, footprint_gridder.py:50-115)
+ * and the met domain of the escape kill (main.py:690-711).
+ * n_cells = n_r*n_t*n_z*n_y*n_x must be < 2^32 - 1.
+ */
+typedef struct GlideFootprintGrid {
+    double lon_min, lon_max, lat_min, lat_max;
+    const float* z_edges; /* device float[n_z + 1], strictly ascending */
+    int32_t n_r, n_t, n_z, n_y, n_x, reserved;
+    double dom_lon_min, dom_lon_max, dom_lat_min, dom_lat_max, dom_alt_max;
+} GlideFootprintGrid;
+
+/*
+ * Per-step scalars and scheme constants.
+ *   dt: this step's delta_s; alpha: the RK2-midpoint time weight;
+ *   cursor_off: pre-step cursor (s after batch start); length_s: backward
+ *   window length; seed/step_index: Philox key and counter word 1.
+ */
+typedef struct GlideStepParams {
+    double dt, alpha, cursor_off, length_s;
+    double substep_c, z_ubl_m, meander_timescale_s;
+    uint64_t seed;
+    int32_t step_index, max_substeps;
+    int32_t flags, reserved;
+} GlideStepParams;
+
+/* Step outputs (device): per-particle footprint key (n_cells = dropped) and
+ * value (weight * dt), and the step's escape count (added to *escaped). */
+typedef struct GlideEmit {
+    uint32_t* keys;
+    float* vals;
+    unsigned long long* escaped;
+} GlideEmit;
+
+int glide_abi_version(void);
+const char* glide_last_error(void);
+
+/* Fused per-particle step: RK2 advection, Hanna sigma/T_L, drifts, substepped
+ * exact-OU with reflection, meander, mask gate; optional footprint key/value
+ * emission and escape kill (flags & GLIDE_F_EMIT). */
+int glide_step(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
+               const GlideWindow* w, const GlideFootprintGrid* fg, const GlideEmit* e,
+               void* stream);
+
+/* FootprintGridder.accumulate index math on arbitrary inputs: positions
+ * lon/lat/z (strided by pos_stride), active mask, weights, per-particle time
+ * bin t_idx and release index (int64, as the reference passes them). */
+int glide_footprint_keys(int64_t n, int64_t pos_stride, const float* lon, const float* lat,
+                         const float* z, const uint8_t* active, const float* weight,
+                         const int64_t* t_idx, const int64_t* rel_idx, double dt,
+                         const GlideFootprintGrid* fg, uint32_t* keys, float* vals,
+                         void* stream);
+
+/* Workspace bytes glide_footprint_accumulate needs for n keys. */
+size_t glide_footprint_workspace_bytes(int64_t n);
+
+/* Deterministic residence accumulation: stable radix sort of (key, value)
+ * by key, segmented reduction in a fixed order, one add per touched cell into
+ * slab[n_cells] (float). comp (may be NULL) receives Neumaier compensation
+ * terms, so slab + comp is the compensated sum. Keys equal to n_cells are
+ * dropped. release_total (may be NULL): double[n_r] += per-release residence,
+ * summed in a fixed order; cells_per_release = n_t*n_z*n_y*n_x. */
+int glide_footprint_accumulate(const uint32_t* keys, const float* vals, int64_t n,
+                               uint32_t n_cells, uint32_t cells_per_release, float* slab,
+                               float* comp, double* release_total, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
+/*
+ * Hourly terrain-following resample of one met hour (device arrays):
+ *   src3 [c3][zp][ny][nx] float, height_agl [zp][ny][nx] float (negative below
+ *   ground), agl_levels [za] double ascending, m_per_deg_lon [ny] double
+ *   (clip(111320 cos(lat), 1, inf)), p_level_hpa [zp] double.
+ *   dlat_deg/dlon_deg: uniform grid spacing (signed) for the terrain gradient,
+ *   terrain [ny][nx] float.
+ * Outputs: dst3 [c3][za][ny][nx] float; w (channel iw) slope-corrected with
+ * u (iu), v (iv); p_out_hpa [za] double = horizontal mean of the regridded
+ * level pressure.
+ */
+int glide_resample_hour(const float* src3, int32_t c3, int32_t zp, int32_t ny, int32_t nx,
+                        const float* height_agl, const double* agl_levels, int32_t za,
+                        const float* terrain, const double* m_per_deg_lon, double dlat_deg,
+                        double dlon_deg, const double* p_level_hpa, int32_t iu, int32_t iv,
+                        int32_t iw, float* dst3, double* p_out_hpa, void* stream);
+
+/*
+ * Window build from two resampled hours (device):
+ *   h3_s, h3_e: [5][nz][ny][nx] float (u, v, w, t, q) at hour start / end;
+ *   h2_s, h2_e: [4][ny][nx] float (blh, sp, ustar, shf);
+ *   p_level_hpa [nz] double (hour start), levels [nz] float.
+ * Writes the packed records of GlideWindow (dtype selects float or bf16):
+ * the window-midpoint free-troposphere sigma/T_L, rho and drho/dz, and the
+ * meander sigma (coefficient * local std over the (2r+1)^2 stencil).
+ */
+int glide_build_window(const float* h3_s, const float* h3_e, const float* h2_s,
+                       const float* h2_e, const double* p_level_hpa, const float* levels,
+                       int32_t nz, int32_t ny, int32_t nx, double meander_coeff,
+                       int32_t meander_radius, int32_t dtype, void* wind, void* aux, void* surf,
+                       void* stream);
+
+/*
+ * Per-column convection descriptors for one met bracket (device):
+ *   conv [4][ny][nx] float: CAPE (J/kg), LCL height, cloud-top height (m AGL),
+ *   T_v excess at LCL+1 (K); t_s, t_e [nz][ny][nx] float resampled temperature
+ *   at the bracket ends, lerped at alpha; p_level_hpa [nz] double; levels [nz].
+ * A column fires when its LCL and cloud top lie on the ladder, top > LCL,
+ * CAPE >= min_cape, T_v excess >= trigger and depth >= min_depth
+ * (emanuel.py:637-650); then m_b = closure_c * rho_LCL * min(sqrt(2 CAPE), 5)
+ * * 3600 s (emanuel.py:652-675). Writes i_lcl/i_lnb (-1 when not firing), m_b
+ * and layer_mass [nz] = |dp|/g half brackets (emanuel.py:263-279).
+ */
+int glide_convect_columns(const float* conv, const float* t_s, const float* t_e, double alpha,
+                          const double* p_level_hpa, const float* levels, int32_t nz, int32_t ny,
+                          int32_t nx, double closure_c, double trigger_dtv_k, double min_cape_j_kg,
+                          double min_cloud_depth_m, int32_t* i_lcl, int32_t* i_lnb, float* m_b,
+                          float* layer_mass, void* stream);
+
+/*
+ * Per-column Emanuel convection (backward transpose sampling).
+ * Column descriptors [ny][nx] (device): i_lcl, i_lnb (int32, -1 = no cloud),
+ * m_b (float, kg/m^2 per event, already times the 3600 s interval).
+ * layer_mass [nz] float (|dp|/g half-brackets), height [nz] float (ascending).
+ * Active particles sample a destination level from the transposed, renormalised
+ * column of the non-divergent mass-flux matrix of their nearest column; movers
+ * get z uniform in [h[dest-1], h[dest]]. colsum_err (may be NULL): per-column
+ * float64 max |row sum - column sum| / m (mass-conservation check).
+ * With GLIDE_F_EMIT in p->flags the footprint keys and the escape kill are
+ * produced after the move, as glide_step does.
+ */
+int glide_convect(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
+                  const int32_t* i_lcl, const int32_t* i_lnb, const float* m_b,
+                  const float* layer_mass, double* colsum_err, const GlideFootprintGrid* fg,
+                  const GlideEmit* e, void* stream);
+
+/*
+ * Step-body graph capture on a caller stream: glide_graph_begin starts stream
+ * capture, glide_graph_end instantiates the captured work into a handle,
+ * glide_graph_launch replays it, glide_graph_destroy frees it. Captured
+ * kernels keep their arguments, so per-step scalars must live in device
+ * memory that the caller refreshes between replays.
+ */
+int glide_graph_begin(void* stream);
+int glide_graph_end(void* stream, void** handle);
+int glide_graph_launch(void* handle, void* stream);
+int glide_graph_destroy(void* handle);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLIDE_H */

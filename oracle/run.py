@@ -1,0 +1,180 @@
+"""Whole-run oracle driver: the reference's static cursor loop for one batch.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Follows reference main.py:1293-1477 (S2 order): active mask, met bracket and
+alpha, the scheme step, convection on a bracket change, time-ago bin from the
+pre-step cursor, gridding of post-step positions, escape kill.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from oracle import gridder as og
+from oracle import lpdm
+from oracle import resample as ors
+from oracle import window as ow
+
+
+def release_offsets(rel_seed: int, n: int, duration_s: float) -> np.ndarray:
+    """Within-release release-time offsets, U[0, duration) from a CPU torch
+    generator seeded with the release seed (reference release_generator.py:286-293)."""
+
+    import torch
+
+    g = torch.Generator(device="cpu")
+    g.manual_seed(int(rel_seed))
+    return torch.empty(n, dtype=torch.float32).uniform_(0.0, float(duration_s), generator=g).numpy()
+
+
+def build_window(hs, he, levels, lon, lat, dty, meander_coeff=0.16, meander_r=1):
+    """Window for the bracket (hs, he) of resampled hours; each h is
+    (out3 [5, Za, Y, X], src2 [4, Y, X], p_level_hpa [Za])."""
+
+    s3, s2, p_hpa = hs
+    e3, e2, _ = he
+    s3 = s3.astype(dty)
+    e3 = e3.astype(dty)
+    surf_s = np.stack([s2[0], s2[1], s2[2], s2[3], s3[3, 0]]).astype(dty)
+    surf_e = np.stack([e2[0], e2[1], e2[2], e2[3], e3[3, 0]]).astype(dty)
+    Za, Y, X = s3.shape[1:]
+    height = np.broadcast_to(np.asarray(levels, dty)[:, None, None], (Za, Y, X))
+    p_pa = np.asarray(p_hpa, dty) * 100.0
+    t_mid = ow.window_mid(s3[3], e3[3])
+    u_mid = ow.window_mid(s3[0], e3[0])
+    v_mid = ow.window_mid(s3[1], e3[1])
+    ft = ow.free_trop_fields(t_mid, u_mid, v_mid, height, p_pa).astype(dty)
+    dens = ow.density_fields(t_mid, height, p_pa).astype(dty)
+    mean = ow.meander_fields(u_mid, v_mid, meander_coeff, meander_r).astype(dty)
+    return lpdm.Window(
+        uvw_s=s3[0:3], uvw_e=e3[0:3], surf_s=surf_s, surf_e=surf_e, ft=ft, dens=dens,
+        meander=mean, lon0=float(lon[0]), lon1=float(lon[-1]), lat0=float(lat[0]),
+        lat1=float(lat[-1]), levels=np.asarray(levels, dty),
+    )
+
+
+@dataclass
+class RunResult:
+    footprint: np.ndarray
+    particles: dict
+    alive: np.ndarray
+    escaped_per_step: list = field(default_factory=list)
+    active_per_step: list = field(default_factory=list)
+    particle_steps: int = 0
+    step_seconds: float = 0.0
+    total_seconds: float = 0.0
+
+
+def schedule(releases, dt: int):
+    """Cursor schedule of one batch (main.py:1338-1389): list of
+    (cursor_s, delta_s) with cursor in seconds after the met epoch."""
+
+    ends = [r.release_time_s + r.duration_s for r in releases]
+    length = releases[0].length_s
+    start = max(ends)
+    term = min(ends) - length
+    out = []
+    t = start
+    while t > term:
+        d = min(dt, t - term)
+        out.append((t, d))
+        t -= d
+    return out
+
+
+def run(cfg, dty=np.float64, params: lpdm.HannaParams | None = None, met=None,
+        max_steps: int | None = None, hours_cache=None, convection=None) -> RunResult:
+    """Run the plain method over the whole single-batch workload ``cfg``
+    (from ``glide_amd.synthetic.workload_config``)."""
+
+    from glide_amd.synthetic import SyntheticMet
+
+    t_all = time.perf_counter()
+    p = params or lpdm.HannaParams()
+    spec = cfg["spec"]
+    met = met or SyntheticMet(spec)
+    levels = spec.agl_levels
+    rels = cfg["releases"]
+    grid, dom = cfg["grid"], cfg["domain"]
+    dt = cfg["dt"]
+    seed = cfg["seed"]
+    batch_start = min(r.release_time_s for r in rels)
+    P = {k: [] for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm", "uid")}
+    w_l, roff_l, weoff_l, ridx_l = [], [], [], []
+    uid0 = 0
+    for r in rels:
+        n = r.n_particles
+        P["lon"].append(np.full(n, r.lon, np.float32))
+        P["lat"].append(np.full(n, r.lat, np.float32))
+        P["z"].append(np.full(n, r.alt_agl_m, np.float32))
+        for k in ("up", "vp", "wp", "um", "vm"):
+            P[k].append(np.zeros(n, np.float32))
+        P["uid"].append(np.arange(uid0, uid0 + n, dtype=np.uint32))
+        uid0 += n
+        w_l.append(np.full(n, 1.0 / n, np.float32))
+        off = float(r.release_time_s - batch_start)
+        roff_l.append(release_offsets(r.seed, n, r.duration_s) + np.float32(off))
+        weoff_l.append(np.full(n, off + r.duration_s, np.float32))
+        ridx_l.append(np.full(n, r.release_idx, np.int64))
+    P = {k: np.concatenate(v) for k, v in P.items()}
+    for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm"):
+        P[k] = P[k].astype(dty)
+    weights = np.concatenate(w_l)
+    roff = np.concatenate(roff_l)
+    weoff = np.concatenate(weoff_l)
+    ridx = np.concatenate(ridx_l)
+    length = float(rels[0].length_s)
+    N = weights.size
+    alive = np.ones(N, bool)
+    fp = np.zeros(grid["n_r"] * grid["n_t"] * grid["n_z"] * grid["n_y"] * grid["n_x"], dty)
+
+    hours = hours_cache if hours_cache is not None else {}
+
+    def hour(h):
+        if h not in hours:
+            sh = met.hour(h)
+            hours[h] = ors.resample_hour(sh.src3, sh.src2, sh.height_agl, met.terrain, met.lat,
+                                         met.lon, levels, sh.p_level_pa)
+        return hours[h]
+
+    windows = {}
+    res = RunResult(footprint=fp, particles=P, alive=alive)
+    sched = schedule(rels, dt)
+    if max_steps is not None:
+        sched = sched[:max_steps]
+    last_bracket = None
+    t_step = 0.0
+    for si, (cur, delta) in enumerate(sched):
+        cursor_off = float(cur - batch_start)
+        active = (roff >= np.float32(cursor_off)) & (roff - np.float32(length) <= np.float32(cursor_off)) & alive
+        bh = cur // 3600
+        if bh not in windows:
+            windows.clear()
+            windows[bh] = build_window(hour(bh), hour(bh + 1), levels, met.lon, met.lat, dty)
+        win = windows[bh]
+        alpha = max(0.0, min(1.0, ((cur - 0.5 * delta) - 3600.0 * bh) / 3600.0))
+        t0 = time.perf_counter()
+        P = lpdm.step(P, active, win, alpha, float(delta), p, seed=seed, step_index=si)
+        if bh != last_bracket:
+            if convection is not None:
+                P = convection(P, active, bh, alpha, si)
+            last_bracket = bh
+        t_idx = og.time_bins(weoff, cursor_off, grid["n_t"])
+        og.accumulate(fp, P["lon"].astype(np.float32), P["lat"].astype(np.float32),
+                      P["z"].astype(np.float32), active, weights, t_idx, ridx, float(delta), grid)
+        inside = og.within_met_domain(P["lon"], P["lat"], P["z"], dom)
+        newly = active & ~inside
+        alive &= ~newly
+        t_step += time.perf_counter() - t0
+        res.escaped_per_step.append(int(newly.sum()))
+        res.active_per_step.append(int(active.sum()))
+    res.particles = P
+    res.alive = alive
+    res.particle_steps = int(sum(res.active_per_step))
+    res.step_seconds = t_step
+    res.total_seconds = time.perf_counter() - t_all
+    return res

@@ -1,0 +1,350 @@
+"""GPU parity: the HIP kernels, called through the C ABI, against the oracle on
+the same seeded inputs.
+
+Bars (from the engine's contract):
+* cell keys, drop decisions, escape flags, resampled fields: bit-exact;
+* one fused step from identical float32 state: lon/lat within 1e-5 relative,
+  z within 1e-5 relative to max(|z|, 1 m), for >= 99.9 % of particles (the
+  rest sit on a discontinuity: a substep-count or regime switch);
+* full 10k-particle / 3 h run: footprint within 1 % relative L2 and total
+  residence within 1e-4 of the float64 oracle on the same Philox noise.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from glide_amd import _lib as L  # noqa: E402
+from glide_amd.runner import BatchEngine, DeviceMet, HannaOptions  # noqa: E402
+from glide_amd.synthetic import SyntheticMet, workload_config  # noqa: E402
+from oracle import convection as OC  # noqa: E402
+from oracle import gridder as OG  # noqa: E402
+from oracle import lpdm as O  # noqa: E402
+from oracle import resample as OR  # noqa: E402
+from oracle import run as ORUN  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    cfg = workload_config("plain-cpu-10k-3h")
+    met = SyntheticMet(cfg["spec"])
+    return cfg, met
+
+
+def _stream():
+    return L.stream_ptr()
+
+
+def _oracle_hour(met, h):
+    sh = met.hour(h)
+    return OR.resample_hour(sh.src3, sh.src2, sh.height_agl, met.terrain, met.lat, met.lon,
+                            met.spec.agl_levels, sh.p_level_pa)
+
+
+# ---------------------------------------------------------------------------
+# (a) resample: bit-exact
+# ---------------------------------------------------------------------------
+
+def test_resample_bit_exact(cfg0):
+    cfg, met = cfg0
+    dm = DeviceMet(met, [0, 1], device="cuda")
+    out3, _, p = dm.resampled(0, _stream())
+    torch.cuda.synchronize()
+    o3, _, op = _oracle_hour(met, 0)
+    got = out3.cpu().numpy()
+    assert got.shape == o3.shape
+    for c in range(5):
+        diff = np.count_nonzero(got[c] != o3[c])
+        assert diff == 0, f"channel {c}: {diff} values differ"
+    np.testing.assert_allclose(p.cpu().numpy(), op, rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# window build: copies exact, derived stacks to float32 tolerance
+# ---------------------------------------------------------------------------
+
+def test_window_fields_match_oracle(cfg0):
+    cfg, met = cfg0
+    dm = DeviceMet(met, [0, 1], device="cuda")
+    dm.build_window(0, _stream())
+    torch.cuda.synchronize()
+    nz, ny, nx = dm.nz, dm.ny, dm.nx
+    wind = dm.wind.view(torch.float32).cpu().numpy().reshape(nz, ny, nx, 8)
+    aux = dm.aux.view(torch.float32).cpu().numpy().reshape(nz, ny, nx, 8)
+    surf = dm.surf.view(torch.float32).cpu().numpy().reshape(ny, nx, 12)
+    h0, h1 = _oracle_hour(met, 0), _oracle_hour(met, 1)
+    win = ORUN.build_window(h0, h1, met.spec.agl_levels, met.lon, met.lat, np.float64)
+    for c in range(3):
+        assert np.array_equal(wind[..., c], h0[0][c]) and np.array_equal(wind[..., 3 + c], h1[0][c])
+    for c in range(5):
+        assert np.array_equal(surf[..., c], win.surf_s[c].astype(np.float32))
+        assert np.array_equal(surf[..., 5 + c], win.surf_e[c].astype(np.float32))
+
+    def close(a, b, rtol, floor, frac=0.999):
+        ok = np.abs(a - b) <= rtol * np.maximum(np.abs(b), floor)
+        assert ok.mean() >= frac, (ok.mean(), np.abs(a - b).max())
+
+    close(aux[..., 2], win.dens[0].transpose(0, 1, 2), 1e-5, 1e-3)
+    close(aux[..., 3], win.dens[1], 1e-3, 1e-6)
+    close(aux[..., 0], win.ft[0], 1e-3, 1e-2)
+    close(aux[..., 1], win.ft[1], 1e-3, 1.0)
+    close(aux[..., 4], win.meander[0], 1e-3, 1e-3)
+    close(aux[..., 5], win.meander[1], 1e-3, 1e-3)
+
+
+# ---------------------------------------------------------------------------
+# (c) footprint: keys bit-exact, accumulation deterministic and accurate
+# ---------------------------------------------------------------------------
+
+def _fg(g, dom=None):
+    dom = dom or dict(lon_min=-1e9, lon_max=1e9, lat_min=-1e9, lat_max=1e9, alt_max=1e9)
+    ze = torch.tensor(g["z_edges"], dtype=torch.float32, device="cuda")
+    fg = L.GlideFootprintGrid(lon_min=g["lon_min"], lon_max=g["lon_max"], lat_min=g["lat_min"],
+                              lat_max=g["lat_max"], z_edges=ze.data_ptr(), n_r=g["n_r"], n_t=g["n_t"],
+                              n_z=g["n_z"], n_y=g["n_y"], n_x=g["n_x"], dom_lon_min=dom["lon_min"],
+                              dom_lon_max=dom["lon_max"], dom_lat_min=dom["lat_min"],
+                              dom_lat_max=dom["lat_max"], dom_alt_max=dom["alt_max"])
+    return fg, ze
+
+
+def _device_keys(g, lon, lat, z, active, w, t, r, dt):
+    n = lon.size
+    d = lambda a, dt_: torch.from_numpy(np.ascontiguousarray(a)).to(dt_).cuda()  # noqa: E731
+    lo, la, zz = d(lon, torch.float32), d(lat, torch.float32), d(z, torch.float32)
+    ac, ww = d(active.astype(np.uint8), torch.uint8), d(w, torch.float32)
+    tt, rr = d(t, torch.int64), d(r, torch.int64)
+    keys = torch.empty(n, dtype=torch.int32, device="cuda")
+    vals = torch.empty(n, dtype=torch.float32, device="cuda")
+    fg, ze = _fg(g)
+    L.check(L.lib().glide_footprint_keys(n, 1, lo.data_ptr(), la.data_ptr(), zz.data_ptr(), ac.data_ptr(),
+                                         ww.data_ptr(), tt.data_ptr(), rr.data_ptr(), dt, C.byref(fg),
+                                         keys.data_ptr(), vals.data_ptr(), _stream()))
+    torch.cuda.synchronize()
+    return keys.cpu().numpy().view(np.uint32), vals.cpu().numpy()
+
+
+def test_footprint_keys_bit_exact():
+    g = dict(lon_min=-1.0, lon_max=1.0, lat_min=-1.0, lat_max=1.0, n_x=37, n_y=29, n_t=5, n_r=3,
+             z_edges=[0.0, 40.0, 100.0, 1000.0, 5000.0], n_z=4)
+    rng = np.random.default_rng(11)
+    n = 200_000
+    lon = rng.uniform(-1.2, 1.2, n).astype(np.float32)
+    lat = rng.uniform(-1.2, 1.2, n).astype(np.float32)
+    z = rng.uniform(-50, 6000, n).astype(np.float32)
+    # edge cases: exact bounds and bin edges
+    lon[:8] = np.float32([-1.0, 1.0, np.nextafter(np.float32(1.0), np.float32(0)), 0.0, -1.0, 1.0, 0.5, 0.5])
+    z[:8] = np.float32([0.0, 40.0, 40.000004, 100.0, 5000.0, 5000.0001, 1e-30, -0.0])
+    lat[8:16] = np.float32([-1.0, 1.0, -1.0000001, 0.99999994, 0, 0, 0, 0])
+    edges = np.float32(np.linspace(-1.0, 1.0, 38))
+    lon[16:54] = edges
+    active = rng.random(n) < 0.9
+    t = rng.integers(-1, 6, n)
+    r = rng.integers(-1, 4, n)
+    w = rng.random(n).astype(np.float32)
+    k, v = _device_keys(g, lon, lat, z, active, w, t, r, 60.0)
+    ok, ov = OG.keys_and_values(lon, lat, z, active, w, t, r, 60.0, g)
+    n_cells = 3 * 5 * 4 * 29 * 37
+    ok = np.where(ok < 0, n_cells, ok).astype(np.uint32)
+    assert np.array_equal(k, ok), np.nonzero(k != ok)[0][:10]
+    assert np.array_equal(v, ov.astype(np.float32))
+
+
+def _accumulate(keys, vals, n_cells, cpr, comp=False):
+    n = keys.size
+    kd = torch.from_numpy(keys.view(np.int32)).cuda()
+    vd = torch.from_numpy(vals).cuda()
+    slab = torch.zeros(n_cells, dtype=torch.float32, device="cuda")
+    cp = torch.zeros(n_cells, dtype=torch.float32, device="cuda") if comp else None
+    tot = torch.zeros(n_cells // cpr, dtype=torch.float64, device="cuda")
+    ws = torch.empty(L.lib().glide_footprint_workspace_bytes(n), dtype=torch.uint8, device="cuda")
+    L.check(L.lib().glide_footprint_accumulate(kd.data_ptr(), vd.data_ptr(), n, n_cells, cpr, slab.data_ptr(),
+                                               cp.data_ptr() if comp else None, tot.data_ptr(), ws.data_ptr(),
+                                               ws.numel(), _stream()))
+    torch.cuda.synchronize()
+    return slab.cpu().numpy(), (cp.cpu().numpy() if comp else None), tot.cpu().numpy()
+
+
+@pytest.mark.parametrize("case", ["random", "one_cell", "runs", "all_dropped", "tiny"])
+def test_footprint_accumulate_matches_oracle_and_is_deterministic(case):
+    rng = np.random.default_rng(5)
+    n_cells, cpr = 921_600, 921_600 // 2
+    if case == "random":
+        n = 1_000_003
+        keys = rng.integers(0, n_cells + 1, n).astype(np.uint32)
+    elif case == "one_cell":
+        n = 1 << 20
+        keys = np.full(n, 12345, np.uint32)
+        keys[::97] = n_cells
+    elif case == "runs":
+        n = 777_777
+        keys = np.repeat(rng.integers(0, n_cells, n // 3000 + 1), 3000)[:n].astype(np.uint32)
+        rng.shuffle(keys)
+    elif case == "all_dropped":
+        n = 50_000
+        keys = np.full(n, n_cells, np.uint32)
+    else:
+        n = 3
+        keys = np.array([7, n_cells, 7], np.uint32)
+    vals = rng.random(n).astype(np.float32)
+    s1, _, t1 = _accumulate(keys, vals, n_cells, cpr)
+    s2, _, t2 = _accumulate(keys, vals, n_cells, cpr)
+    assert np.array_equal(s1, s2) and np.array_equal(t1, t2)  # run-to-run bitwise
+    want = np.zeros(n_cells + 1)
+    np.add.at(want, keys.astype(np.int64), vals.astype(np.float64))
+    assert np.array_equal(s1 != 0, want[:-1] != 0)
+    np.testing.assert_allclose(s1, want[:-1], rtol=2e-6, atol=0)
+    tw = np.array([want[:cpr].sum(), want[cpr:n_cells].sum()])
+    np.testing.assert_allclose(t1, tw, rtol=1e-12)
+    # compensated variant: slab + comp closer to the float64 sum
+    sc, cc, _ = _accumulate(keys, vals, n_cells, cpr, comp=True)
+    err_c = np.abs(sc.astype(np.float64) + cc - want[:-1]).max()
+    assert err_c <= np.abs(s1 - want[:-1]).max() + 1e-12
+
+
+# ---------------------------------------------------------------------------
+# (b) fused step: one step from identical state, first met hour
+# ---------------------------------------------------------------------------
+
+def _set_state(eng, S):
+    for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm"):
+        getattr(eng, k).copy_(torch.from_numpy(np.asarray(S[k], np.float32)))
+
+
+def test_step_parity_first_met_hour(cfg0):
+    cfg, met = cfg0
+    eng = BatchEngine(cfg, device="cuda", met=met)
+    sched = eng.sched
+    ra = eng.ra
+    p = O.HannaParams()
+    hours = {}
+
+    def hour(h):
+        if h not in hours:
+            hours[h] = _oracle_hour(met, h)
+        return hours[h]
+
+    windows = {}
+    S = {k: ra.__dict__[k].astype(np.float64) for k in ("lon", "lat", "z")}
+    for k in ("up", "vp", "wp", "um", "vm"):
+        S[k] = np.zeros(eng.n)
+    S["uid"] = ra.uid
+    worst = []
+    n_checked = 0
+    for st in sched[:61]:
+        bh = st.bracket_hour
+        if bh not in windows:
+            windows[bh] = ORUN.build_window(hour(bh), hour(bh + 1), met.spec.agl_levels, met.lon, met.lat,
+                                            np.float64)
+        win = windows[bh]
+        active = (ra.r_off >= np.float32(st.cursor_off)) & (ra.r_off - np.float32(eng.length_s)
+                                                            <= np.float32(st.cursor_off))
+        # identical float32 start for both
+        S32 = {k: (np.asarray(v, np.float32).astype(np.float64) if k != "uid" else v) for k, v in S.items()}
+        _set_state(eng, S32)
+        eng.step_i = st.index
+        eng.last_bracket = bh
+        eng.alive.fill_(1)
+        eng.run_step()
+        got = eng.state_numpy()
+        ref = O.step(S32, active, win, st.alpha, float(st.delta_s), p, seed=eng.seed, step_index=st.index)
+        if active.any():
+            n_checked += 1
+            e_lon = np.abs(got["lon"][active] - ref["lon"][active]) / np.abs(ref["lon"][active])
+            e_lat = np.abs(got["lat"][active] - ref["lat"][active]) / np.abs(ref["lat"][active])
+            e_z = np.abs(got["z"][active] - ref["z"][active]) / np.maximum(np.abs(ref["z"][active]), 1.0)
+            e = np.maximum(np.maximum(e_lon, e_lat), e_z)
+            frac_ok = float((e <= 1e-5).mean())
+            worst.append((st.index, frac_ok, float(np.median(e)), float(e.max())))
+            assert frac_ok >= 0.999, worst[-1]
+        S = {k: (ref[k] if k != "uid" else v) for k, v in S.items()}  # oracle chain in float64
+    assert n_checked >= 59
+    print("step parity (index, frac within 1e-5, median rel, max rel):", worst[:3], worst[-3:])
+
+
+# ---------------------------------------------------------------------------
+# (d) convection: descriptors and destinations against the oracle
+# ---------------------------------------------------------------------------
+
+def test_convection_parity():
+    cfg = workload_config("plain-cpu-10k-3h")
+    met = SyntheticMet(cfg["spec"])
+    eng = BatchEngine(cfg, device="cuda", met=met)
+    h = 1
+    s3, _, ph = eng.met.resampled(h, _stream())
+    e3, _, _ = eng.met.resampled(h + 1, _stream())
+    conv = eng.met.src[h]["conv"]
+    nz, ny, nx = eng.met.nz, eng.met.ny, eng.met.nx
+    nlev = nz * ny * nx
+    alpha = 0.37
+    lib = L.lib()
+    L.check(lib.glide_convect_columns(conv.data_ptr(), s3.data_ptr() + 3 * nlev * 4, e3.data_ptr() + 3 * nlev * 4,
+                                      alpha, ph.data_ptr(), eng.met.levels_f.data_ptr(), nz, ny, nx, 0.03, 0.9,
+                                      50.0, 500.0, eng.i_lcl.data_ptr(), eng.i_lnb.data_ptr(), eng.m_b.data_ptr(),
+                                      eng.layer_mass.data_ptr(), _stream()))
+    torch.cuda.synchronize()
+    t_s = s3[3].cpu().numpy()
+    t_e = e3[3].cpu().numpy()
+    olcl, olnb, omb = OC.column_descriptors(conv.cpu().numpy(), t_s, t_e, alpha, ph.cpu().numpy(),
+                                            met.spec.agl_levels.astype(np.float32))
+    assert np.array_equal(eng.i_lcl.cpu().numpy().reshape(ny, nx), olcl)
+    assert np.array_equal(eng.i_lnb.cpu().numpy().reshape(ny, nx), olnb)
+    np.testing.assert_allclose(eng.m_b.cpu().numpy().reshape(ny, nx), omb, rtol=1e-6)
+    assert (olcl >= 0).mean() > 0.1, "synthetic CAPE should make some columns fire"
+    # particles spread over the domain and the column
+    rng = np.random.default_rng(9)
+    n = eng.n
+    lon = rng.uniform(met.lon[0], met.lon[-1], n).astype(np.float32)
+    lat = rng.uniform(met.lat[-1], met.lat[0], n).astype(np.float32)
+    z = rng.uniform(0, 8000, n).astype(np.float32)
+    eng.lon.copy_(torch.from_numpy(lon))
+    eng.lat.copy_(torch.from_numpy(lat))
+    eng.z.copy_(torch.from_numpy(z))
+    eng.alive.fill_(1)
+    active = np.ones(n, bool)
+    act_d = torch.ones(n, dtype=torch.uint8, device="cuda")
+    parts = L.GlideParticles(**{f: getattr(eng.parts, f) for f, _ in L.GlideParticles._fields_})
+    parts.active = act_d.data_ptr()
+    p = eng.step_params(eng.sched[5], L.F_MASK_INPUT)
+    err = torch.zeros(ny * nx, dtype=torch.float64, device="cuda")
+    L.check(lib.glide_convect(C.byref(p), C.byref(parts), C.byref(eng.met.grid), eng.i_lcl.data_ptr(),
+                              eng.i_lnb.data_ptr(), eng.m_b.data_ptr(), eng.layer_mass.data_ptr(), err.data_ptr(),
+                              None, None, _stream()))
+    torch.cuda.synchronize()
+    z_dev = eng.z.cpu().numpy()
+    col = OC.nearest_column(lon, lat, met.lon[0], met.lon[-1], met.lat[0], met.lat[-1], nx, ny)
+    z_ref = OC.convect_columns(z.astype(np.float64), active, col, olcl, olnb, omb.astype(np.float64),
+                               ph.cpu().numpy(), met.spec.agl_levels, eng.seed, eng.ra.uid, 5)
+    moved_dev = z_dev != z
+    moved_ref = z_ref != z.astype(np.float64)
+    assert moved_ref.mean() > 0.01
+    agree = (moved_dev == moved_ref) & (np.abs(z_dev - z_ref) <= 1e-4 * np.maximum(np.abs(z_ref), 1.0))
+    assert agree.mean() >= 0.9999, agree.mean()
+    assert err.max().item() < 1e-6  # per-column non-divergence in float64
+
+
+# ---------------------------------------------------------------------------
+# full run: footprint vs the float64 oracle on the same noise
+# ---------------------------------------------------------------------------
+
+def test_full_run_cpu_workload_footprint(cfg0):
+    cfg, met = cfg0
+    eng = BatchEngine(cfg, device="cuda", met=met)
+    eng.run()
+    torch.cuda.synchronize()
+    fp = eng.footprint.cpu().numpy().astype(np.float64)
+    esc = eng.escaped.cpu().numpy()
+    ref = ORUN.run(cfg, dty=np.float64, met=met)
+    rel_l2 = np.linalg.norm(fp - ref.footprint) / np.linalg.norm(ref.footprint)
+    assert rel_l2 <= 0.01, rel_l2
+    tot_dev = float(eng.release_total.cpu().numpy().sum())
+    assert abs(tot_dev - ref.footprint.sum()) <= 1e-4 * ref.footprint.sum()
+    assert abs(fp.sum() - ref.footprint.sum()) <= 1e-4 * ref.footprint.sum()
+    assert int(esc.sum()) == int(sum(ref.escaped_per_step))
+    print(f"full run: rel L2 {rel_l2:.3e}, total {tot_dev:.6f} vs {ref.footprint.sum():.6f}")
