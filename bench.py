@@ -48,7 +48,23 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=32768)
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a float32 copy kernel of this many MiB (reads and writes that "
+                         "many bytes) 5 times, to calibrate the HBM byte counters; prints one JSON line")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int) -> int:
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": n * 4, "bytes_written": n * 4, "launches": 5}))
+    return 0
 
 
 def cpu_baseline(cfg, n_particles: int, n_steps: int) -> dict:
@@ -84,6 +100,8 @@ def main() -> int:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path would shard by release with no collective in the step)", file=sys.stderr)
         return 2
+    if args.copy_calibration_mib:
+        return copy_calibration(args.copy_calibration_mib)
     import numpy as np
     import torch
 

@@ -2,15 +2,18 @@
 // reference footprint_gridder.py:197).
 //
 //  1. Stable LSD radix sort of (key, value) pairs by key, 8-bit digits over the
-//     bits the key range needs. Per pass: per-tile digit histograms, one
-//     exclusive scan in digit-major order, stable per-tile scatter (wavefront
-//     ballot ranking, waves in order, rounds in order).
+//     bits the key range needs. Per pass: per-tile digit histograms, a scan of
+//     each digit's row of tile counts (one workgroup per digit), and a stable
+//     per-tile scatter (wavefront ballot ranking; waves, rounds and tiles in
+//     order) that adds the digit bases it scans itself.
 //  2. Segmented reduction of the sorted values: sequential per thread, a
-//     fixed-shape segmented scan per tile, and a per-tile chain for segments
-//     that cross tiles. Every cell's sum is formed in one fixed order, so the
-//     result does not depend on scheduling.
+//     fixed-shape segmented scan per tile, and one workgroup that scans the
+//     tiles for segments crossing tile boundaries. Every cell's sum is formed in
+//     one fixed order, so the result does not depend on scheduling.
 //  3. One read-modify-write per touched cell: slab[key] += sum, optionally with
 //     a Neumaier compensation term kept in comp[key].
+//  4. Optionally the same reduction keyed by release (key / cells_per_release)
+//     accumulates each release's residence total in float64.
 #include "common.h"
 
 namespace glide {
@@ -29,11 +32,12 @@ struct Workspace {
     float* vals_a;
     uint32_t* keys_b;
     float* vals_b;
-    uint32_t* hist;  // [kRadix][n_sort_tiles]
-    float* head_part;  // [n_red_tiles] sum before the tile's first head (whole tile if none)
-    float* tail_part;  // [n_red_tiles] open segment at the tile end (from its last head)
-    uint32_t* first_key;  // [n_red_tiles] key of the tile's first item
-    uint32_t* flags;  // bit0 any head, bit1 open segment continues, bit2 first item is a head
+    uint32_t* hist;       // [kRadix][n_sort_tiles], scanned per row in place
+    uint32_t* row_total;  // [kRadix]
+    double* head_part;    // [n_red_tiles] sum before the tile's first head (whole tile if none)
+    double* tail_part;    // [n_red_tiles] open segment at the tile end (from its last head)
+    uint32_t* first_key;  // [n_red_tiles] segment key of the tile's first item
+    uint32_t* flags;      // bit0 any head, bit1 open segment continues, bit2 first item is a head
 };
 
 static inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -54,8 +58,9 @@ static size_t layout(int64_t n, Workspace* w, char* base) {
     char* kb = take(sizeof(uint32_t) * n);
     char* vb = take(sizeof(float) * n);
     char* hi = take(sizeof(uint32_t) * kRadix * ts);
-    char* hp = take(sizeof(float) * tr);
-    char* tp = take(sizeof(float) * tr);
+    char* rt = take(sizeof(uint32_t) * kRadix);
+    char* hp = take(sizeof(double) * tr);
+    char* tp = take(sizeof(double) * tr);
     char* fk = take(sizeof(uint32_t) * tr);
     char* fl = take(sizeof(uint32_t) * tr);
     if (w) {
@@ -64,8 +69,9 @@ static size_t layout(int64_t n, Workspace* w, char* base) {
         w->keys_b = reinterpret_cast<uint32_t*>(kb);
         w->vals_b = reinterpret_cast<float*>(vb);
         w->hist = reinterpret_cast<uint32_t*>(hi);
-        w->head_part = reinterpret_cast<float*>(hp);
-        w->tail_part = reinterpret_cast<float*>(tp);
+        w->row_total = reinterpret_cast<uint32_t*>(rt);
+        w->head_part = reinterpret_cast<double*>(hp);
+        w->tail_part = reinterpret_cast<double*>(tp);
         w->first_key = reinterpret_cast<uint32_t*>(fk);
         w->flags = reinterpret_cast<uint32_t*>(fl);
     }
@@ -91,34 +97,39 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
     hist[static_cast<int64_t>(threadIdx.x) * n_tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of m counts in place, one workgroup.
-__global__ __launch_bounds__(kScanThreads) void exclusive_scan_kernel(uint32_t* __restrict__ a, int64_t m) {
-    __shared__ uint32_t part[kScanThreads];
-    const int64_t per = (m + kScanThreads - 1) / kScanThreads;
-    const int64_t lo = threadIdx.x * per;
-    const int64_t hi = lo + per < m ? lo + per : m;
+// Exclusive scan of one digit's row of tile counts (in place) and its total.
+__global__ __launch_bounds__(256) void radix_row_scan_kernel(uint32_t* __restrict__ hist, int64_t n_tiles,
+                                                             uint32_t* __restrict__ row_total) {
+    __shared__ uint32_t part[256];
+    uint32_t* row = hist + static_cast<int64_t>(blockIdx.x) * n_tiles;
+    const int t = threadIdx.x;
+    // contiguous chunk per thread, coalesced in strided passes
+    const int64_t per = (n_tiles + 255) / 256;
+    const int64_t lo = t * per;
+    const int64_t hi = lo + per < n_tiles ? lo + per : n_tiles;
     uint32_t s = 0;
-    for (int64_t j = lo; j < hi; ++j) s += a[j];
-    part[threadIdx.x] = s;
+    for (int64_t j = lo; j < hi; ++j) s += row[j];
+    part[t] = s;
     __syncthreads();
-    for (int off = 1; off < kScanThreads; off <<= 1) {
-        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
         __syncthreads();
-        part[threadIdx.x] += v;
+        part[t] += v;
         __syncthreads();
     }
-    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    uint32_t run = t ? part[t - 1] : 0u;
     for (int64_t j = lo; j < hi; ++j) {
-        const uint32_t v = a[j];
-        a[j] = run;
+        const uint32_t v = row[j];
+        row[j] = run;
         run += v;
     }
+    if (t == 255) row_total[blockIdx.x] = part[255];
 }
 
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ kin, const float* __restrict__ vin, uint32_t* __restrict__ kout,
     float* __restrict__ vout, int64_t n, int shift, int bits, const uint32_t* __restrict__ offs,
-    int64_t n_tiles) {
+    const uint32_t* __restrict__ row_total, int64_t n_tiles) {
     constexpr int kWaves = kSortThreads / 64;
     __shared__ uint32_t base[kRadix];
     __shared__ uint32_t wcnt[kWaves][kRadix];
@@ -126,7 +137,18 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const int wave = t >> 6;
     const int lane = t & 63;
     const uint32_t mask = (1u << bits) - 1u;
-    base[t] = offs[static_cast<int64_t>(t) * n_tiles + blockIdx.x];
+    // digit base = sum of the totals of all smaller digits (block scan)
+    base[t] = row_total[t];
+    __syncthreads();
+    for (int off = 1; off < kRadix; off <<= 1) {
+        const uint32_t v = t >= off ? base[t - off] : 0u;
+        __syncthreads();
+        base[t] += v;
+        __syncthreads();
+    }
+    const uint32_t digit_base = base[t] - row_total[t];
+    __syncthreads();
+    base[t] = digit_base + offs[static_cast<int64_t>(t) * n_tiles + blockIdx.x];
     for (int w = 0; w < kWaves; ++w) wcnt[w][t] = 0;
     __syncthreads();
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -174,47 +196,64 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
 }
 
 // --------------------------------------------------------------------------
-// Segmented reduction
+// Segmented reduction, by cell (float sums into the slab) or by release
+// (float64 sums into the release totals).
 // --------------------------------------------------------------------------
-__device__ __forceinline__ void add_cell(float* slab, float* comp, uint32_t key, float x) {
+struct CellTarget {
+    using Acc = float;
+    uint32_t n_cells;
+    float* slab;
+    float* comp;
+    __device__ __forceinline__ uint32_t seg(uint32_t key) const { return key; }
+    __device__ __forceinline__ bool valid(uint32_t s) const { return s < n_cells; }
+    __device__ __forceinline__ void emit(uint32_t key, float x) const {
 #pragma clang fp contract(off)
-    if (comp == nullptr) {
-        slab[key] += x;
-        return;
+        if (comp == nullptr) {
+            slab[key] += x;
+            return;
+        }
+        // Neumaier: keep the low-order part lost by slab + x in comp.
+        const float s = slab[key];
+        const float t = s + x;
+        const float c = (fabsf(s) >= fabsf(x)) ? (s - t) + x : (x - t) + s;
+        slab[key] = t;
+        comp[key] += c;
     }
-    // Neumaier: keep the low-order part lost by slab + x in comp.
-    const float s = slab[key];
-    const float t = s + x;
-    const float c = (fabsf(s) >= fabsf(x)) ? (s - t) + x : (x - t) + s;
-    slab[key] = t;
-    comp[key] += c;
-}
+};
 
-// One tile of sorted items. Emits every segment that starts and ends inside
-// the tile; records the tile's leading continuation sum (head_part), its open
-// trailing sum (tail_part) and flags for segments that cross tile boundaries.
-__global__ __launch_bounds__(kRedThreads) void seg_reduce_tiles_kernel(
-    const uint32_t* __restrict__ keys, const float* __restrict__ vals, int64_t n, uint32_t n_cells,
-    float* __restrict__ slab, float* __restrict__ comp, Workspace w) {
+struct ReleaseTarget {
+    using Acc = double;
+    uint32_t n_cells, cpr, n_rel;
+    double* total;
+    __device__ __forceinline__ uint32_t seg(uint32_t key) const { return key < n_cells ? key / cpr : n_rel; }
+    __device__ __forceinline__ bool valid(uint32_t s) const { return s < n_rel; }
+    __device__ __forceinline__ void emit(uint32_t r, double x) const { total[r] += x; }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void seg_reduce_tiles_kernel(const uint32_t* __restrict__ keys,
+                                                                       const float* __restrict__ vals, int64_t n,
+                                                                       T tg, Workspace w) {
 #pragma clang fp contract(off)
+    using Acc = typename T::Acc;
     __shared__ uint32_t sk[kRedTile];
     __shared__ float sv[kRedTile];
-    __shared__ float scan_v[kRedThreads];
+    __shared__ Acc scan_v[kRedThreads];
     __shared__ uint32_t scan_f[kRedThreads];
-    __shared__ uint32_t edge_key[2];  // key before the tile, key after the tile
+    __shared__ uint32_t edge_key[2];  // segment key before the tile, after the tile
     const int t = threadIdx.x;
     const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * kRedTile;
     const int cnt = static_cast<int>((n - tile0) < kRedTile ? (n - tile0) : kRedTile);
     for (int q = 0; q < kRedItems; ++q) {
         const int idx = q * kRedThreads + t;
         if (idx < cnt) {
-            sk[idx] = keys[tile0 + idx];
+            sk[idx] = tg.seg(keys[tile0 + idx]);
             sv[idx] = vals[tile0 + idx];
         }
     }
     if (t == 0) {
-        edge_key[0] = tile0 > 0 ? keys[tile0 - 1] : 0xffffffffu;
-        edge_key[1] = (tile0 + cnt < n) ? keys[tile0 + cnt] : 0xffffffffu;
+        edge_key[0] = tile0 > 0 ? tg.seg(keys[tile0 - 1]) : 0xffffffffu;
+        edge_key[1] = (tile0 + cnt < n) ? tg.seg(keys[tile0 + cnt]) : 0xffffffffu;
     }
     __syncthreads();
     const bool first_tile = tile0 == 0;
@@ -222,8 +261,8 @@ __global__ __launch_bounds__(kRedThreads) void seg_reduce_tiles_kernel(
     const bool item0_head = first_tile || sk[0] != edge_key[0];
 
     // Sequential pass over this thread's consecutive items.
-    float acc = 0.0f;
-    float first_part = 0.0f;
+    Acc acc = 0;
+    Acc first_part = 0;
     bool has_head = false;
     uint32_t cur_key = 0;
     const int i0 = t * kRedItems;
@@ -234,15 +273,15 @@ __global__ __launch_bounds__(kRedThreads) void seg_reduce_tiles_kernel(
         const bool head = (idx == 0) ? item0_head : (k != sk[idx - 1]);
         if (head) {
             if (has_head) {
-                if (cur_key < n_cells) add_cell(slab, comp, cur_key, acc);
+                if (tg.valid(cur_key)) tg.emit(cur_key, acc);
             } else {
                 first_part = acc;
             }
             has_head = true;
-            acc = 0.0f;
+            acc = 0;
             cur_key = k;
         }
-        acc += sv[idx];
+        acc += static_cast<Acc>(sv[idx]);
     }
     if (!has_head) first_part = acc;
 
@@ -251,7 +290,7 @@ __global__ __launch_bounds__(kRedThreads) void seg_reduce_tiles_kernel(
     scan_f[t] = has_head ? 1u : 0u;
     __syncthreads();
     for (int off = 1; off < kRedThreads; off <<= 1) {
-        float v = 0.0f;
+        Acc v = 0;
         uint32_t f = 0;
         if (t >= off) {
             v = scan_v[t - off];
@@ -266,28 +305,28 @@ __global__ __launch_bounds__(kRedThreads) void seg_reduce_tiles_kernel(
     }
     if (has_head) {
         // The segment that closes just before this thread's first head.
-        const float s_prev = t > 0 ? scan_v[t - 1] : 0.0f;
+        const Acc s_prev = t > 0 ? scan_v[t - 1] : Acc(0);
         const bool head_before = t > 0 && scan_f[t - 1] != 0;
-        const float tot = s_prev + first_part;
+        const Acc tot = s_prev + first_part;
         if (head_before) {
             const uint32_t k = sk[i0 - 1];
-            if (k < n_cells) add_cell(slab, comp, k, tot);
+            if (tg.valid(k)) tg.emit(k, tot);
         } else if (!item0_head) {
-            w.head_part[blockIdx.x] = tot;  // continuation of an earlier tile's segment
+            w.head_part[blockIdx.x] = static_cast<double>(tot);  // continuation of an earlier tile's segment
         }
     }
     if (t == kRedThreads - 1) {
         const uint32_t lk = sk[cnt - 1];
         const bool next_head = last_tile || edge_key[1] != lk;
         const bool any_head = scan_f[t] != 0;
-        const float open = scan_v[t];
-        uint32_t fl = (any_head ? 1u : 0u) | (next_head ? 0u : 2u) | (item0_head ? 4u : 0u);
+        const Acc open = scan_v[t];
+        const uint32_t fl = (any_head ? 1u : 0u) | (next_head ? 0u : 2u) | (item0_head ? 4u : 0u);
         if (!any_head) {
-            w.head_part[blockIdx.x] = open;  // the whole tile continues one segment
+            w.head_part[blockIdx.x] = static_cast<double>(open);  // the whole tile continues one segment
         } else if (next_head) {
-            if (lk < n_cells) add_cell(slab, comp, lk, open);
+            if (tg.valid(lk)) tg.emit(lk, open);
         } else {
-            w.tail_part[blockIdx.x] = open;
+            w.tail_part[blockIdx.x] = static_cast<double>(open);
         }
         w.first_key[blockIdx.x] = sk[0];
         w.flags[blockIdx.x] = fl;
@@ -297,29 +336,31 @@ __global__ __launch_bounds__(kRedThreads) void seg_reduce_tiles_kernel(
 // Segments crossing tiles. With P_j the open-segment sum at the end of tile j
 // (P_j = tail_j if tile j has a head, else P_{j-1} + head_j), the segment that
 // closes in tile c has total P_{c-1} + head_c. One workgroup scans the tiles in
-// fixed chunks, so the order of additions is fixed.
-__global__ __launch_bounds__(kScanThreads) void seg_fixup_kernel(int64_t n_tiles, uint32_t n_cells,
-                                                                 float* __restrict__ slab,
-                                                                 float* __restrict__ comp, Workspace w) {
+// fixed chunks, so the order of additions is fixed. Partial sums are held in
+// the target's accumulator type (float sums are exact in the double slots).
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void seg_fixup_kernel(int64_t n_tiles, T tg, Workspace w) {
 #pragma clang fp contract(off)
-    __shared__ float agg_v[kScanThreads];
+    using Acc = typename T::Acc;
+    __shared__ Acc agg_v[kScanThreads];
     __shared__ uint32_t agg_f[kScanThreads];
     const int t = threadIdx.x;
     const int64_t per = (n_tiles + kScanThreads - 1) / kScanThreads;
     const int64_t lo = t * per;
     const int64_t hi = lo + per < n_tiles ? lo + per : n_tiles;
-    // Chunk aggregate: (any tile with a head, open sum at the chunk end).
-    float p = 0.0f;
+    auto head = [&](int64_t j) { return static_cast<Acc>(w.head_part[j]); };
+    auto tail = [&](int64_t j) { return static_cast<Acc>(w.tail_part[j]); };
+    Acc p = 0;
     uint32_t f = 0;
     for (int64_t j = lo; j < hi; ++j) {
         const uint32_t fl = w.flags[j];
         if (fl & 1u) {
-            p = (fl & 2u) ? w.tail_part[j] : 0.0f;
+            p = (fl & 2u) ? tail(j) : Acc(0);
             f = 1u;
         } else {
-            p = p + w.head_part[j];
+            p = p + head(j);
             if (!(fl & 2u)) {
-                p = 0.0f;
+                p = 0;
                 f = 1u;
             }
         }
@@ -328,7 +369,7 @@ __global__ __launch_bounds__(kScanThreads) void seg_fixup_kernel(int64_t n_tiles
     agg_f[t] = f;
     __syncthreads();
     for (int off = 1; off < kScanThreads; off <<= 1) {
-        float v = 0.0f;
+        Acc v = 0;
         uint32_t g = 0;
         if (t >= off) {
             v = agg_v[t - off];
@@ -341,58 +382,34 @@ __global__ __launch_bounds__(kScanThreads) void seg_fixup_kernel(int64_t n_tiles
         }
         __syncthreads();
     }
-    float run = t > 0 ? agg_v[t - 1] : 0.0f;
+    Acc run = t > 0 ? agg_v[t - 1] : Acc(0);
     for (int64_t j = lo; j < hi; ++j) {
         const uint32_t fl = w.flags[j];
         if (fl & 1u) {
             if (!(fl & 4u)) {
                 const uint32_t k = w.first_key[j];
-                if (k < n_cells) add_cell(slab, comp, k, run + w.head_part[j]);
+                if (tg.valid(k)) tg.emit(k, run + head(j));
             }
-            run = (fl & 2u) ? w.tail_part[j] : 0.0f;
+            run = (fl & 2u) ? tail(j) : Acc(0);
         } else {
-            run = run + w.head_part[j];
+            run = run + head(j);
             if (!(fl & 2u)) {
                 // A pass-through tile whose segment ends exactly at its end.
                 const uint32_t k = w.first_key[j];
-                if (k < n_cells) add_cell(slab, comp, k, run);
-                run = 0.0f;
+                if (tg.valid(k)) tg.emit(k, run);
+                run = 0;
             }
         }
     }
 }
 
-// Per-release residence totals in float64 over the sorted stream: release r
-// owns keys [r * cpr, (r + 1) * cpr); one block per release sums its range in a
-// fixed strided order and tree.
-__global__ __launch_bounds__(256) void release_total_kernel(const uint32_t* __restrict__ keys,
-                                                            const float* __restrict__ vals, int64_t n,
-                                                            uint32_t n_cells, uint32_t cpr,
-                                                            double* __restrict__ total) {
-    __shared__ double part[256];
-    const uint32_t r = blockIdx.x;
-    const uint64_t klo = static_cast<uint64_t>(r) * cpr;
-    uint64_t khi = klo + cpr;
-    if (khi > n_cells) khi = n_cells;
-    auto lower = [&](uint64_t key) {
-        int64_t a = 0, b = n;
-        while (a < b) {
-            const int64_t m = (a + b) >> 1;
-            if (static_cast<uint64_t>(keys[m]) < key) a = m + 1; else b = m;
-        }
-        return a;
-    };
-    const int64_t lo = lower(klo);
-    const int64_t hi = lower(khi);
-    double s = 0.0;
-    for (int64_t j = lo + threadIdx.x; j < hi; j += blockDim.x) s += static_cast<double>(vals[j]);
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) total[r] += part[0];
+template <typename T>
+static void segmented_reduce(const uint32_t* keys, const float* vals, int64_t n, const T& tg, const Workspace& w,
+                             hipStream_t st) {
+    const int64_t tr = div_up(n, kRedTile);
+    hipLaunchKernelGGL(seg_reduce_tiles_kernel<T>, dim3(static_cast<unsigned>(tr)), dim3(kRedThreads), 0, st, keys,
+                       vals, n, tg, w);
+    hipLaunchKernelGGL(seg_fixup_kernel<T>, dim3(1), dim3(kScanThreads), 0, st, tr, tg, w);
 }
 
 }  // namespace glide
@@ -439,22 +456,16 @@ extern "C" int glide_footprint_accumulate(const uint32_t* keys, const float* val
         float* vout = (ps & 1) ? w.vals_b : w.vals_a;
         hipLaunchKernelGGL(radix_hist_kernel, dim3(static_cast<unsigned>(ts)), dim3(kSortThreads), 0, st, kin, n,
                            shift, mask, w.hist, ts);
-        hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(kScanThreads), 0, st, w.hist,
-                           static_cast<int64_t>(kRadix) * ts);
+        hipLaunchKernelGGL(radix_row_scan_kernel, dim3(kRadix), dim3(256), 0, st, w.hist, ts, w.row_total);
         hipLaunchKernelGGL(radix_scatter_kernel, dim3(static_cast<unsigned>(ts)), dim3(kSortThreads), 0, st, kin,
-                           vin, kout, vout, n, shift, bits, w.hist, ts);
+                           vin, kout, vout, n, shift, bits, w.hist, w.row_total, ts);
         kin = kout;
         vin = vout;
     }
-    GLIDE_HIP(hipGetLastError());
-    const int64_t tr = div_up(n, kRedTile);
-    hipLaunchKernelGGL(seg_reduce_tiles_kernel, dim3(static_cast<unsigned>(tr)), dim3(kRedThreads), 0, st, kin,
-                       vin, n, n_cells, slab, comp, w);
-    hipLaunchKernelGGL(seg_fixup_kernel, dim3(1), dim3(kScanThreads), 0, st, tr, n_cells, slab, comp, w);
+    segmented_reduce(kin, vin, n, CellTarget{n_cells, slab, comp}, w, st);
     if (release_total != nullptr) {
-        const uint32_t nr = n_cells / cells_per_release;
-        hipLaunchKernelGGL(release_total_kernel, dim3(nr), dim3(256), 0, st, kin, vin, n, n_cells,
-                           cells_per_release, release_total);
+        segmented_reduce(kin, vin, n, ReleaseTarget{n_cells, cells_per_release, n_cells / cells_per_release,
+                                                    release_total}, w, st);
     }
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
