@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=1440)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--met", choices=("f32", "bf16"), default="f32")
+    ap.add_argument("--reorder-every", type=int, default=4,
+                    help="sort the particle state by met cell every N steps (0 = never)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=32768)
     ap.add_argument("--cpu-steps", type=int, default=40)
@@ -116,7 +118,7 @@ def main() -> int:
     cfg = workload_config(WORKLOAD)
     met = SyntheticMet(cfg["spec"])
     met_dtype = L.GLIDE_MET_BF16 if args.met == "bf16" else L.GLIDE_MET_F32
-    eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=met_dtype)
+    eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=met_dtype, reorder_every=args.reorder_every)
     n_sched = len(eng.sched)
     K = max(1, min(args.steps, n_sched))
     W = max(0, min(args.warmup, n_sched))
@@ -191,6 +193,7 @@ def main() -> int:
             "convection": "per-column emanuel",
             "meander": True,
             "max_substeps": eng.h.max_substeps,
+            "reorder_every": args.reorder_every,
             "footprint": f"1x{cfg['grid']['n_t']}x1x{cfg['grid']['n_y']}x{cfg['grid']['n_x']} f32",
         },
         "active_particle_steps": psteps,

@@ -48,7 +48,7 @@ extern "C" int glide_graph_end(void* stream, void** handle) {
     hipGraphExec_t ex = nullptr;
     hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         return check_hip(e, "hipGraphInstantiate");
     }
     auto* h = new GraphHandle{g, ex};
@@ -66,8 +66,8 @@ extern "C" int glide_graph_launch(void* handle, void* stream) {
 extern "C" int glide_graph_destroy(void* handle) {
     if (handle == nullptr) return GLIDE_OK;
     auto* h = static_cast<GraphHandle*>(handle);
-    hipGraphExecDestroy(h->exec);
-    hipGraphDestroy(h->graph);
+    (void)hipGraphExecDestroy(h->exec);
+    (void)hipGraphDestroy(h->graph);
     delete h;
     return GLIDE_OK;
 }

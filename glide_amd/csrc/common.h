@@ -64,13 +64,12 @@ __device__ __forceinline__ float u01(uint32_t x) {
     return (static_cast<float>(x >> 9) + 0.5f) * 1.1920928955078125e-07f;
 }
 
-// Box-Muller pair from (ua, ub): (r cos(2 pi ub), r sin(2 pi ub)).
+// Box-Muller pair from (ua, ub): (r cos(2 pi ub), r sin(2 pi ub)), on the
+// hardware log2 / sqrt / sin / cos (the sin and cos take revolutions).
 __device__ __forceinline__ void box_muller(float ua, float ub, float& n0, float& n1) {
-    const float r = sqrtf(-2.0f * logf(ua));
-    float s, c;
-    sincospif(2.0f * ub, &s, &c);
-    n0 = r * c;
-    n1 = r * s;
+    const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(ua));
+    n0 = r * __builtin_amdgcn_cosf(ub);
+    n1 = r * __builtin_amdgcn_sinf(ub);
 }
 
 // ---------------------------------------------------------------------------
@@ -195,7 +194,7 @@ __device__ __forceinline__ float level_index(const float* __restrict__ lev, int 
     const int up = min(max(cnt, 1), nz - 1);
     const float lo = lev[up - 1];
     const float hi = lev[up];
-    float fr = (z - lo) / fmaxf(hi - lo, 1e-6f);
+    float fr = (z - lo) * __builtin_amdgcn_rcpf(fmaxf(hi - lo, 1e-6f));
     fr = fminf(fmaxf(fr, 0.0f), 1.0f);
     return static_cast<float>(up - 1) + fr;
 }

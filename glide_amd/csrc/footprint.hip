@@ -1,6 +1,8 @@
 // Deterministic footprint accumulation (replaces scatter_add_ float atomics,
-// reference footprint_gridder.py:197).
+// reference footprint_gridder.py:197), and the radix sort it is built on.
 //
+//  0. Order-preserving compaction of the valid (key < n_cells) pairs, with the
+//     count kept in device memory: dropped particles never reach the sort.
 //  1. Stable LSD radix sort of (key, value) pairs by key, 8-bit digits over the
 //     bits the key range needs. Per pass: per-tile digit histograms, a scan of
 //     each digit's row of tile counts (one workgroup per digit), and a stable
@@ -14,84 +16,150 @@
 //     a Neumaier compensation term kept in comp[key].
 //  4. Optionally the same reduction keyed by release (key / cells_per_release)
 //     accumulates each release's residence total in float64.
-#include "common.h"
+//
+// Item counts after compaction live in device memory; kernels are launched for
+// the upper bound and tiles past the count exit (nothing waits on the host).
+#include "sort.h"
 
 namespace glide {
 
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;  // 4096
-constexpr int kRadix = 256;
 constexpr int kRedThreads = 256;
 constexpr int kRedItems = 8;
 constexpr int kRedTile = kRedThreads * kRedItems;  // 2048
 constexpr int kScanThreads = 1024;
 
-struct Workspace {
-    uint32_t* keys_a;
-    float* vals_a;
-    uint32_t* keys_b;
-    float* vals_b;
-    uint32_t* hist;       // [kRadix][n_sort_tiles], scanned per row in place
-    uint32_t* row_total;  // [kRadix]
-    double* head_part;    // [n_red_tiles] sum before the tile's first head (whole tile if none)
-    double* tail_part;    // [n_red_tiles] open segment at the tile end (from its last head)
-    uint32_t* first_key;  // [n_red_tiles] segment key of the tile's first item
-    uint32_t* flags;      // bit0 any head, bit1 open segment continues, bit2 first item is a head
-};
-
-static inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
-
 static size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
 
-static size_t layout(int64_t n, Workspace* w, char* base) {
-    const int64_t ts = div_up(n, kSortTile);
-    const int64_t tr = div_up(n, kRedTile);
+struct Bump {
+    char* base;
     size_t off = 0;
-    auto take = [&](size_t bytes) {
-        char* p = base ? base + off : nullptr;
-        off += align256(bytes > 0 ? bytes : 1);
+    template <typename T>
+    T* take(size_t count) {
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += align256(sizeof(T) * (count > 0 ? count : 1));
         return p;
-    };
-    char* ka = take(sizeof(uint32_t) * n);
-    char* va = take(sizeof(float) * n);
-    char* kb = take(sizeof(uint32_t) * n);
-    char* vb = take(sizeof(float) * n);
-    char* hi = take(sizeof(uint32_t) * kRadix * ts);
-    char* rt = take(sizeof(uint32_t) * kRadix);
-    char* hp = take(sizeof(double) * tr);
-    char* tp = take(sizeof(double) * tr);
-    char* fk = take(sizeof(uint32_t) * tr);
-    char* fl = take(sizeof(uint32_t) * tr);
-    if (w) {
-        w->keys_a = reinterpret_cast<uint32_t*>(ka);
-        w->vals_a = reinterpret_cast<float*>(va);
-        w->keys_b = reinterpret_cast<uint32_t*>(kb);
-        w->vals_b = reinterpret_cast<float*>(vb);
-        w->hist = reinterpret_cast<uint32_t*>(hi);
-        w->row_total = reinterpret_cast<uint32_t*>(rt);
-        w->head_part = reinterpret_cast<double*>(hp);
-        w->tail_part = reinterpret_cast<double*>(tp);
-        w->first_key = reinterpret_cast<uint32_t*>(fk);
-        w->flags = reinterpret_cast<uint32_t*>(fl);
     }
-    return off;
+};
+
+size_t sort_layout(int64_t n, SortWork* w, char* base) {
+    const int64_t ts = div_up(n, kSortTile);
+    Bump b{base};
+    SortWork s{};
+    s.count = b.take<uint32_t>(1);
+    s.tile_off = b.take<uint32_t>(ts);
+    s.keys_a = b.take<uint32_t>(n);
+    s.vals_a = b.take<uint32_t>(n);
+    s.keys_b = b.take<uint32_t>(n);
+    s.vals_b = b.take<uint32_t>(n);
+    s.hist = b.take<uint32_t>(static_cast<size_t>(kRadix) * ts);
+    s.row_total = b.take<uint32_t>(kRadix);
+    if (w) *w = s;
+    return b.off;
 }
 
 // --------------------------------------------------------------------------
-// Radix sort kernels
+// Compaction of valid pairs (order-preserving)
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys, int64_t n,
-                                                                  int shift, uint32_t mask,
-                                                                  uint32_t* __restrict__ hist,
+__global__ __launch_bounds__(kSortThreads) void compact_count_kernel(const uint32_t* __restrict__ keys, int64_t n,
+                                                                     uint32_t limit, uint32_t* __restrict__ tile_cnt) {
+    __shared__ uint32_t c;
+    if (threadIdx.x == 0) c = 0;
+    __syncthreads();
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    uint32_t mine = 0;
+    for (int r = 0; r < kSortItems; ++r) {
+        const int64_t j = base + r * kSortThreads + threadIdx.x;
+        if (j < n && keys[j] < limit) ++mine;
+    }
+    atomicAdd(&c, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = c;
+}
+
+// Exclusive scan of m tile counts in place; writes the total to *count.
+__global__ __launch_bounds__(kScanThreads) void scan_tiles_kernel(uint32_t* __restrict__ a, int64_t m,
+                                                                  uint32_t* __restrict__ count) {
+    __shared__ uint32_t part[kScanThreads];
+    const int t = threadIdx.x;
+    const int64_t per = (m + kScanThreads - 1) / kScanThreads;
+    const int64_t lo = t * per;
+    const int64_t hi = lo + per < m ? lo + per : m;
+    uint32_t s = 0;
+    for (int64_t j = lo; j < hi; ++j) s += a[j];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0u;
+    for (int64_t j = lo; j < hi; ++j) {
+        const uint32_t v = a[j];
+        a[j] = run;
+        run += v;
+    }
+    if (t == kScanThreads - 1) *count = part[t];
+}
+
+__global__ __launch_bounds__(kSortThreads) void compact_scatter_kernel(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, int64_t n, uint32_t limit,
+    const uint32_t* __restrict__ tile_off, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ uint32_t wsum[kWaves];
+    __shared__ uint32_t run;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (t == 0) run = tile_off[blockIdx.x];
+    __syncthreads();
+    const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        const int64_t j = tile0 + r * kSortThreads + t;
+        uint32_t key = 0, val = 0;
+        bool v = false;
+        if (j < n) {
+            key = kin[j];
+            val = vin[j];
+            v = key < limit;
+        }
+        const unsigned long long m = __ballot(v);
+        if (lane == 0) wsum[wave] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        uint32_t before = run;
+        for (int q = 0; q < wave; ++q) before += wsum[q];
+        if (v) {
+            const uint32_t pos = before + static_cast<uint32_t>(__popcll(m & lt));
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        if (t == 0) {
+            uint32_t s = 0;
+            for (int q = 0; q < kWaves; ++q) s += wsum[q];
+            run += s;
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------------------
+// Radix sort kernels (counts from device memory)
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys,
+                                                                  const uint32_t* __restrict__ n_dev, int shift,
+                                                                  uint32_t mask, uint32_t* __restrict__ hist,
                                                                   int64_t n_tiles) {
     __shared__ uint32_t cnt[kRadix];
+    const int64_t n = *n_dev;
     cnt[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
-    for (int r = 0; r < kSortItems; ++r) {
-        const int64_t j = base + r * kSortThreads + threadIdx.x;
-        if (j < n) atomicAdd(&cnt[(keys[j] >> shift) & mask], 1u);
+    if (base < n) {
+        for (int r = 0; r < kSortItems; ++r) {
+            const int64_t j = base + r * kSortThreads + threadIdx.x;
+            if (j < n) atomicAdd(&cnt[(keys[j] >> shift) & mask], 1u);
+        }
     }
     __syncthreads();
     hist[static_cast<int64_t>(threadIdx.x) * n_tiles + blockIdx.x] = cnt[threadIdx.x];
@@ -103,7 +171,6 @@ __global__ __launch_bounds__(256) void radix_row_scan_kernel(uint32_t* __restric
     __shared__ uint32_t part[256];
     uint32_t* row = hist + static_cast<int64_t>(blockIdx.x) * n_tiles;
     const int t = threadIdx.x;
-    // contiguous chunk per thread, coalesced in strided passes
     const int64_t per = (n_tiles + 255) / 256;
     const int64_t lo = t * per;
     const int64_t hi = lo + per < n_tiles ? lo + per : n_tiles;
@@ -127,12 +194,15 @@ __global__ __launch_bounds__(256) void radix_row_scan_kernel(uint32_t* __restric
 }
 
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
-    const uint32_t* __restrict__ kin, const float* __restrict__ vin, uint32_t* __restrict__ kout,
-    float* __restrict__ vout, int64_t n, int shift, int bits, const uint32_t* __restrict__ offs,
-    const uint32_t* __restrict__ row_total, int64_t n_tiles) {
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, const uint32_t* __restrict__ n_dev, int shift, int bits,
+    const uint32_t* __restrict__ offs, const uint32_t* __restrict__ row_total, int64_t n_tiles) {
     constexpr int kWaves = kSortThreads / 64;
     __shared__ uint32_t base[kRadix];
     __shared__ uint32_t wcnt[kWaves][kRadix];
+    const int64_t n = *n_dev;
+    const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * kSortTile;
+    if (tile0 >= n) return;  // uniform across the block
     const int t = threadIdx.x;
     const int wave = t >> 6;
     const int lane = t & 63;
@@ -152,13 +222,10 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     for (int w = 0; w < kWaves; ++w) wcnt[w][t] = 0;
     __syncthreads();
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * kSortTile;
     for (int r = 0; r < kSortItems; ++r) {
         const int64_t j = tile0 + r * kSortThreads + t;
         const bool valid = j < n;
-        uint32_t key = 0;
-        float val = 0.0f;
-        uint32_t d = 0;
+        uint32_t key = 0, val = 0, d = 0;
         if (valid) {
             key = kin[j];
             val = vin[j];
@@ -195,10 +262,61 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     }
 }
 
+__global__ void set_count_kernel(uint32_t* count, uint32_t n) { *count = n; }
+
+// Sorts (keys, vals) 4-byte pairs by the low key_bits of the key. With
+// compact_limit > 0 only pairs with key < compact_limit are kept (in order).
+// Returns pointers to the sorted arrays (in the workspace) and leaves the item
+// count in w.count.
+void radix_sort_pairs(const uint32_t* keys, const uint32_t* vals, int64_t n, int key_bits, uint32_t compact_limit,
+                      const SortWork& w, hipStream_t st, const uint32_t** out_keys, const uint32_t** out_vals) {
+    const int64_t ts = div_up(n, kSortTile);
+    const uint32_t* kin = keys;
+    const uint32_t* vin = vals;
+    int slot = 0;
+    if (compact_limit > 0) {
+        hipLaunchKernelGGL(compact_count_kernel, dim3(static_cast<unsigned>(ts)), dim3(kSortThreads), 0, st, keys, n,
+                           compact_limit, w.tile_off);
+        hipLaunchKernelGGL(scan_tiles_kernel, dim3(1), dim3(kScanThreads), 0, st, w.tile_off, ts, w.count);
+        hipLaunchKernelGGL(compact_scatter_kernel, dim3(static_cast<unsigned>(ts)), dim3(kSortThreads), 0, st, keys,
+                           vals, n, compact_limit, w.tile_off, w.keys_a, w.vals_a);
+        kin = w.keys_a;
+        vin = w.vals_a;
+        slot = 1;
+    } else {
+        hipLaunchKernelGGL(set_count_kernel, dim3(1), dim3(1), 0, st, w.count, static_cast<uint32_t>(n));
+    }
+    const int npass = (key_bits + 7) / 8;
+    for (int ps = 0; ps < npass; ++ps) {
+        const int shift = ps * 8;
+        const int bits = (key_bits - shift) < 8 ? (key_bits - shift) : 8;
+        const uint32_t mask = (1u << bits) - 1u;
+        uint32_t* kout = slot ? w.keys_b : w.keys_a;
+        uint32_t* vout = slot ? w.vals_b : w.vals_a;
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(static_cast<unsigned>(ts)), dim3(kSortThreads), 0, st, kin, w.count,
+                           shift, mask, w.hist, ts);
+        hipLaunchKernelGGL(radix_row_scan_kernel, dim3(kRadix), dim3(256), 0, st, w.hist, ts, w.row_total);
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(static_cast<unsigned>(ts)), dim3(kSortThreads), 0, st, kin, vin,
+                           kout, vout, w.count, shift, bits, w.hist, w.row_total, ts);
+        kin = kout;
+        vin = vout;
+        slot ^= 1;
+    }
+    *out_keys = kin;
+    *out_vals = vin;
+}
+
 // --------------------------------------------------------------------------
 // Segmented reduction, by cell (float sums into the slab) or by release
 // (float64 sums into the release totals).
 // --------------------------------------------------------------------------
+struct RedWork {
+    double* head_part;    // [n_red_tiles] sum before the tile's first head (whole tile if none)
+    double* tail_part;    // [n_red_tiles] open segment at the tile end (from its last head)
+    uint32_t* first_key;  // [n_red_tiles] segment key of the tile's first item
+    uint32_t* flags;      // bit0 any head, bit1 open segment continues, bit2 first item is a head
+};
+
 struct CellTarget {
     using Acc = float;
     uint32_t n_cells;
@@ -232,8 +350,9 @@ struct ReleaseTarget {
 
 template <typename T>
 __global__ __launch_bounds__(kRedThreads) void seg_reduce_tiles_kernel(const uint32_t* __restrict__ keys,
-                                                                       const float* __restrict__ vals, int64_t n,
-                                                                       T tg, Workspace w) {
+                                                                       const float* __restrict__ vals,
+                                                                       const uint32_t* __restrict__ n_dev, T tg,
+                                                                       RedWork w) {
 #pragma clang fp contract(off)
     using Acc = typename T::Acc;
     __shared__ uint32_t sk[kRedTile];
@@ -241,8 +360,10 @@ __global__ __launch_bounds__(kRedThreads) void seg_reduce_tiles_kernel(const uin
     __shared__ Acc scan_v[kRedThreads];
     __shared__ uint32_t scan_f[kRedThreads];
     __shared__ uint32_t edge_key[2];  // segment key before the tile, after the tile
+    const int64_t n = *n_dev;
     const int t = threadIdx.x;
     const int64_t tile0 = static_cast<int64_t>(blockIdx.x) * kRedTile;
+    if (tile0 >= n) return;  // uniform across the block
     const int cnt = static_cast<int>((n - tile0) < kRedTile ? (n - tile0) : kRedTile);
     for (int q = 0; q < kRedItems; ++q) {
         const int idx = q * kRedThreads + t;
@@ -339,11 +460,13 @@ __global__ __launch_bounds__(kRedThreads) void seg_reduce_tiles_kernel(const uin
 // fixed chunks, so the order of additions is fixed. Partial sums are held in
 // the target's accumulator type (float sums are exact in the double slots).
 template <typename T>
-__global__ __launch_bounds__(kScanThreads) void seg_fixup_kernel(int64_t n_tiles, T tg, Workspace w) {
+__global__ __launch_bounds__(kScanThreads) void seg_fixup_kernel(const uint32_t* __restrict__ n_dev, T tg,
+                                                                 RedWork w) {
 #pragma clang fp contract(off)
     using Acc = typename T::Acc;
     __shared__ Acc agg_v[kScanThreads];
     __shared__ uint32_t agg_f[kScanThreads];
+    const int64_t n_tiles = div_up(static_cast<int64_t>(*n_dev), kRedTile);
     const int t = threadIdx.x;
     const int64_t per = (n_tiles + kScanThreads - 1) / kScanThreads;
     const int64_t lo = t * per;
@@ -404,12 +527,25 @@ __global__ __launch_bounds__(kScanThreads) void seg_fixup_kernel(int64_t n_tiles
 }
 
 template <typename T>
-static void segmented_reduce(const uint32_t* keys, const float* vals, int64_t n, const T& tg, const Workspace& w,
-                             hipStream_t st) {
-    const int64_t tr = div_up(n, kRedTile);
+static void segmented_reduce(const uint32_t* keys, const float* vals, int64_t n_max, const uint32_t* n_dev,
+                             const T& tg, const RedWork& w, hipStream_t st) {
+    const int64_t tr = div_up(n_max, kRedTile);
     hipLaunchKernelGGL(seg_reduce_tiles_kernel<T>, dim3(static_cast<unsigned>(tr)), dim3(kRedThreads), 0, st, keys,
-                       vals, n, tg, w);
-    hipLaunchKernelGGL(seg_fixup_kernel<T>, dim3(1), dim3(kScanThreads), 0, st, tr, tg, w);
+                       vals, n_dev, tg, w);
+    hipLaunchKernelGGL(seg_fixup_kernel<T>, dim3(1), dim3(kScanThreads), 0, st, n_dev, tg, w);
+}
+
+static size_t footprint_layout(int64_t n, SortWork* sw, RedWork* rw, char* base) {
+    const size_t s = sort_layout(n, sw, base);
+    const int64_t tr = div_up(n, kRedTile);
+    Bump b{base ? base + s : nullptr};
+    RedWork r{};
+    r.head_part = b.take<double>(tr);
+    r.tail_part = b.take<double>(tr);
+    r.first_key = b.take<uint32_t>(tr);
+    r.flags = b.take<uint32_t>(tr);
+    if (rw) *rw = r;
+    return s + b.off;
 }
 
 }  // namespace glide
@@ -418,7 +554,7 @@ using namespace glide;
 
 extern "C" size_t glide_footprint_workspace_bytes(int64_t n) {
     if (n < 0) n = 0;
-    return layout(n, nullptr, nullptr);
+    return footprint_layout(n, nullptr, nullptr, nullptr);
 }
 
 extern "C" int glide_footprint_accumulate(const uint32_t* keys, const float* vals, int64_t n,
@@ -430,43 +566,57 @@ extern "C" int glide_footprint_accumulate(const uint32_t* keys, const float* val
     GLIDE_REQUIRE(keys && vals && slab, "NULL keys/vals/slab");
     GLIDE_REQUIRE(n_cells > 0 && n_cells < 0xffffffffu, "bad n_cells");
     GLIDE_REQUIRE(n < (1ll << 32) - 1, "too many keys (positions must fit 32 bits)");
-    const size_t need = layout(n, nullptr, nullptr);
+    const size_t need = footprint_layout(n, nullptr, nullptr, nullptr);
     GLIDE_REQUIRE(workspace != nullptr && workspace_bytes >= need,
                   "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
     if (release_total != nullptr) {
         GLIDE_REQUIRE(cells_per_release > 0 && n_cells % cells_per_release == 0,
                       "cells_per_release must divide n_cells");
     }
-    Workspace w{};
-    layout(n, &w, static_cast<char*>(workspace));
+    SortWork sw{};
+    RedWork rw{};
+    footprint_layout(n, &sw, &rw, static_cast<char*>(workspace));
     hipStream_t st = static_cast<hipStream_t>(stream);
-
-    // Key bits: enough to represent the sentinel n_cells.
+    // Key bits: enough for any key below n_cells (dropped pairs are compacted out).
     int key_bits = 1;
-    while (key_bits < 32 && (static_cast<uint64_t>(n_cells) >> key_bits) != 0) ++key_bits;
-    const int64_t ts = div_up(n, kSortTile);
-    const int npass = (key_bits + 7) / 8;
-    const uint32_t* kin = keys;
-    const float* vin = vals;
-    for (int ps = 0; ps < npass; ++ps) {
-        const int shift = ps * 8;
-        const int bits = (key_bits - shift) < 8 ? (key_bits - shift) : 8;
-        const uint32_t mask = (1u << bits) - 1u;
-        uint32_t* kout = (ps & 1) ? w.keys_b : w.keys_a;
-        float* vout = (ps & 1) ? w.vals_b : w.vals_a;
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(static_cast<unsigned>(ts)), dim3(kSortThreads), 0, st, kin, n,
-                           shift, mask, w.hist, ts);
-        hipLaunchKernelGGL(radix_row_scan_kernel, dim3(kRadix), dim3(256), 0, st, w.hist, ts, w.row_total);
-        hipLaunchKernelGGL(radix_scatter_kernel, dim3(static_cast<unsigned>(ts)), dim3(kSortThreads), 0, st, kin,
-                           vin, kout, vout, n, shift, bits, w.hist, w.row_total, ts);
-        kin = kout;
-        vin = vout;
-    }
-    segmented_reduce(kin, vin, n, CellTarget{n_cells, slab, comp}, w, st);
+    while (key_bits < 32 && (static_cast<uint64_t>(n_cells - 1) >> key_bits) != 0) ++key_bits;
+    const uint32_t* sk = nullptr;
+    const uint32_t* sv = nullptr;
+    radix_sort_pairs(keys, reinterpret_cast<const uint32_t*>(vals), n, key_bits, n_cells, sw, st, &sk, &sv);
+    const float* svf = reinterpret_cast<const float*>(sv);
+    segmented_reduce(sk, svf, n, sw.count, CellTarget{n_cells, slab, comp}, rw, st);
     if (release_total != nullptr) {
-        segmented_reduce(kin, vin, n, ReleaseTarget{n_cells, cells_per_release, n_cells / cells_per_release,
-                                                    release_total}, w, st);
+        segmented_reduce(sk, svf, n, sw.count,
+                         ReleaseTarget{n_cells, cells_per_release, n_cells / cells_per_release, release_total}, rw,
+                         st);
     }
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}
+
+extern "C" size_t glide_sort_workspace_bytes(int64_t n) {
+    if (n < 0) n = 0;
+    return sort_layout(n, nullptr, nullptr);
+}
+
+extern "C" int glide_sort_pairs(const uint32_t* keys, const uint32_t* vals, int64_t n, int32_t key_bits,
+                                uint32_t* out_keys, uint32_t* out_vals, void* workspace, size_t workspace_bytes,
+                                void* stream) {
+    GLIDE_REQUIRE(n >= 0 && key_bits >= 1 && key_bits <= 32, "bad n / key_bits");
+    if (n == 0) return GLIDE_OK;
+    GLIDE_REQUIRE(keys && vals && out_keys && out_vals, "NULL array");
+    GLIDE_REQUIRE(n < (1ll << 32) - 1, "too many keys (positions must fit 32 bits)");
+    const size_t need = sort_layout(n, nullptr, nullptr);
+    GLIDE_REQUIRE(workspace != nullptr && workspace_bytes >= need,
+                  "workspace too small: need %zu bytes, got %zu", need, workspace_bytes);
+    SortWork sw{};
+    sort_layout(n, &sw, static_cast<char*>(workspace));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const uint32_t* sk = nullptr;
+    const uint32_t* sv = nullptr;
+    radix_sort_pairs(keys, vals, n, key_bits, 0u, sw, st, &sk, &sv);
+    GLIDE_HIP(hipMemcpyAsync(out_keys, sk, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+    GLIDE_HIP(hipMemcpyAsync(out_vals, sv, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
 }

@@ -1,0 +1,127 @@
+// Particle reordering for gather locality.
+//
+// The fused step kernel gathers its met records per particle; when the 64
+// lanes of a wavefront hold particles in unrelated cells every lane touches its
+// own cache lines and the gathers become L2-bandwidth bound. Sorting the
+// particle state by met cell (in the records' memory order: level, row,
+// column) puts each wavefront's particles in a few neighbouring cells, so the
+// corner loads coalesce. Particles that can never be active again (escaped, or
+// past their backward window) sort to the end.
+//
+// Results do not depend on the order: noise is keyed by the particle id, which
+// moves with the particle.
+#include "sort.h"
+
+namespace glide {
+
+struct ReorderKeyArgs {
+    GlideParticles s;
+    float lon0, lat0, sx, sy;
+    int nx, ny, nz;
+    const float* levels;
+    float cursor_off, length_s;
+    uint32_t sentinel;
+    uint32_t* keys;
+    uint32_t* idx;
+};
+
+constexpr int kMaxLev = 128;
+
+__global__ __launch_bounds__(256) void reorder_key_kernel(const ReorderKeyArgs a) {
+    __shared__ float lev[kMaxLev];
+    for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
+    __syncthreads();
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.s.n) return;
+    const GlideParticles& s = a.s;
+    const bool done = (s.alive[i] == 0) || (s.r_off[i] - a.length_s > a.cursor_off);
+    uint32_t key = a.sentinel;
+    if (!done) {
+        const int64_t ps = s.pos_stride;
+        const int ix = static_cast<int>(frac_index(s.lon[i * ps], a.lon0, a.sx, a.nx));
+        const int iy = static_cast<int>(frac_index(s.lat[i * ps], a.lat0, a.sy, a.ny));
+        const int iz = static_cast<int>(level_index(lev, a.nz, s.z[i * ps]));
+        key = static_cast<uint32_t>((static_cast<int64_t>(iz) * a.ny + iy) * a.nx + ix);
+    }
+    a.keys[i] = key;
+    a.idx[i] = static_cast<uint32_t>(i);
+}
+
+struct PermuteArgs {
+    GlideParticles src, dst;
+    const uint32_t* idx;
+};
+
+template <typename T>
+__device__ __forceinline__ void mv(const T* s, T* d, int64_t j, int64_t i) {
+    if (s != nullptr && d != nullptr) d[j] = s[i];
+}
+
+__global__ __launch_bounds__(256) void permute_kernel(const PermuteArgs a) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= a.src.n) return;
+    const int64_t i = a.idx[j];
+    const GlideParticles& s = a.src;
+    const GlideParticles& d = a.dst;
+    mv(s.lon, d.lon, j, i);
+    mv(s.lat, d.lat, j, i);
+    mv(s.z, d.z, j, i);
+    mv(s.up, d.up, j, i);
+    mv(s.vp, d.vp, j, i);
+    mv(s.wp, d.wp, j, i);
+    mv(s.um, d.um, j, i);
+    mv(s.vm, d.vm, j, i);
+    mv(s.weight, const_cast<float*>(d.weight), j, i);
+    mv(s.r_off, const_cast<float*>(d.r_off), j, i);
+    mv(s.we_off, const_cast<float*>(d.we_off), j, i);
+    mv(s.rel_idx, const_cast<int32_t*>(d.rel_idx), j, i);
+    mv(s.uid, const_cast<uint32_t*>(d.uid), j, i);
+    mv(s.alive, d.alive, j, i);
+}
+
+}  // namespace glide
+
+using namespace glide;
+
+extern "C" int glide_reorder_keys(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
+                                  uint32_t* keys, uint32_t* idx, void* stream) {
+    GLIDE_REQUIRE(p && s && g && keys && idx, "glide_reorder_keys: NULL argument");
+    GLIDE_REQUIRE(s->lon && s->lat && s->z && s->alive && s->r_off, "particle arrays are NULL");
+    GLIDE_REQUIRE(s->pos_stride >= 1 && s->n >= 0, "bad particles");
+    GLIDE_REQUIRE(g->nz >= 2 && g->nz <= kMaxLev && g->nx >= 2 && g->ny >= 2, "bad grid");
+    GLIDE_REQUIRE(g->lon1 != g->lon0 && g->lat1 != g->lat0, "Grid dimension has 0 span (first == last)");
+    if (s->n == 0) return GLIDE_OK;
+    ReorderKeyArgs a{};
+    a.s = *s;
+    a.nx = g->nx;
+    a.ny = g->ny;
+    a.nz = g->nz;
+    a.lon0 = static_cast<float>(g->lon0);
+    a.lat0 = static_cast<float>(g->lat0);
+    a.sx = static_cast<float>((g->nx - 1) / (g->lon1 - g->lon0));
+    a.sy = static_cast<float>((g->ny - 1) / (g->lat1 - g->lat0));
+    a.levels = g->levels;
+    a.cursor_off = static_cast<float>(p->cursor_off);
+    a.length_s = static_cast<float>(p->length_s);
+    a.sentinel = static_cast<uint32_t>(static_cast<int64_t>(g->nz) * g->ny * g->nx);
+    a.keys = keys;
+    a.idx = idx;
+    hipLaunchKernelGGL(reorder_key_kernel, dim3(static_cast<unsigned>((s->n + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), a);
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}
+
+extern "C" int glide_permute_particles(const GlideParticles* src, const GlideParticles* dst, const uint32_t* idx,
+                                       void* stream) {
+    GLIDE_REQUIRE(src && dst && idx, "glide_permute_particles: NULL argument");
+    GLIDE_REQUIRE(src->n == dst->n && src->n >= 0, "source and destination sizes differ");
+    GLIDE_REQUIRE(src->pos_stride == 1 && dst->pos_stride == 1, "permutation needs structure-of-arrays positions");
+    GLIDE_REQUIRE(src->lon != dst->lon, "permutation cannot run in place");
+    if (src->n == 0) return GLIDE_OK;
+    PermuteArgs a{*src, *dst, idx};
+    hipLaunchKernelGGL(permute_kernel, dim3(static_cast<unsigned>((src->n + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), a);
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}

@@ -1,0 +1,36 @@
+// Stable LSD radix sort of 4-byte (key, value) pairs, shared by the footprint
+// accumulation and the particle reordering.
+#pragma once
+
+#include "common.h"
+
+namespace glide {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // 4096
+constexpr int kRadix = 256;
+
+__host__ __device__ inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+struct SortWork {
+    uint32_t* count;      // [1] items being sorted (device)
+    uint32_t* tile_off;   // [n_sort_tiles] compaction offsets
+    uint32_t* keys_a;
+    uint32_t* vals_a;
+    uint32_t* keys_b;
+    uint32_t* vals_b;
+    uint32_t* hist;       // [kRadix][n_sort_tiles], scanned per row in place
+    uint32_t* row_total;  // [kRadix]
+};
+
+// Bytes of sort workspace for n pairs; fills *w when base is not NULL.
+size_t sort_layout(int64_t n, SortWork* w, char* base);
+
+// Sorts by the low key_bits of the key; compact_limit > 0 first keeps only the
+// pairs with key < compact_limit (order-preserving). The sorted arrays are in
+// the workspace (returned) and the item count is left in w.count.
+void radix_sort_pairs(const uint32_t* keys, const uint32_t* vals, int64_t n, int key_bits, uint32_t compact_limit,
+                      const SortWork& w, hipStream_t st, const uint32_t** out_keys, const uint32_t** out_vals);
+
+}  // namespace glide

@@ -83,8 +83,8 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
         const float u = v[0] * (1.0f - al) + v[3] * al;
         const float vv = v[1] * (1.0f - al) + v[4] * al;
         const float w = v[2] * (1.0f - al) + v[5] * al;
-        dlo = u / m_per_deg_lon(la);
-        dla = vv / kMPerDegLat;
+        dlo = fdiv(u, m_per_deg_lon(la));
+        dla = vv * (1.0f / kMPerDegLat);
         dzz = w;
     };
     {
@@ -136,8 +136,8 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
         stability(sf.blh, sp, sf.ustar, shf, tsf, L, wstar);
         sf.L = L;
         sf.wstar = wstar;
-        sf.hol = sf.blh / L;
-        sf.fcor = 2.0f * kOmega * sinf(lat * kDeg2Rad);
+        sf.hol = fdiv(sf.blh, L);
+        sf.fcor = 2.0f * kOmega * fsin_deg(lat);
 
         const bool tl_floors = (p.flags & GLIDE_F_TL_FLOORS) != 0;
         const bool sl_over = (p.flags & GLIDE_F_SL_OVERRIDE) != 0;
@@ -198,14 +198,15 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
         const float z_lo = fmaxf(z_eval - 1.0f, zu);
         const float sw_hi = sw_probe(z_hi);
         const float sw_lo = sw_probe(z_lo);
-        const float dsig2 = (sw_hi * sw_hi - sw_lo * sw_lo) / fmaxf(z_hi - z_lo, 1.0f);
+        const float dsig2 = fdiv(sw_hi * sw_hi - sw_lo * sw_lo, fmaxf(z_hi - z_lo, 1.0f));
         const float sw_sq = fmaxf(t.sw * t.sw, kSigmaMin * kSigmaMin);
 
         const float rho = fmaxf(ax[2], 1e-3f);
-        const float dens_b = -(sw_sq * ax[3] / rho);
+        const float dens_b = -fdiv(sw_sq * ax[3], rho);
         const float half_b = -0.5f * dsig2;
 
         // Substep schedule (hanna.py:1162-1186).
+        // IEEE division: the ratio sits exactly on integers for the T_L floors.
         const float ratio = dt / (static_cast<float>(p.substep_c) * fmaxf(t.tw, 1e-3f));
         int k = p.max_substeps;
         if (ratio < static_cast<float>(p.max_substeps)) k = max(static_cast<int>(ceilf(ratio)), 1);
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
             box_muller(u01(r.x), u01(r.y), n0, n1);
             box_muller(u01(r.z), u01(r.w), n2, n3);
             (void)n3;
-            const float factor = 1.0f + wp * wp / sw_sq;
+            const float factor = 1.0f + fdiv(wp * wp, sw_sq);
             const float drift = factor * half_b + dens_b;
             float un = ou(up, t.tu, su2, sdt, n0, 0.0f);
             float vn = ou(vp, t.tv, sv2, sdt, n1, 0.0f);
@@ -232,8 +233,8 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
             vn = fminf(fmaxf(vn, -vmax), vmax);
             z = z + (-wn) * sdt;
             const float ml = m_per_deg_lon(lat);
-            lon = lon + (-un) * sdt / ml;
-            lat = lat + (-vn) * sdt / kMPerDegLat;
+            lon = lon + fdiv((-un) * sdt, ml);
+            lat = lat + (-vn) * sdt * (1.0f / kMPerDegLat);
             if (z < 0.0f) {
                 z = 0.0f - z;
                 wn = -wn;
@@ -251,8 +252,8 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
             um = ou(um, tm, ax[4] * ax[4], dt, n0, 0.0f);
             vm = ou(vm, tm, ax[5] * ax[5], dt, n1, 0.0f);
             const float ml = m_per_deg_lon(lat);
-            lon = lon + (-um) * dt / ml;
-            lat = lat + (-vm) * dt / kMPerDegLat;
+            lon = lon + fdiv((-um) * dt, ml);
+            lat = lat + (-vm) * dt * (1.0f / kMPerDegLat);
         }
     }
 

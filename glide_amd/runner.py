@@ -152,8 +152,9 @@ class BatchEngine:
     def __init__(self, cfg: dict, *, device="cuda", met: SyntheticMet | None = None,
                  met_dtype: int = L.GLIDE_MET_F32, hanna: HannaOptions | None = None,
                  convection: ConvectionOptions | None = None, compensated: bool = False,
-                 device_met: DeviceMet | None = None):
+                 device_met: DeviceMet | None = None, reorder_every: int = 0):
         self.cfg = cfg
+        self.reorder_every = int(reorder_every)
         self.device = device
         self.h = hanna or HannaOptions()
         self.conv_opts = convection if convection is not None else (
@@ -185,24 +186,19 @@ class BatchEngine:
         self._alloc()
         self.reset()
 
+    STATE = ("lon", "lat", "z", "up", "vp", "wp", "um", "vm", "weight", "r_off", "we_off", "rel_idx",
+             "uid", "alive")
+
     def _alloc(self):
         d = self.device
         n = self.n
         f32 = torch.float32
-        self.lon = torch.empty(n, dtype=f32, device=d)
-        self.lat = torch.empty(n, dtype=f32, device=d)
-        self.z = torch.empty(n, dtype=f32, device=d)
-        self.up = torch.empty(n, dtype=f32, device=d)
-        self.vp = torch.empty(n, dtype=f32, device=d)
-        self.wp = torch.empty(n, dtype=f32, device=d)
-        self.um = torch.empty(n, dtype=f32, device=d)
-        self.vm = torch.empty(n, dtype=f32, device=d)
-        self.weight = _dev(self.ra.weight, None, d)
-        self.r_off = _dev(self.ra.r_off, None, d)
-        self.we_off = _dev(self.ra.we_off, None, d)
-        self.rel_idx = _dev(self.ra.rel_idx, None, d)
-        self.uid = _dev(self.ra.uid.view(np.int32), None, d)
-        self.alive = torch.empty(n, dtype=torch.uint8, device=d)
+        dtypes = dict(rel_idx=torch.int32, uid=torch.int32, alive=torch.uint8)
+        self.bufs = []
+        for _ in range(2):  # double buffer for the reordering permutation
+            self.bufs.append({k: torch.empty(n, dtype=dtypes.get(k, f32), device=d) for k in self.STATE})
+        self.cur = 0
+        self._parts = [self._make_parts(b) for b in self.bufs]
         self.keys = torch.empty(n, dtype=torch.int32, device=d)
         self.vals = torch.empty(n, dtype=f32, device=d)
         self.escaped = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)
@@ -211,30 +207,51 @@ class BatchEngine:
         self.release_total = torch.zeros(self.grid_cfg["n_r"], dtype=torch.float64, device=d)
         ws = L.lib().glide_footprint_workspace_bytes(n)
         self.workspace = torch.empty(ws, dtype=torch.uint8, device=d)
+        self.sort_ws = torch.empty(L.lib().glide_sort_workspace_bytes(n), dtype=torch.uint8, device=d)
+        self.ro_keys = torch.empty(n, dtype=torch.int32, device=d)
+        self.ro_idx = torch.empty(n, dtype=torch.int32, device=d)
+        self.ro_keys_s = torch.empty(n, dtype=torch.int32, device=d)
+        self.ro_idx_s = torch.empty(n, dtype=torch.int32, device=d)
+        ncell = self.met.nz * self.met.ny * self.met.nx
+        self.ro_bits = max(1, int(ncell).bit_length())
         nc = self.met.ny * self.met.nx
         self.i_lcl = torch.empty(nc, dtype=torch.int32, device=d)
         self.i_lnb = torch.empty(nc, dtype=torch.int32, device=d)
         self.m_b = torch.empty(nc, dtype=f32, device=d)
         self.layer_mass = torch.empty(self.met.nz, dtype=f32, device=d)
         self.colsum_err = torch.zeros(nc, dtype=torch.float64, device=d)
-        self.colsum_err_max = 0.0
-        self.parts = L.GlideParticles(
-            n=n, pos_stride=1, lon=self.lon.data_ptr(), lat=self.lat.data_ptr(), z=self.z.data_ptr(),
-            up=self.up.data_ptr(), vp=self.vp.data_ptr(), wp=self.wp.data_ptr(),
-            um=self.um.data_ptr(), vm=self.vm.data_ptr(), weight=self.weight.data_ptr(),
-            r_off=self.r_off.data_ptr(), we_off=self.we_off.data_ptr(),
-            rel_idx=self.rel_idx.data_ptr(), uid=self.uid.data_ptr(), alive=self.alive.data_ptr(),
-            active=None)
+
+    def _make_parts(self, b) -> L.GlideParticles:
+        return L.GlideParticles(
+            n=self.n, pos_stride=1, lon=b["lon"].data_ptr(), lat=b["lat"].data_ptr(), z=b["z"].data_ptr(),
+            up=b["up"].data_ptr(), vp=b["vp"].data_ptr(), wp=b["wp"].data_ptr(), um=b["um"].data_ptr(),
+            vm=b["vm"].data_ptr(), weight=b["weight"].data_ptr(), r_off=b["r_off"].data_ptr(),
+            we_off=b["we_off"].data_ptr(), rel_idx=b["rel_idx"].data_ptr(), uid=b["uid"].data_ptr(),
+            alive=b["alive"].data_ptr(), active=None)
+
+    @property
+    def parts(self) -> L.GlideParticles:
+        return self._parts[self.cur]
+
+    def __getattr__(self, name):
+        # current-buffer state arrays: eng.lon, eng.z, eng.alive, ...
+        if name in BatchEngine.STATE and "bufs" in self.__dict__:
+            return self.__dict__["bufs"][self.__dict__["cur"]][name]
+        raise AttributeError(name)
 
     def reset(self):
-        """Back to the release state (positions at the sources, state zero)."""
+        """Back to the release state (positions at the sources, state zero,
+        release order)."""
 
-        self.lon.copy_(torch.from_numpy(self.ra.lon))
-        self.lat.copy_(torch.from_numpy(self.ra.lat))
-        self.z.copy_(torch.from_numpy(self.ra.z))
-        for t in (self.up, self.vp, self.wp, self.um, self.vm):
-            t.zero_()
-        self.alive.fill_(1)
+        self.cur = 0
+        b = self.bufs[0]
+        ra = self.ra
+        for k, v in (("lon", ra.lon), ("lat", ra.lat), ("z", ra.z), ("weight", ra.weight), ("r_off", ra.r_off),
+                     ("we_off", ra.we_off), ("rel_idx", ra.rel_idx), ("uid", ra.uid.view(np.int32))):
+            b[k].copy_(torch.from_numpy(np.ascontiguousarray(v)))
+        for k in ("up", "vp", "wp", "um", "vm"):
+            b[k].zero_()
+        b["alive"].fill_(1)
         self.escaped.zero_()
         self.footprint.zero_()
         if self.comp is not None:
@@ -243,6 +260,22 @@ class BatchEngine:
         self.step_i = 0
         self.last_bracket = None
         self.step_events = None
+
+    def reorder(self, st, stream: int) -> None:
+        """Sort the particle state by met cell (gather locality); particles that
+        can never be active again go to the end."""
+
+        lib = L.lib()
+        p = self.step_params(st, 0)
+        L.check(lib.glide_reorder_keys(C.byref(p), C.byref(self.parts), C.byref(self.met.grid),
+                                       self.ro_keys.data_ptr(), self.ro_idx.data_ptr(), stream))
+        L.check(lib.glide_sort_pairs(self.ro_keys.data_ptr(), self.ro_idx.data_ptr(), self.n, self.ro_bits,
+                                     self.ro_keys_s.data_ptr(), self.ro_idx_s.data_ptr(), self.sort_ws.data_ptr(),
+                                     self.sort_ws.numel(), stream))
+        nxt = 1 - self.cur
+        L.check(lib.glide_permute_particles(C.byref(self._parts[self.cur]), C.byref(self._parts[nxt]),
+                                            self.ro_idx_s.data_ptr(), stream))
+        self.cur = nxt
 
     def step_params(self, st, flags: int) -> L.GlideStepParams:
         h = self.h
@@ -263,6 +296,8 @@ class BatchEngine:
         if stream is None:
             stream = L.stream_ptr()
         st = self.sched[self.step_i]
+        if self.reorder_every and self.step_i > 0 and self.step_i % self.reorder_every == 0:
+            self.reorder(st, stream)
         new_bracket = st.bracket_hour != self.last_bracket
         self.met.build_window(st.bracket_hour, stream)
         convect = new_bracket and self.conv_opts is not None
@@ -321,8 +356,26 @@ class BatchEngine:
         return int(act.sum())
 
     def state_numpy(self) -> dict:
+        """Particle state on the host, in particle-id (release) order."""
+
         torch.cuda.synchronize()
-        out = {k: getattr(self, k).cpu().numpy() for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm")}
-        out["alive"] = self.alive.cpu().numpy().astype(bool)
-        out["uid"] = self.ra.uid
+        b = self.bufs[self.cur]
+        uid = b["uid"].cpu().numpy().view(np.uint32)
+        order = np.argsort(uid, kind="stable")
+        out = {k: b[k].cpu().numpy()[order] for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm")}
+        out["alive"] = b["alive"].cpu().numpy().astype(bool)[order]
+        out["uid"] = uid[order]
         return out
+
+    def set_state(self, S: dict) -> None:
+        """Load per-particle state given in particle-id order (tests)."""
+
+        self.cur = 0
+        b = self.bufs[0]
+        ra = self.ra
+        for k, v in (("weight", ra.weight), ("r_off", ra.r_off), ("we_off", ra.we_off), ("rel_idx", ra.rel_idx),
+                     ("uid", ra.uid.view(np.int32))):
+            b[k].copy_(torch.from_numpy(np.ascontiguousarray(v)))
+        for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm"):
+            b[k].copy_(torch.from_numpy(np.asarray(S[k], np.float32)))
+        b["alive"].fill_(1)

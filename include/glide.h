@@ -184,6 +184,32 @@ int glide_footprint_accumulate(const uint32_t* keys, const float* vals, int64_t 
                                float* comp, double* release_total, void* workspace,
                                size_t workspace_bytes, void* stream);
 
+/* Workspace bytes glide_sort_pairs needs for n pairs. */
+size_t glide_sort_workspace_bytes(int64_t n);
+
+/* Stable LSD radix sort of n 4-byte (key, value) pairs by the low key_bits of
+ * the key (the engine's building block for the footprint and the particle
+ * reordering; torch.sort(stable=True) of the reference's index math).
+ * Outputs go to out_keys / out_vals (device). */
+int glide_sort_pairs(const uint32_t* keys, const uint32_t* vals, int64_t n, int32_t key_bits,
+                     uint32_t* out_keys, uint32_t* out_vals, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/*
+ * Particle reordering for gather locality (no counterpart in the reference,
+ * which keeps particles in release order): keys[i] = met cell of particle i in
+ * record memory order ((level * ny + row) * nx + column), or nz*ny*nx for a
+ * particle that can never be active again (escaped, or r_off - length_s >
+ * cursor_off); idx[i] = i. Sort with glide_sort_pairs, then permute.
+ */
+int glide_reorder_keys(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
+                       uint32_t* keys, uint32_t* idx, void* stream);
+
+/* dst.array[j] = src.array[idx[j]] for every non-NULL array of the particle
+ * state (structure of arrays, pos_stride 1, src and dst distinct). */
+int glide_permute_particles(const GlideParticles* src, const GlideParticles* dst,
+                            const uint32_t* idx, void* stream);
+
 /*
  * Hourly terrain-following resample of one met hour (device arrays):
  *   src3 [c3][zp][ny][nx] float, height_agl [zp][ny][nx] float (negative below

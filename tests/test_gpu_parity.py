@@ -228,8 +228,7 @@ def test_footprint_accumulate_matches_oracle_and_is_deterministic(case):
 # ---------------------------------------------------------------------------
 
 def _set_state(eng, S):
-    for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm"):
-        getattr(eng, k).copy_(torch.from_numpy(np.asarray(S[k], np.float32)))
+    eng.set_state(S)
 
 
 def test_step_parity_first_met_hour(cfg0):
@@ -265,7 +264,6 @@ def test_step_parity_first_met_hour(cfg0):
         _set_state(eng, S32)
         eng.step_i = st.index
         eng.last_bracket = bh
-        eng.alive.fill_(1)
         eng.run_step()
         got = eng.state_numpy()
         ref = O.step(S32, active, win, st.alpha, float(st.delta_s), p, seed=eng.seed, step_index=st.index)
@@ -367,3 +365,41 @@ def test_full_run_cpu_workload_footprint(cfg0):
     assert abs(fp.sum() - ref.footprint.sum()) <= 1e-4 * ref.footprint.sum()
     assert int(esc.sum()) == int(sum(ref.escaped_per_step))
     print(f"full run: rel L2 {rel_l2:.3e}, total {tot_dev:.6f} vs {ref.footprint.sum():.6f}")
+
+
+def test_reordering_leaves_every_particle_bit_identical(cfg0):
+    """Sorting the particle state by met cell changes which thread runs which
+    particle, not the result: noise is keyed by particle id. Footprints differ
+    only in the summation order inside a cell."""
+    cfg, met = cfg0
+    a = BatchEngine(cfg, device="cuda", met=met, reorder_every=0)
+    b = BatchEngine(cfg, device="cuda", met=met, reorder_every=3)
+    a.run()
+    b.run()
+    sa, sb = a.state_numpy(), b.state_numpy()
+    for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm", "alive"):
+        assert np.array_equal(sa[k], sb[k]), k
+    assert np.array_equal(a.escaped.cpu().numpy(), b.escaped.cpu().numpy())
+    fa, fb = a.footprint.cpu().numpy(), b.footprint.cpu().numpy()
+    assert np.array_equal(fa != 0, fb != 0)
+    np.testing.assert_allclose(fb, fa, rtol=2e-5, atol=0)
+    assert not np.array_equal(b.bufs[b.cur]["uid"].cpu().numpy(), np.arange(b.n, dtype=np.int32))
+
+
+def test_sort_pairs_matches_stable_numpy_sort():
+    rng = np.random.default_rng(3)
+    for n, bits in ((1, 5), (4097, 13), (300_001, 22), (1 << 20, 32)):
+        keys = rng.integers(0, 2**bits, n, dtype=np.uint64).astype(np.uint32)
+        keys[: n // 3] = keys[n // 2]  # many ties: stability matters
+        vals = np.arange(n, dtype=np.uint32)
+        kd = torch.from_numpy(keys.view(np.int32)).cuda()
+        vd = torch.from_numpy(vals.view(np.int32)).cuda()
+        ko = torch.empty_like(kd)
+        vo = torch.empty_like(vd)
+        ws = torch.empty(L.lib().glide_sort_workspace_bytes(n), dtype=torch.uint8, device="cuda")
+        L.check(L.lib().glide_sort_pairs(kd.data_ptr(), vd.data_ptr(), n, bits, ko.data_ptr(), vo.data_ptr(),
+                                         ws.data_ptr(), ws.numel(), _stream()))
+        torch.cuda.synchronize()
+        order = np.argsort(keys, kind="stable")
+        assert np.array_equal(ko.cpu().numpy().view(np.uint32), keys[order])
+        assert np.array_equal(vo.cpu().numpy().view(np.uint32), vals[order])
