@@ -240,39 +240,71 @@ __global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
     }
 }
 
-// Mass conservation per column in float64: materialise f[i][j] and report
-// max_i |row_i - col_i| / m_i (non-divergence, emanuel.py:437-446).
-__global__ void colsum_check_kernel(const int32_t* i_lcl, const int32_t* i_lnb, const float* m_b,
-                                    const float* lm_g, int nz, int64_t ncol, double* err) {
-    const int64_t col = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (col >= ncol) return;
-    const int lcl = i_lcl[col], lnb = i_lnb[col];
-    const float mb = m_b[col];
-    double worst = 0.0;
-    if (lcl >= 0 && lnb > lcl && mb > 0.0f) {
-        float lm[kConvLevels];
-        for (int k = 0; k < nz; ++k) lm[k] = lm_g[k];
-        const Cloud c = make_cloud(lcl, lnb, mb, lm, nz);
-        float phi[kConvLevels];
-        float ce = 0.0f, cd = 0.0f;
-        for (int k = 0; k < nz; ++k) {
-            ce += cloud_e(c, lm, k);
-            cd += cloud_d(c, k);
-            phi[k] = (k < nz - 1) ? fmaxf(ce - cd, 0.0f) : 0.0f;
+// Mass conservation per column in float64: the float entries
+// f[i][j] = scale (e_i d_j / M_b + [i == j+1] phi_j), summed in float64 along
+// rows and columns, give max_i |row_i - col_i| / m_i (non-divergence,
+// emanuel.py:437-446). One wavefront per column; lane i takes rows i, i+64.
+constexpr int kColsPerBlock = 4;
+
+__global__ __launch_bounds__(64 * kColsPerBlock) void colsum_check_kernel(const int32_t* i_lcl, const int32_t* i_lnb,
+                                                                         const float* m_b, const float* lm_g, int nz,
+                                                                         int64_t ncol, double* err) {
+    __shared__ float lm[kConvLevels];
+    __shared__ float es[kColsPerBlock][kConvLevels];
+    __shared__ float dsh[kColsPerBlock][kConvLevels];
+    __shared__ float ph[kColsPerBlock][kConvLevels];
+    for (int k = threadIdx.x; k < nz; k += blockDim.x) lm[k] = lm_g[k];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t col = static_cast<int64_t>(blockIdx.x) * kColsPerBlock + wave;
+    const bool have = col < ncol;
+    int lcl = -1, lnb = -1;
+    float mb = 0.0f;
+    if (have) {
+        lcl = i_lcl[col];
+        lnb = i_lnb[col];
+        mb = m_b[col];
+    }
+    const bool cloud = have && lcl >= 0 && lnb > lcl && mb > 0.0f;
+    Cloud c{};
+    if (cloud) {
+        c = make_cloud(lcl, lnb, mb, lm, nz);
+        for (int k = lane; k < nz; k += 64) {
+            es[wave][k] = cloud_e(c, lm, k);
+            dsh[wave][k] = cloud_d(c, k);
         }
-        for (int i = 0; i < nz; ++i) {
+        if (lane == 0) {
+            float ce = 0.0f, cd = 0.0f;
+            for (int k = 0; k < nz; ++k) {
+                ce += cloud_e(c, lm, k);
+                cd += cloud_d(c, k);
+                ph[wave][k] = (k < nz - 1) ? fmaxf(ce - cd, 0.0f) : 0.0f;
+            }
+        }
+    }
+    __syncthreads();
+    double worst = 0.0;
+    if (cloud) {
+        const float* e = es[wave];
+        const float* d = dsh[wave];
+        const float* phi = ph[wave];
+        for (int i = lane; i < nz; i += 64) {
             double row = 0.0, cols = 0.0;
             for (int j = 0; j < nz; ++j) {
-                const float fij = c.scale * (cloud_e(c, lm, i) * cloud_d(c, j) / mb + ((i == j + 1) ? phi[j] : 0.0f));
-                const float fji = c.scale * (cloud_e(c, lm, j) * cloud_d(c, i) / mb + ((j == i + 1) ? phi[i] : 0.0f));
+                const float fij = c.scale * (e[i] * d[j] / mb + ((i == j + 1) ? phi[j] : 0.0f));
+                const float fji = c.scale * (e[j] * d[i] / mb + ((j == i + 1) ? phi[i] : 0.0f));
                 row += static_cast<double>(fij);
                 cols += static_cast<double>(fji);
             }
-            const double e = fabs(row - cols) / fmax(static_cast<double>(lm[i]), 1e-12);
-            worst = e > worst ? e : worst;
+            const double r = fabs(row - cols) / fmax(static_cast<double>(lm[i]), 1e-12);
+            worst = r > worst ? r : worst;
         }
     }
-    err[col] = worst;
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(worst, off, 64);
+        worst = o > worst ? o : worst;
+    }
+    if (have && lane == 0) err[col] = worst;
 }
 
 }  // namespace glide
@@ -341,8 +373,8 @@ extern "C" int glide_convect(const GlideStepParams* p, const GlideParticles* s, 
     }
     if (colsum_err != nullptr) {
         const int64_t ncol = static_cast<int64_t>(g->ny) * g->nx;
-        hipLaunchKernelGGL(colsum_check_kernel, dim3(static_cast<unsigned>((ncol + 127) / 128)), dim3(128), 0, st,
-                           i_lcl, i_lnb, m_b, layer_mass, g->nz, ncol, colsum_err);
+        hipLaunchKernelGGL(colsum_check_kernel, dim3(static_cast<unsigned>((ncol + kColsPerBlock - 1) / kColsPerBlock)),
+                           dim3(64 * kColsPerBlock), 0, st, i_lcl, i_lnb, m_b, layer_mass, g->nz, ncol, colsum_err);
     }
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;

@@ -388,7 +388,9 @@ def test_reordering_leaves_every_particle_bit_identical(cfg0):
 
 def test_sort_pairs_matches_stable_numpy_sort():
     rng = np.random.default_rng(3)
-    for n, bits in ((1, 5), (4097, 13), (300_001, 22), (1 << 20, 32)):
+    # digit widths 8..11 bits and one to three passes
+    for n, bits in ((1, 5), (1000, 1), (4097, 8), (70_000, 9), (4097, 13), (150_000, 17), (921_600, 20),
+                    (300_001, 22), (1 << 20, 32)):
         keys = rng.integers(0, 2**bits, n, dtype=np.uint64).astype(np.uint32)
         keys[: n // 3] = keys[n // 2]  # many ties: stability matters
         vals = np.arange(n, dtype=np.uint32)
