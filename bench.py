@@ -45,10 +45,15 @@ def parse():
     ap.add_argument("--steps", type=int, default=1440)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--met", choices=("f32", "bf16"), default="f32")
-    ap.add_argument("--reorder-every", type=int, default=4,
+    ap.add_argument("--reorder-every", type=int, default=0,
                     help="sort the particle state by met cell every N steps (0 = never)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every step eagerly instead of replaying the captured step body")
+    ap.add_argument("--time-every", type=int, default=8,
+                    help="time the step kernel with HIP events on every Nth timed step (those steps "
+                         "launch eagerly; the others replay the captured step body)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-particles", type=int, default=32768)
+    ap.add_argument("--cpu-particles", type=int, default=65536)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a float32 copy kernel of this many MiB (reads and writes that "
@@ -118,20 +123,23 @@ def main() -> int:
     cfg = workload_config(WORKLOAD)
     met = SyntheticMet(cfg["spec"])
     met_dtype = L.GLIDE_MET_BF16 if args.met == "bf16" else L.GLIDE_MET_F32
-    eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=met_dtype, reorder_every=args.reorder_every)
+    stream = torch.cuda.Stream()  # default priority; graph capture needs a non-default stream
+    sptr = int(stream.cuda_stream)
+    torch.cuda.set_stream(stream)
+    eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=met_dtype, reorder_every=args.reorder_every,
+                      use_graph=not args.no_graph)
     n_sched = len(eng.sched)
     K = max(1, min(args.steps, n_sched))
     W = max(0, min(args.warmup, n_sched))
 
-    # warmup, then back to the release state
+    # warmup (captures the step body), then back to the release state
     eng.run(W)
     torch.cuda.synchronize()
     eng.reset()
     torch.cuda.synchronize()
 
-    stream = torch.cuda.current_stream()
-    sptr = int(stream.cuda_stream)
-    ev = eng.time_step_kernel(K, stream)  # HIP events around each step-kernel launch
+    every = max(1, args.time_every)
+    ev = eng.time_step_kernel(K, stream, every=every)  # HIP events around sampled step-kernel launches
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(K):
@@ -146,12 +154,13 @@ def main() -> int:
         active.append(eng.active_count(eng.sched[i]) - cum_esc)
         cum_esc += int(esc[i])
     psteps = int(sum(active))
-    step_ms = np.array([a.elapsed_time(b) for a, b in ev])
     act = np.array(active, dtype=np.float64)
-    busy = act > 0
-    avg_ms = float(step_ms[busy].mean()) if busy.any() else float(step_ms.mean())
-    bytes_launch = BYTES_PER_PARTICLE_STEP * float(act[busy].mean()) if busy.any() else 0.0
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    timed = [(i, a.elapsed_time(b)) for i, (a, b) in sorted(ev.items()) if act[i] > 0]
+    t_ms = np.array([t for _, t in timed])
+    t_act = np.array([act[i] for i, _ in timed])
+    avg_ms = float(t_ms.mean()) if len(t_ms) else float("nan")
+    bytes_launch = BYTES_PER_PARTICLE_STEP * float(t_act.mean()) if len(t_act) else 0.0
+    achieved = BYTES_PER_PARTICLE_STEP * float(t_act.sum()) / (float(t_ms.sum()) * 1e-3) / 1e9
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 2),
@@ -161,6 +170,8 @@ def main() -> int:
         "traffic": None,
         "kernel": "glide::step_kernel<false>" if met_dtype == L.GLIDE_MET_F32 else "glide::step_kernel<true>",
         "kernel_avg_us": round(avg_ms * 1e3, 3),
+        "kernel_launches_timed": len(t_ms),
+        "bytes_per_launch": round(bytes_launch),
         "bytes_per_particle_step": BYTES_PER_PARTICLE_STEP,
     }
     traffic_file = ROOT / "profiles" / f"traffic_{WORKLOAD}_{args.met}.json"
@@ -194,6 +205,7 @@ def main() -> int:
             "meander": True,
             "max_substeps": eng.h.max_substeps,
             "reorder_every": args.reorder_every,
+            "step_graph": not args.no_graph,
             "footprint": f"1x{cfg['grid']['n_t']}x1x{cfg['grid']['n_y']}x{cfg['grid']['n_x']} f32",
         },
         "active_particle_steps": psteps,

@@ -86,6 +86,10 @@ SIGNATURES = {
     "glide_last_error": (C.c_char_p, []),
     "glide_step": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), P(GlideWindow),
                              P(GlideFootprintGrid), P(GlideEmit), vp]),
+    "glide_step_table": (C.c_int, [P(GlideStepParams), vp, vp, P(GlideParticles), P(GlideGrid), P(GlideWindow),
+                                   P(GlideFootprintGrid), P(GlideEmit), vp]),
+    "glide_row_set": (C.c_int, [vp, C.c_int32, vp]),
+    "glide_row_advance": (C.c_int, [vp, vp]),
     "glide_footprint_keys": (C.c_int, [C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                        P(GlideFootprintGrid), vp, vp, vp]),
     "glide_footprint_workspace_bytes": (C.c_size_t, [C.c_int64]),

@@ -45,10 +45,11 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
-        const uint32_t lo0 = 0xD2511F53u * c0;
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        // one 32 x 32 -> 64-bit product each (v_mad_u64_u32)
+        const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+        const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+        const uint32_t lo0 = static_cast<uint32_t>(p0), hi0 = static_cast<uint32_t>(p0 >> 32);
+        const uint32_t lo1 = static_cast<uint32_t>(p1), hi1 = static_cast<uint32_t>(p1 >> 32);
         const uint32_t n0 = hi1 ^ c1 ^ k0;
         const uint32_t n2 = hi0 ^ c3 ^ k1;
         c0 = n0;
@@ -72,6 +73,17 @@ __device__ __forceinline__ void box_muller(float ua, float ub, float& n0, float&
     n1 = r * __builtin_amdgcn_sinf(ub);
 }
 
+// Element access at a 32-bit byte offset from a uniform base pointer (arrays
+// below 4 GiB): keeps per-lane addresses 32-bit.
+template <typename T>
+__device__ __forceinline__ T ldo(const T* p, uint32_t off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + off);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T* p, uint32_t off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + off) = v;
+}
+
 // ---------------------------------------------------------------------------
 // bfloat16 <-> float (round to nearest even)
 // ---------------------------------------------------------------------------
@@ -93,6 +105,20 @@ template <>
 struct Rec8<false> {
     using Store = float4;
     static constexpr int kStride = 2;  // float4 per record
+    static constexpr uint32_t kBytes = 32;
+    // Loads at a 32-bit byte offset from the (uniform) base: the scalar-base,
+    // vector-offset addressing form, no 64-bit address arithmetic per lane.
+    __device__ __forceinline__ static void load_at(const void* base, uint32_t off, float (&v)[8]) {
+        const float4* p = reinterpret_cast<const float4*>(static_cast<const char*>(base) + off);
+        const float4 a = p[0];
+        const float4 b = p[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    }
+    __device__ __forceinline__ static void load2_at(const void* base, uint32_t off, float (&v)[2]) {
+        const float2 a = *reinterpret_cast<const float2*>(static_cast<const char*>(base) + off);
+        v[0] = a.x; v[1] = a.y;
+    }
     __device__ __forceinline__ static void load(const void* base, int64_t cell, float (&v)[8]) {
         const float4* p = static_cast<const float4*>(base) + cell * 2;
         const float4 a = p[0];
@@ -117,6 +143,18 @@ struct Rec8<false> {
 
 template <>
 struct Rec8<true> {
+    static constexpr uint32_t kBytes = 16;
+    __device__ __forceinline__ static void load_at(const void* base, uint32_t off, float (&v)[8]) {
+        const uint4 a = *reinterpret_cast<const uint4*>(static_cast<const char*>(base) + off);
+        v[0] = bf16_lo(a.x); v[1] = bf16_hi(a.x);
+        v[2] = bf16_lo(a.y); v[3] = bf16_hi(a.y);
+        v[4] = bf16_lo(a.z); v[5] = bf16_hi(a.z);
+        v[6] = bf16_lo(a.w); v[7] = bf16_hi(a.w);
+    }
+    __device__ __forceinline__ static void load2_at(const void* base, uint32_t off, float (&v)[2]) {
+        const uint32_t a = *reinterpret_cast<const uint32_t*>(static_cast<const char*>(base) + off);
+        v[0] = bf16_lo(a); v[1] = bf16_hi(a);
+    }
     __device__ __forceinline__ static void load(const void* base, int64_t cell, float (&v)[8]) {
         const uint4 a = static_cast<const uint4*>(base)[cell];
         v[0] = bf16_lo(a.x); v[1] = bf16_hi(a.x);
@@ -160,6 +198,26 @@ __device__ __forceinline__ void load_surf(const void* base, int64_t cell, float 
     }
 }
 
+// Surface record at a byte offset (48 B float, 24 B bfloat16).
+template <bool BF16>
+__device__ __forceinline__ void load_surf_at(const void* base, uint32_t off, float (&v)[10]) {
+    const char* b = static_cast<const char*>(base) + off;
+    if constexpr (!BF16) {
+        const float4* p = reinterpret_cast<const float4*>(b);
+        const float4 a = p[0], c1 = p[1], c2 = p[2];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = c1.x;
+        v[5] = c1.y; v[6] = c1.z; v[7] = c1.w; v[8] = c2.x; v[9] = c2.y;
+    } else {
+        const uint2* p = reinterpret_cast<const uint2*>(b);
+        const uint2 a = p[0], c1 = p[1], c2 = p[2];
+        v[0] = bf16_lo(a.x); v[1] = bf16_hi(a.x); v[2] = bf16_lo(a.y); v[3] = bf16_hi(a.y);
+        v[4] = bf16_lo(c1.x); v[5] = bf16_hi(c1.x); v[6] = bf16_lo(c1.y); v[7] = bf16_hi(c1.y);
+        v[8] = bf16_lo(c2.x); v[9] = bf16_hi(c2.x);
+    }
+}
+template <bool BF16>
+constexpr uint32_t kSurfBytes = BF16 ? 24u : 48u;
+
 // torch.lerp(start, end, weight) with a scalar weight: two-sided form.
 __device__ __forceinline__ float torch_lerp(float a, float b, float w) {
     return (fabsf(w) < 0.5f) ? a + w * (b - a) : b - (b - a) * (1.0f - w);
@@ -199,39 +257,41 @@ __device__ __forceinline__ float level_index(const float* __restrict__ lev, int 
     return static_cast<float>(up - 1) + fr;
 }
 
+// Byte offsets and weights of the 8 trilinear corners (order x fastest, then
+// y, then z: 000 001 010 011 100 101 110 111).
 struct Corner3 {
-    int64_t c000, c001, c010, c011, c100, c101, c110, c111;
-    float w000, w001, w010, w011, w100, w101, w110, w111;
+    uint32_t o[8];
+    float w[8];
 };
 
-// Trilinear corners at fractional indices (ix, iy, iz) (already clamped).
-__device__ __forceinline__ Corner3 corners3(float ix, float iy, float iz, int nx, int ny, int nz) {
+// Trilinear corners at fractional indices (ix, iy, iz) (already clamped) of a
+// [nz][ny][nx] array of rb-byte records (the whole array below 4 GiB).
+__device__ __forceinline__ Corner3 corners3(float ix, float iy, float iz, int nx, int ny, int nz, uint32_t rb) {
     const float fx0 = floorf(ix), fy0 = floorf(iy), fz0 = floorf(iz);
     const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0), z0 = static_cast<int>(fz0);
     const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1), z1 = min(z0 + 1, nz - 1);
     const float fx = ix - fx0, fy = iy - fy0, fz = iz - fz0;
     const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
-    const int64_t sz = static_cast<int64_t>(ny) * nx;
-    const int64_t b0 = z0 * sz, b1 = z1 * sz;
-    const int64_t r0 = static_cast<int64_t>(y0) * nx, r1 = static_cast<int64_t>(y1) * nx;
+    const uint32_t row = static_cast<uint32_t>(nx) * rb;
+    const uint32_t lay = static_cast<uint32_t>(ny) * row;
+    const uint32_t dx = static_cast<uint32_t>(x1 - x0) * rb;
+    const uint32_t dy = static_cast<uint32_t>(y1 - y0) * row;
+    const uint32_t dz = static_cast<uint32_t>(z1 - z0) * lay;
+    const uint32_t o = static_cast<uint32_t>(z0) * lay + static_cast<uint32_t>(y0) * row + static_cast<uint32_t>(x0) * rb;
     Corner3 c;
-    c.c000 = b0 + r0 + x0; c.c001 = b0 + r0 + x1;
-    c.c010 = b0 + r1 + x0; c.c011 = b0 + r1 + x1;
-    c.c100 = b1 + r0 + x0; c.c101 = b1 + r0 + x1;
-    c.c110 = b1 + r1 + x0; c.c111 = b1 + r1 + x1;
-    c.w000 = gx * gy * gz; c.w001 = fx * gy * gz;
-    c.w010 = gx * fy * gz; c.w011 = fx * fy * gz;
-    c.w100 = gx * gy * fz; c.w101 = fx * gy * fz;
-    c.w110 = gx * fy * fz; c.w111 = fx * fy * fz;
+    c.o[0] = o;
+    c.o[1] = o + dx;
+    c.o[2] = o + dy;
+    c.o[3] = o + dy + dx;
+    c.o[4] = o + dz;
+    c.o[5] = o + dz + dx;
+    c.o[6] = o + dz + dy;
+    c.o[7] = o + dz + dy + dx;
+    c.w[0] = gx * gy * gz; c.w[1] = fx * gy * gz;
+    c.w[2] = gx * fy * gz; c.w[3] = fx * fy * gz;
+    c.w[4] = gx * gy * fz; c.w[5] = fx * gy * fz;
+    c.w[6] = gx * fy * fz; c.w[7] = fx * fy * fz;
     return c;
-}
-
-template <bool BF16, int K>
-__device__ __forceinline__ void accum8(const void* base, int64_t cell, float w, float (&acc)[K]) {
-    float v[8];
-    Rec8<BF16>::load(base, cell, v);
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] += w * v[k];
 }
 
 // Trilinear sample of the first K (<= 8) channels of a record field.
@@ -239,30 +299,27 @@ template <bool BF16, int K>
 __device__ __forceinline__ void sample_rec(const void* base, const Corner3& c, float (&acc)[K]) {
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0f;
-    accum8<BF16, K>(base, c.c000, c.w000, acc);
-    accum8<BF16, K>(base, c.c001, c.w001, acc);
-    accum8<BF16, K>(base, c.c010, c.w010, acc);
-    accum8<BF16, K>(base, c.c011, c.w011, acc);
-    accum8<BF16, K>(base, c.c100, c.w100, acc);
-    accum8<BF16, K>(base, c.c101, c.w101, acc);
-    accum8<BF16, K>(base, c.c110, c.w110, acc);
-    accum8<BF16, K>(base, c.c111, c.w111, acc);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        float v[8];
+        Rec8<BF16>::load_at(base, c.o[q], v);
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += c.w[q] * v[k];
+    }
 }
 
 // Trilinear sample of channels 0..1 only (the free-troposphere pair).
 template <bool BF16>
 __device__ __forceinline__ void sample_rec2(const void* base, const Corner3& c, float& a0, float& a1) {
-    float v[2];
     a0 = 0.0f;
     a1 = 0.0f;
-#define GLIDE_ACC2(cell, w)                  \
-    Rec8<BF16>::load2(base, (cell), v);      \
-    a0 += (w) * v[0];                        \
-    a1 += (w) * v[1];
-    GLIDE_ACC2(c.c000, c.w000) GLIDE_ACC2(c.c001, c.w001) GLIDE_ACC2(c.c010, c.w010)
-    GLIDE_ACC2(c.c011, c.w011) GLIDE_ACC2(c.c100, c.w100) GLIDE_ACC2(c.c101, c.w101)
-    GLIDE_ACC2(c.c110, c.w110) GLIDE_ACC2(c.c111, c.w111)
-#undef GLIDE_ACC2
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        float v[2];
+        Rec8<BF16>::load2_at(base, c.o[q], v);
+        a0 += c.w[q] * v[0];
+        a1 += c.w[q] * v[1];
+    }
 }
 
 }  // namespace glide

@@ -243,7 +243,9 @@ __global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
 // Mass conservation per column in float64: the float entries
 // f[i][j] = scale (e_i d_j / M_b + [i == j+1] phi_j), summed in float64 along
 // rows and columns, give max_i |row_i - col_i| / m_i (non-divergence,
-// emanuel.py:437-446). One wavefront per column; lane i takes rows i, i+64.
+// emanuel.py:437-446). One wavefront per column. The cloud profile is the one
+// make_cloud builds (same per-level terms, sums in the same order), with the
+// per-level terms spread over the lanes and the running sums on lane 0.
 constexpr int kColsPerBlock = 4;
 
 __global__ __launch_bounds__(64 * kColsPerBlock) void colsum_check_kernel(const int32_t* i_lcl, const int32_t* i_lnb,
@@ -253,8 +255,9 @@ __global__ __launch_bounds__(64 * kColsPerBlock) void colsum_check_kernel(const 
     __shared__ float es[kColsPerBlock][kConvLevels];
     __shared__ float dsh[kColsPerBlock][kConvLevels];
     __shared__ float ph[kColsPerBlock][kConvLevels];
+    __shared__ float rowv[kColsPerBlock][kConvLevels];
+    __shared__ float totals[kColsPerBlock][3];  // bl_total, w_total, sum_d
     for (int k = threadIdx.x; k < nz; k += blockDim.x) lm[k] = lm_g[k];
-    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t col = static_cast<int64_t>(blockIdx.x) * kColsPerBlock + wave;
     const bool have = col < ncol;
@@ -266,28 +269,62 @@ __global__ __launch_bounds__(64 * kColsPerBlock) void colsum_check_kernel(const 
         mb = m_b[col];
     }
     const bool cloud = have && lcl >= 0 && lnb > lcl && mb > 0.0f;
-    Cloud c{};
+    Cloud c{lcl, lnb, mb, 0.0f, 0.0f, 1.0f};
+    float* e = es[wave];
+    float* d = dsh[wave];
+    float* phi = ph[wave];
+    float* rv = rowv[wave];
     if (cloud) {
-        c = make_cloud(lcl, lnb, mb, lm, nz);
+        for (int k = lane; k < nz; k += 64) d[k] = cloud_w(c, k);  // detrainment shape first
+    }
+    __syncthreads();
+    if (cloud && lane == 0) {
+        float bl = 0.0f;
+        for (int i = 0; i < lcl; ++i) bl += lm[i];
+        float wt = 0.0f;
+        for (int k = lcl; k <= lnb; ++k) wt += d[k];
+        totals[wave][0] = fmaxf(bl, 1e-12f);
+        totals[wave][1] = fmaxf(wt, 1e-12f);
+    }
+    __syncthreads();
+    if (cloud) {
+        c.bl_total = totals[wave][0];
+        c.w_total = totals[wave][1];
         for (int k = lane; k < nz; k += 64) {
-            es[wave][k] = cloud_e(c, lm, k);
-            dsh[wave][k] = cloud_d(c, k);
+            e[k] = cloud_e(c, lm, k);
+            d[k] = cloud_d(c, k);
         }
-        if (lane == 0) {
-            float ce = 0.0f, cd = 0.0f;
-            for (int k = 0; k < nz; ++k) {
-                ce += cloud_e(c, lm, k);
-                cd += cloud_d(c, k);
-                ph[wave][k] = (k < nz - 1) ? fmaxf(ce - cd, 0.0f) : 0.0f;
-            }
+    }
+    __syncthreads();
+    if (cloud && lane == 0) {
+        float sum_d = 0.0f;
+        for (int k = 0; k < nz; ++k) sum_d += d[k];
+        totals[wave][2] = sum_d;
+    }
+    __syncthreads();
+    if (cloud) {
+        const float sum_d = totals[wave][2];
+        for (int k = lane; k < nz; k += 64) rv[k] = e[k] * sum_d / mb;
+    }
+    __syncthreads();
+    if (cloud && lane == 0) {
+        float ce = 0.0f, cd = 0.0f, phi_prev = 0.0f;
+        for (int i = 0; i < nz; ++i) {
+            rv[i] = rv[i] + phi_prev;  // unscaled row sum of row i
+            ce += e[i];
+            cd += d[i];
+            phi_prev = (i < nz - 1) ? fmaxf(ce - cd, 0.0f) : 0.0f;
+            phi[i] = phi_prev;
         }
     }
     __syncthreads();
     double worst = 0.0;
     if (cloud) {
-        const float* e = es[wave];
-        const float* d = dsh[wave];
-        const float* phi = ph[wave];
+        // CFL cap on the per-layer outflux fraction (emanuel.py:500-504).
+        float mx = 0.0f;
+        for (int i = lane; i < nz; i += 64) mx = fmaxf(mx, rv[i] / fmaxf(lm[i], 1e-12f));
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+        if (mx > 0.9f) c.scale = 0.9f / mx;
         for (int i = lane; i < nz; i += 64) {
             double row = 0.0, cols = 0.0;
             for (int j = 0; j < nz; ++j) {

@@ -31,10 +31,29 @@ struct StepArgs {
     const void* surf;
     FpDev fp;
     GlideEmit e;
+    // Table mode: the step's scalars come from table[*row] on the device and
+    // the escape count goes to e.escaped[table[*row].step_index].
+    const GlideStepParams* table;
+    const int32_t* row;
 };
 
 constexpr int kMaxLevels = 128;
 constexpr int kStepBlock = 256;
+
+// The step's row of the table, read into scalar registers (one readfirstlane
+// per 32-bit word), so everything derived from it stays scalar.
+__device__ __forceinline__ GlideStepParams load_row(const GlideStepParams* table, const int32_t* row) {
+    static_assert(sizeof(GlideStepParams) % 4 == 0, "GlideStepParams is a whole number of words");
+    constexpr int kWords = sizeof(GlideStepParams) / 4;
+    const int r = __builtin_amdgcn_readfirstlane(*row);
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(table + r);
+    uint32_t buf[kWords];
+#pragma unroll
+    for (int k = 0; k < kWords; ++k) buf[k] = __builtin_amdgcn_readfirstlane(w[k]);
+    GlideStepParams p;
+    __builtin_memcpy(&p, buf, sizeof(p));
+    return p;
+}
 
 template <bool BF16>
 __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
@@ -44,30 +63,39 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
 
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.s.n) return;
+    // 32-bit byte offsets of this particle (the host checks the arrays fit)
+    const uint32_t i1 = static_cast<uint32_t>(i);
+    const uint32_t i4 = i1 * 4u;
+    const uint32_t ip = static_cast<uint32_t>(i * a.s.pos_stride) * 4u;
     const GlideParticles& s = a.s;
-    const GlideStepParams& p = a.p;
+    GlideStepParams p = a.p;
+    unsigned long long* escaped = a.e.escaped;
+    if (a.table != nullptr) {
+        p = load_row(a.table, a.row);
+        escaped += p.step_index;
+    }
     const bool emit = (p.flags & GLIDE_F_EMIT) != 0;
     const float co = static_cast<float>(p.cursor_off);
 
     bool active;
     if (p.flags & GLIDE_F_MASK_INPUT) {
-        active = s.active[i] != 0;
+        active = ldo(s.active, i1) != 0;
     } else {
-        const float ro = s.r_off[i];
-        active = (ro >= co) && (ro - static_cast<float>(p.length_s) <= co) && (s.alive[i] != 0);
+        const float ro = ldo(s.r_off, i4);
+        active = (ro >= co) && (ro - static_cast<float>(p.length_s) <= co) && (ldo(s.alive, i1) != 0);
     }
     if (!active) {
         if (emit) {
-            a.e.keys[i] = a.fp.n_cells;
-            a.e.vals[i] = 0.0f;
+            sto(a.e.keys, i4, a.fp.n_cells);
+            sto(a.e.vals, i4, 0.0f);
         }
         return;
     }
 
     const int64_t ps = s.pos_stride;
-    float lon = s.lon[i * ps];
-    float lat = s.lat[i * ps];
-    float z = s.z[i * ps];
+    float lon = ldo(s.lon, ip);
+    float lat = ldo(s.lat, ip);
+    float z = ldo(s.z, ip);
     const float dt = static_cast<float>(p.dt);
     const float al = static_cast<float>(p.alpha);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -77,7 +105,7 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
         const float ix = frac_index(lo, a.lon0, a.sx, nx);
         const float iy = frac_index(la, a.lat0, a.sy, ny);
         const float iz = level_index(lev, nz, zz);
-        const Corner3 c = corners3(ix, iy, iz, nx, ny, nz);
+        const Corner3 c = corners3(ix, iy, iz, nx, ny, nz, Rec8<BF16>::kBytes);
         float v[6];
         sample_rec<BF16, 6>(a.wind, c, v);
         const float u = v[0] * (1.0f - al) + v[3] * al;
@@ -98,9 +126,9 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
         z = z - dt * e2;
     }
 
-    float up = s.up[i], vp = s.vp[i], wp = s.wp[i];
+    float up = ldo(s.up, i4), vp = ldo(s.vp, i4), wp = ldo(s.wp, i4);
     const bool meander = (p.flags & GLIDE_F_MEANDER) != 0;
-    float um = meander ? s.um[i] : 0.0f, vm = meander ? s.vm[i] : 0.0f;
+    float um = meander ? ldo(s.um, i4) : 0.0f, vm = meander ? ldo(s.vm, i4) : 0.0f;
 
     if (!(p.flags & GLIDE_F_NO_TURB)) {
         // ---- surface fields at the post-advection position ----
@@ -116,13 +144,14 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
             const float w00 = (1.0f - fx) * (1.0f - fy), w01 = fx * (1.0f - fy);
             const float w10 = (1.0f - fx) * fy, w11 = fx * fy;
             float acc[5] = {0, 0, 0, 0, 0};
-            const int64_t cells[4] = {static_cast<int64_t>(y0) * nx + x0, static_cast<int64_t>(y0) * nx + x1,
-                                      static_cast<int64_t>(y1) * nx + x0, static_cast<int64_t>(y1) * nx + x1};
+            constexpr uint32_t sb = kSurfBytes<BF16>;
+            const uint32_t r0 = static_cast<uint32_t>(y0 * nx), r1 = static_cast<uint32_t>(y1 * nx);
+            const uint32_t offs[4] = {(r0 + x0) * sb, (r0 + x1) * sb, (r1 + x0) * sb, (r1 + x1) * sb};
             const float ws[4] = {w00, w01, w10, w11};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 float r[10];
-                load_surf<BF16>(a.surf, cells[q], r);
+                load_surf_at<BF16>(a.surf, offs[q], r);
 #pragma unroll
                 for (int ch = 0; ch < 5; ++ch) acc[ch] += ws[q] * torch_lerp(r[ch], r[5 + ch], al);
             }
@@ -149,7 +178,7 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
         // aux record at z_eval: FT pair, density pair, meander pair.
         float ax[6];
         {
-            const Corner3 c = corners3(ix, iy, level_index(lev, nz, z_eval), nx, ny, nz);
+            const Corner3 c = corners3(ix, iy, level_index(lev, nz, z_eval), nx, ny, nz, Rec8<BF16>::kBytes);
             sample_rec<BF16, 6>(a.aux, c, ax);
         }
 
@@ -187,7 +216,7 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
             float sw = in_bl_sw(zq, sf);
             if (sl_over && zq < 0.1f * sf.blh) sw = surface_layer_sw(zq, sf.ustar, sf.L);
             if (zq > sf.blh) {
-                const Corner3 c = corners3(ix, iy, level_index(lev, nz, zq), nx, ny, nz);
+                const Corner3 c = corners3(ix, iy, level_index(lev, nz, zq), nx, ny, nz, Rec8<BF16>::kBytes);
                 float f0, f1;
                 sample_rec2<BF16>(a.aux, c, f0, f1);
                 sw = f0;
@@ -214,7 +243,7 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
 
         const float su2 = t.su * t.su, sv2 = t.sv * t.sv, sw2 = t.sw * t.sw;
         const float umax = 4.0f * t.su, vmax = 4.0f * t.sv, wmax = 4.0f * t.sw;
-        const uint32_t uid = s.uid ? s.uid[i] : static_cast<uint32_t>(i);
+        const uint32_t uid = s.uid ? ldo(s.uid, i4) : static_cast<uint32_t>(i);
         const uint32_t k0 = static_cast<uint32_t>(p.seed), k1 = static_cast<uint32_t>(p.seed >> 32);
         const uint32_t step = static_cast<uint32_t>(p.step_index);
         for (int q = 0; q < k; ++q) {
@@ -257,25 +286,25 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
         }
     }
 
-    s.lon[i * ps] = lon;
-    s.lat[i * ps] = lat;
-    s.z[i * ps] = z;
-    s.up[i] = up;
-    s.vp[i] = vp;
-    s.wp[i] = wp;
+    sto(s.lon, ip, lon);
+    sto(s.lat, ip, lat);
+    sto(s.z, ip, z);
+    sto(s.up, i4, up);
+    sto(s.vp, i4, vp);
+    sto(s.wp, i4, wp);
     if (meander) {
-        s.um[i] = um;
-        s.vm[i] = vm;
+        sto(s.um, i4, um);
+        sto(s.vm, i4, vm);
     }
 
     if (emit) {
-        const int64_t tb = time_bin(s.we_off[i], co, a.fp.n_t);
-        const int64_t rb = s.rel_idx ? s.rel_idx[i] : 0;
-        a.e.keys[i] = footprint_key(a.fp, true, lon, lat, z, tb, rb);
-        a.e.vals[i] = s.weight[i] * dt;
+        const int64_t tb = time_bin(ldo(s.we_off, i4), co, a.fp.n_t);
+        const int64_t rb = s.rel_idx ? ldo(s.rel_idx, i4) : 0;
+        sto(a.e.keys, i4, footprint_key(a.fp, true, lon, lat, z, tb, rb));
+        sto(a.e.vals, i4, ldo(s.weight, i4) * dt);
         const bool esc = !within_domain(a.fp, lon, lat, z);
-        if (esc) s.alive[i] = 0;
-        count_escapes(esc, a.e.escaped);
+        if (esc) sto(s.alive, i1, static_cast<uint8_t>(0));
+        count_escapes(esc, escaped);
     }
 }
 
@@ -297,6 +326,8 @@ static int fill_grid(StepArgs& a, const GlideGrid& g, const GlideWindow& w) {
     GLIDE_REQUIRE(g.lon1 != g.lon0 && g.lat1 != g.lat0, "Grid dimension has 0 span (first == last)");
     GLIDE_REQUIRE(g.levels != nullptr, "levels pointer is NULL");
     GLIDE_REQUIRE(w.wind && w.aux && w.surf, "met window pointers are NULL");
+    GLIDE_REQUIRE(static_cast<uint64_t>(g.nz) * g.ny * g.nx * 32u < (1ull << 32),
+                  "met window too large for 32-bit record offsets (%d x %d x %d)", g.nz, g.ny, g.nx);
     a.nx = g.nx;
     a.ny = g.ny;
     a.nz = g.nz;
@@ -319,13 +350,15 @@ static int fill_grid(StepArgs& a, const GlideGrid& g, const GlideWindow& w) {
 
 using namespace glide;
 
-extern "C" int glide_step(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
-                          const GlideWindow* w, const GlideFootprintGrid* fg, const GlideEmit* e,
-                          void* stream) {
+static int launch_step(const GlideStepParams* p, const GlideStepParams* table, const int32_t* row,
+                       const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
+                       const GlideFootprintGrid* fg, const GlideEmit* e, void* stream) {
     GLIDE_REQUIRE(p && s && g && w, "glide_step: NULL argument");
     GLIDE_REQUIRE(s->n >= 0, "n must be >= 0");
     GLIDE_REQUIRE(s->lon && s->lat && s->z && s->up && s->vp && s->wp, "particle arrays are NULL");
     GLIDE_REQUIRE(s->pos_stride >= 1, "pos_stride must be >= 1");
+    GLIDE_REQUIRE(static_cast<uint64_t>(s->n) * static_cast<uint64_t>(s->pos_stride) * 4u < (1ull << 32),
+                  "particle arrays too large for 32-bit offsets");
     GLIDE_REQUIRE(p->max_substeps >= 1, "max_substeps must be >= 1");
     GLIDE_REQUIRE(p->substep_c > 0, "substep_c must be > 0");
     GLIDE_REQUIRE(p->dt >= 0, "dt must be >= 0");
@@ -339,6 +372,8 @@ extern "C" int glide_step(const GlideStepParams* p, const GlideParticles* s, con
     StepArgs a{};
     a.p = *p;
     a.s = *s;
+    a.table = table;
+    a.row = row;
     int rc = fill_grid(a, *g, *w);
     if (rc) return rc;
     if (p->flags & GLIDE_F_EMIT) {
@@ -361,6 +396,35 @@ extern "C" int glide_step(const GlideStepParams* p, const GlideParticles* s, con
     } else {
         hipLaunchKernelGGL(step_kernel<false>, dim3(static_cast<unsigned>(blocks)), dim3(kStepBlock), 0, st, a);
     }
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}
+
+extern "C" int glide_step(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
+                          const GlideWindow* w, const GlideFootprintGrid* fg, const GlideEmit* e,
+                          void* stream) {
+    return launch_step(p, nullptr, nullptr, s, g, w, fg, e, stream);
+}
+
+extern "C" int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, const int32_t* row,
+                                const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
+                                const GlideFootprintGrid* fg, const GlideEmit* e, void* stream) {
+    GLIDE_REQUIRE(table != nullptr && row != nullptr, "glide_step_table: NULL table/row");
+    return launch_step(tmpl, table, row, s, g, w, fg, e, stream);
+}
+
+__global__ void row_kernel(int32_t* row, int32_t value, int32_t add) { *row = add ? *row + value : value; }
+
+extern "C" int glide_row_set(int32_t* row, int32_t value, void* stream) {
+    GLIDE_REQUIRE(row != nullptr, "NULL row");
+    hipLaunchKernelGGL(row_kernel, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), row, value, 0);
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}
+
+extern "C" int glide_row_advance(int32_t* row, void* stream) {
+    GLIDE_REQUIRE(row != nullptr, "NULL row");
+    hipLaunchKernelGGL(row_kernel, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), row, 1, 1);
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
 }

@@ -152,9 +152,12 @@ class BatchEngine:
     def __init__(self, cfg: dict, *, device="cuda", met: SyntheticMet | None = None,
                  met_dtype: int = L.GLIDE_MET_F32, hanna: HannaOptions | None = None,
                  convection: ConvectionOptions | None = None, compensated: bool = False,
-                 device_met: DeviceMet | None = None, reorder_every: int = 0):
+                 device_met: DeviceMet | None = None, reorder_every: int = 0, use_graph: bool = False):
         self.cfg = cfg
         self.reorder_every = int(reorder_every)
+        self.use_graph = bool(use_graph)
+        self._graphs: dict[int, int] = {}
+        self.graph_launches = 0
         self.device = device
         self.h = hanna or HannaOptions()
         self.conv_opts = convection if convection is not None else (
@@ -220,6 +223,14 @@ class BatchEngine:
         self.m_b = torch.empty(nc, dtype=f32, device=d)
         self.layer_mass = torch.empty(self.met.nz, dtype=f32, device=d)
         self.colsum_err = torch.zeros(nc, dtype=torch.float64, device=d)
+        # Step table for the captured step body: one GlideStepParams row per
+        # schedule step (the flags of a plain emitting step), and the device
+        # cursor that the graph advances.
+        flags = self.h.flags() | L.F_EMIT
+        rows = (L.GlideStepParams * max(len(self.sched), 1))(*[self.step_params(st, flags) for st in self.sched])
+        self.step_table = torch.frombuffer(bytearray(bytes(rows)), dtype=torch.uint8).to(d)
+        self.step_tmpl = self.step_params(self.sched[0], flags) if self.sched else None
+        self.row = torch.zeros(1, dtype=torch.int32, device=d)
 
     def _make_parts(self, b) -> L.GlideParticles:
         return L.GlideParticles(
@@ -260,6 +271,7 @@ class BatchEngine:
         self.step_i = 0
         self.last_bracket = None
         self.step_events = None
+        self.row.zero_()
 
     def reorder(self, st, stream: int) -> None:
         """Sort the particle state by met cell (gather locality); particles that
@@ -289,6 +301,46 @@ class BatchEngine:
         esc = self.escaped.data_ptr() + 8 * st.index
         return L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(), escaped=esc)
 
+    def _capture(self, stream: int) -> int:
+        """Capture the plain step body (table-driven step, footprint
+        accumulation, cursor advance) for the current particle buffer."""
+
+        lib = L.lib()
+        if stream == 0:
+            raise ValueError("graph capture needs a non-default stream")
+        em = L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(), escaped=self.escaped.data_ptr())
+        L.check(lib.glide_graph_begin(stream))
+        try:
+            L.check(lib.glide_step_table(C.byref(self.step_tmpl), self.step_table.data_ptr(), self.row.data_ptr(),
+                                         C.byref(self.parts), C.byref(self.met.grid), C.byref(self.met.window),
+                                         C.byref(self.fg), C.byref(em), stream))
+            self._accumulate(stream)
+            L.check(lib.glide_row_advance(self.row.data_ptr(), stream))
+        finally:
+            h = C.c_void_p()
+            rc = lib.glide_graph_end(stream, C.byref(h))
+        L.check(rc)
+        return h.value
+
+    def _accumulate(self, stream: int) -> None:
+        L.check(L.lib().glide_footprint_accumulate(
+            self.keys.data_ptr(), self.vals.data_ptr(), self.n, self.n_cells, self.cells_per_release,
+            self.footprint.data_ptr(), self.comp.data_ptr() if self.comp is not None else None,
+            self.release_total.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(), stream))
+
+    def close(self) -> None:
+        """Free the captured graphs."""
+
+        for h in self._graphs.values():
+            L.lib().glide_graph_destroy(h)
+        self._graphs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
     def run_step(self, stream: int | None = None) -> None:
         """Advance one step of the schedule."""
 
@@ -301,10 +353,19 @@ class BatchEngine:
         new_bracket = st.bracket_hour != self.last_bracket
         self.met.build_window(st.bracket_hour, stream)
         convect = new_bracket and self.conv_opts is not None
+        ev = self.step_events.get(self.step_i) if self.step_events is not None else None
+        if self.use_graph and not convect and ev is None:
+            g = self._graphs.get(self.cur)
+            if g is None:
+                g = self._graphs[self.cur] = self._capture(stream)
+            L.check(lib.glide_graph_launch(g, stream))
+            self.graph_launches += 1
+            self.last_bracket = st.bracket_hour
+            self.step_i += 1
+            return
         flags = self.h.flags() | (0 if convect else L.F_EMIT)
         p = self.step_params(st, flags)
         em = self.emit(st)
-        ev = self.step_events[self.step_i] if self.step_events is not None else None
         if ev is not None:
             ev[0].record(self._torch_stream)
         L.check(lib.glide_step(C.byref(p), C.byref(self.parts), C.byref(self.met.grid),
@@ -328,21 +389,22 @@ class BatchEngine:
                                       self.i_lcl.data_ptr(), self.i_lnb.data_ptr(), self.m_b.data_ptr(),
                                       self.layer_mass.data_ptr(), self.colsum_err.data_ptr(),
                                       C.byref(self.fg), C.byref(em), stream))
-        L.check(lib.glide_footprint_accumulate(
-            self.keys.data_ptr(), self.vals.data_ptr(), self.n, self.n_cells, self.cells_per_release,
-            self.footprint.data_ptr(), self.comp.data_ptr() if self.comp is not None else None,
-            self.release_total.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(), stream))
+        self._accumulate(stream)
         self.last_bracket = st.bracket_hour
         self.step_i += 1
+        if self.use_graph:
+            L.check(lib.glide_row_set(self.row.data_ptr(), self.step_i, stream))
 
-    def time_step_kernel(self, n_steps: int, stream) -> list:
-        """Record HIP events around the next ``n_steps`` step-kernel launches on
-        ``stream`` (a torch stream, the one the kernels are launched on)."""
+    def time_step_kernel(self, n_steps: int, stream, every: int = 1) -> dict:
+        """Record HIP events around every ``every``-th of the next ``n_steps``
+        step-kernel launches on ``stream`` (a torch stream, the one the kernels
+        are launched on). Those steps launch eagerly. Returns {offset from the
+        current step: (start, end)}."""
 
         self._torch_stream = stream
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-               for _ in range(n_steps)]
-        self.step_events = {self.step_i + i: e for i, e in enumerate(evs)}
+        evs = {i: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for i in range(0, n_steps, max(1, every))}
+        self.step_events = {self.step_i + i: e for i, e in evs.items()}
         return evs
 
     def run(self, n_steps: int | None = None) -> None:

@@ -163,6 +163,74 @@ def mass_flux_matrix(m, i_lcl, i_lnb, m_b):
     return f
 
 
+def mass_flux_matrix_f32(m, i_lcl, i_lnb, m_b):
+    """mass_flux_matrix (emanuel.py:429-506) in float32 arithmetic, as the
+    reference computes it on float32 met: every term rounded to float32, sums
+    taken sequentially in level order. Returns the float32 entries."""
+
+    f32 = np.float32
+    m = np.asarray(m, dtype=np.float32)
+    n = m.size
+    out = np.zeros((n, n), dtype=np.float32)
+    mb = f32(m_b)
+    if i_lcl < 0 or i_lnb <= i_lcl or mb <= 0:
+        return out
+    bl = f32(0)
+    for i in range(i_lcl):
+        bl = f32(bl + m[i])
+    bl = max(bl, f32(1e-12))
+    depth = f32(i_lnb - i_lcl)
+    w = np.zeros(n, dtype=np.float32)
+    for k in range(i_lcl, i_lnb + 1):
+        w[k] = f32(1) - f32(k - i_lcl) / depth
+    wt = f32(0)
+    for k in range(i_lcl, i_lnb + 1):
+        wt = f32(wt + w[k])
+    wt = max(wt, f32(1e-12))
+    e = np.zeros(n, dtype=np.float32)
+    for i in range(i_lcl):
+        e[i] = f32(mb * m[i]) / bl
+    d = np.array([f32(mb * w[k]) / wt for k in range(n)], dtype=np.float32)
+    sum_d = f32(0)
+    for k in range(n):
+        sum_d = f32(sum_d + d[k])
+    phi = np.zeros(n, dtype=np.float32)
+    ce = cd = f32(0)
+    mx = f32(0)
+    prev = f32(0)
+    for i in range(n):
+        row = f32(f32(e[i] * sum_d) / mb) + prev
+        mx = max(mx, f32(row / max(m[i], f32(1e-12))))
+        ce = f32(ce + e[i])
+        cd = f32(cd + d[i])
+        prev = max(f32(ce - cd), f32(0)) if i < n - 1 else f32(0)
+        phi[i] = prev
+    scale = f32(CFL_MAX_OUTFLUX_FRAC) / mx if mx > f32(CFL_MAX_OUTFLUX_FRAC) else f32(1)
+    for i in range(n):
+        for j in range(n):
+            v = f32(f32(e[i] * d[j]) / mb)
+            if i == j + 1:
+                v = f32(v + phi[j])
+            out[i, j] = f32(scale * v)
+    return out
+
+
+def column_sum_error(f, m):
+    """max_i |sum_j f_ij - sum_j f_ji| / m_i, summed in float64 in index order
+    (the per-column non-divergence check, emanuel.py:437-446)."""
+
+    f = np.asarray(f, dtype=np.float64)
+    n = f.shape[0]
+    worst = 0.0
+    for i in range(n):
+        row = col = 0.0
+        for j in range(n):
+            row += f[i, j]
+            col += f[j, i]
+        worst = max(worst, abs(row - col) / max(float(m[i]), 1e-12))
+    return worst
+
+
 def backward_probabilities(f, layer_mass_clamped):
     """Transposed, row-normalised transition (emanuel.py:756-763)."""
 

@@ -5,6 +5,7 @@ brute-force check of the backward sampling distribution."""
 from __future__ import annotations
 
 import numpy as np
+import pytest
 
 from oracle import convection as CV
 
@@ -56,6 +57,25 @@ def test_matrix_non_divergent_and_mass_stationary_both_directions():
     for P in (ff / m[:, None], ff.T / m[:, None]):
         assert np.allclose(P.sum(1), 1.0, atol=1e-9) and (P >= -1e-12).all()
         assert np.allclose(m @ P, m, rtol=1e-9)
+
+
+def test_float32_matrix_rounds_the_float64_matrix():
+    # The float32 restatement is the float64 matrix up to float32 rounding:
+    # a dropped term (phi, the CFL scale) or a transposed product would show
+    # at the 1e-2 level, rounding stays near 1e-6.
+    t, q, p = moist_column()
+    m = CV.layer_masses(p)
+    lcl, lnb, _, _ = CV.compute_lcl_lnb_cape(t, q, p)
+    for m_b in (0.05 * 3600.0, 5.0 * 3600.0):  # the second trips the CFL cap
+        f64 = CV.mass_flux_matrix(m, lcl, lnb, m_b)
+        f32 = CV.mass_flux_matrix_f32(m.astype(np.float32), lcl, lnb, m_b)
+        assert f32.dtype == np.float32
+        np.testing.assert_allclose(f32, f64, rtol=2e-5, atol=1e-6 * np.abs(f64).max())
+        err = CV.column_sum_error(f32, m)
+        assert 0.0 <= err < 1e-5
+        assert CV.column_sum_error(f64, m) < 1e-12
+    assert (f64.sum(1) / m).max() == pytest.approx(0.9, rel=1e-9)
+    assert CV.mass_flux_matrix_f32(m.astype(np.float32), -1, -1, 1.0).sum() == 0.0
 
 
 def test_backward_sampling_distribution_brute_force():
