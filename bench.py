@@ -47,13 +47,14 @@ def parse():
     ap.add_argument("--met", choices=("f32", "bf16"), default="f32")
     ap.add_argument("--reorder-every", type=int, default=0,
                     help="sort the particle state by met cell every N steps (0 = never)")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="launch every step eagerly instead of replaying the captured step body")
+    ap.add_argument("--graph", action="store_true",
+                    help="time the captured step body (graph replay) as the headline run instead of "
+                         "eager launches (both are always measured)")
     ap.add_argument("--time-every", type=int, default=8,
-                    help="time the step kernel with HIP events on every Nth timed step (those steps "
-                         "launch eagerly; the others replay the captured step body)")
+                    help="time the step kernel with HIP events on every Nth timed step (in graph "
+                         "mode those steps launch eagerly)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-particles", type=int, default=65536)
+    ap.add_argument("--cpu-particles", type=int, default=262144)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a float32 copy kernel of this many MiB (reads and writes that "
@@ -127,12 +128,28 @@ def main() -> int:
     sptr = int(stream.cuda_stream)
     torch.cuda.set_stream(stream)
     eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=met_dtype, reorder_every=args.reorder_every,
-                      use_graph=not args.no_graph)
+                      use_graph=True)
     n_sched = len(eng.sched)
     K = max(1, min(args.steps, n_sched))
     W = max(0, min(args.warmup, n_sched))
 
-    # warmup (captures the step body), then back to the release state
+    def timed_run(graph: bool) -> float:
+        """Warm up (captures the step body in graph mode), reset, time K steps."""
+        eng.use_graph = graph
+        eng.step_events = None
+        eng.run(W)
+        torch.cuda.synchronize()
+        eng.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            eng.run_step(sptr)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # the other launch mode first (replay vs eager), then the headline run
+    other_elapsed = timed_run(not args.graph)
+    eng.use_graph = args.graph
     eng.run(W)
     torch.cuda.synchronize()
     eng.reset()
@@ -160,7 +177,8 @@ def main() -> int:
     t_act = np.array([act[i] for i, _ in timed])
     avg_ms = float(t_ms.mean()) if len(t_ms) else float("nan")
     bytes_launch = BYTES_PER_PARTICLE_STEP * float(t_act.mean()) if len(t_act) else 0.0
-    achieved = BYTES_PER_PARTICLE_STEP * float(t_act.sum()) / (float(t_ms.sum()) * 1e-3) / 1e9
+    achieved = (BYTES_PER_PARTICLE_STEP * float(t_act.sum()) / (float(t_ms.sum()) * 1e-3) / 1e9
+                if len(t_ms) else float("nan"))
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 2),
@@ -205,10 +223,14 @@ def main() -> int:
             "meander": True,
             "max_substeps": eng.h.max_substeps,
             "reorder_every": args.reorder_every,
-            "step_graph": not args.no_graph,
+            "step_graph": bool(args.graph),
             "footprint": f"1x{cfg['grid']['n_t']}x1x{cfg['grid']['n_y']}x{cfg['grid']['n_x']} f32",
         },
         "active_particle_steps": psteps,
+        ("eager" if args.graph else "graph_replay"): {
+            "ms_per_step": other_elapsed * 1e3 / K,
+            "value": psteps / other_elapsed,
+        },
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:

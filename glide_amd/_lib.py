@@ -28,6 +28,8 @@ F_EMIT = 0x8
 F_MASK_INPUT = 0x10
 F_NO_TURB = 0x20
 
+GLIDE_ROW_WORDS = 66  # int32 words of the step cursor buffer (glide_step_table)
+
 vp = C.c_void_p
 
 
@@ -89,7 +91,6 @@ SIGNATURES = {
     "glide_step_table": (C.c_int, [P(GlideStepParams), vp, vp, P(GlideParticles), P(GlideGrid), P(GlideWindow),
                                    P(GlideFootprintGrid), P(GlideEmit), vp]),
     "glide_row_set": (C.c_int, [vp, C.c_int32, vp]),
-    "glide_row_advance": (C.c_int, [vp, vp]),
     "glide_footprint_keys": (C.c_int, [C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                        P(GlideFootprintGrid), vp, vp, vp]),
     "glide_footprint_workspace_bytes": (C.c_size_t, [C.c_int64]),

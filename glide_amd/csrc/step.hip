@@ -31,14 +31,16 @@ struct StepArgs {
     const void* surf;
     FpDev fp;
     GlideEmit e;
-    // Table mode: the step's scalars come from table[*row] on the device and
-    // the escape count goes to e.escaped[table[*row].step_index].
+    // Table mode: the step's scalars come from table[row[0]] on the device, the
+    // escape count goes to e.escaped[table[row[0]].step_index], and the last
+    // workgroup to finish advances row[0] (row[1..] count finished workgroups).
     const GlideStepParams* table;
-    const int32_t* row;
+    int32_t* row;
 };
 
 constexpr int kMaxLevels = 128;
 constexpr int kStepBlock = 256;
+constexpr uint32_t kRowGroups = GLIDE_ROW_WORDS - 2;  // finished-workgroup counters of the step cursor
 
 // The step's row of the table, read into scalar registers (one readfirstlane
 // per 32-bit word), so everything derived from it stays scalar.
@@ -56,24 +58,13 @@ __device__ __forceinline__ GlideStepParams load_row(const GlideStepParams* table
 }
 
 template <bool BF16>
-__global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
-    __shared__ float lev[kMaxLevels];
-    for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
-    __syncthreads();
-
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= a.s.n) return;
+__device__ __forceinline__ void step_particle(const StepArgs& a, const float* lev, int64_t i,
+                                              const GlideStepParams& p, unsigned long long* escaped) {
     // 32-bit byte offsets of this particle (the host checks the arrays fit)
     const uint32_t i1 = static_cast<uint32_t>(i);
     const uint32_t i4 = i1 * 4u;
     const uint32_t ip = static_cast<uint32_t>(i * a.s.pos_stride) * 4u;
     const GlideParticles& s = a.s;
-    GlideStepParams p = a.p;
-    unsigned long long* escaped = a.e.escaped;
-    if (a.table != nullptr) {
-        p = load_row(a.table, a.row);
-        escaped += p.step_index;
-    }
     const bool emit = (p.flags & GLIDE_F_EMIT) != 0;
     const float co = static_cast<float>(p.cursor_off);
 
@@ -308,6 +299,42 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
     }
 }
 
+template <bool BF16>
+__global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
+    __shared__ float lev[kMaxLevels];
+    for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
+    __syncthreads();
+    GlideStepParams p = a.p;
+    unsigned long long* escaped = a.e.escaped;
+    if (a.table != nullptr) {
+        p = load_row(a.table, a.row);
+        escaped += p.step_index;
+    }
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < a.s.n) step_particle<BF16>(a, lev, i, p, escaped);
+    if (a.table != nullptr) {
+        // Every workgroup has used row[0] before it counts itself finished, so
+        // the last one to finish can advance the cursor (no waiting). The count
+        // is split over kRowGroups counters and a top counter, so no single
+        // address takes more than gridDim / kRowGroups atomics.
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t nb = gridDim.x;
+            const uint32_t g = blockIdx.x % kRowGroups;
+            const uint32_t members = (nb - g + kRowGroups - 1) / kRowGroups;
+            uint32_t* cnt = reinterpret_cast<uint32_t*>(a.row + 2);
+            if (atomicAdd(cnt + g, 1u) == members - 1) {
+                atomicExch(cnt + g, 0u);
+                const uint32_t groups = nb < kRowGroups ? nb : kRowGroups;
+                if (atomicAdd(reinterpret_cast<uint32_t*>(a.row + 1), 1u) == groups - 1) {
+                    atomicExch(reinterpret_cast<uint32_t*>(a.row + 1), 0u);
+                    atomicAdd(a.row, 1);
+                }
+            }
+        }
+    }
+}
+
 __global__ void footprint_key_kernel(int64_t n, int64_t ps, const float* lon, const float* lat,
                                      const float* z, const uint8_t* active, const float* weight,
                                      const int64_t* t_idx, const int64_t* rel_idx, float dt,
@@ -350,7 +377,7 @@ static int fill_grid(StepArgs& a, const GlideGrid& g, const GlideWindow& w) {
 
 using namespace glide;
 
-static int launch_step(const GlideStepParams* p, const GlideStepParams* table, const int32_t* row,
+static int launch_step(const GlideStepParams* p, const GlideStepParams* table, int32_t* row,
                        const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
                        const GlideFootprintGrid* fg, const GlideEmit* e, void* stream) {
     GLIDE_REQUIRE(p && s && g && w, "glide_step: NULL argument");
@@ -406,25 +433,20 @@ extern "C" int glide_step(const GlideStepParams* p, const GlideParticles* s, con
     return launch_step(p, nullptr, nullptr, s, g, w, fg, e, stream);
 }
 
-extern "C" int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, const int32_t* row,
+extern "C" int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, int32_t* row,
                                 const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
                                 const GlideFootprintGrid* fg, const GlideEmit* e, void* stream) {
     GLIDE_REQUIRE(table != nullptr && row != nullptr, "glide_step_table: NULL table/row");
     return launch_step(tmpl, table, row, s, g, w, fg, e, stream);
 }
 
-__global__ void row_kernel(int32_t* row, int32_t value, int32_t add) { *row = add ? *row + value : value; }
+__global__ void row_set_kernel(int32_t* row, int32_t value) {
+    for (int k = threadIdx.x; k < GLIDE_ROW_WORDS; k += blockDim.x) row[k] = k == 0 ? value : 0;
+}
 
 extern "C" int glide_row_set(int32_t* row, int32_t value, void* stream) {
     GLIDE_REQUIRE(row != nullptr, "NULL row");
-    hipLaunchKernelGGL(row_kernel, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), row, value, 0);
-    GLIDE_HIP(hipGetLastError());
-    return GLIDE_OK;
-}
-
-extern "C" int glide_row_advance(int32_t* row, void* stream) {
-    GLIDE_REQUIRE(row != nullptr, "NULL row");
-    hipLaunchKernelGGL(row_kernel, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), row, 1, 1);
+    hipLaunchKernelGGL(row_set_kernel, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), row, value);
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
 }

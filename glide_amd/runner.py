@@ -230,7 +230,7 @@ class BatchEngine:
         rows = (L.GlideStepParams * max(len(self.sched), 1))(*[self.step_params(st, flags) for st in self.sched])
         self.step_table = torch.frombuffer(bytearray(bytes(rows)), dtype=torch.uint8).to(d)
         self.step_tmpl = self.step_params(self.sched[0], flags) if self.sched else None
-        self.row = torch.zeros(1, dtype=torch.int32, device=d)
+        self.row = torch.zeros(L.GLIDE_ROW_WORDS, dtype=torch.int32, device=d)  # cursor, finished-workgroup counts
 
     def _make_parts(self, b) -> L.GlideParticles:
         return L.GlideParticles(
@@ -302,8 +302,9 @@ class BatchEngine:
         return L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(), escaped=esc)
 
     def _capture(self, stream: int) -> int:
-        """Capture the plain step body (table-driven step, footprint
-        accumulation, cursor advance) for the current particle buffer."""
+        """Capture the plain step body (table-driven step, which also advances
+        the device cursor, and the footprint accumulation) for the current
+        particle buffer."""
 
         lib = L.lib()
         if stream == 0:
@@ -315,7 +316,6 @@ class BatchEngine:
                                          C.byref(self.parts), C.byref(self.met.grid), C.byref(self.met.window),
                                          C.byref(self.fg), C.byref(em), stream))
             self._accumulate(stream)
-            L.check(lib.glide_row_advance(self.row.data_ptr(), stream))
         finally:
             h = C.c_void_p()
             rc = lib.glide_graph_end(stream, C.byref(h))

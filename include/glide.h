@@ -164,17 +164,20 @@ int glide_step(const GlideStepParams* p, const GlideParticles* s, const GlideGri
 /* glide_step with the per-step scalars in device memory, for a captured step
  * body (replaces the reference's static-buffer graph replay,
  * hanna.py:1117-1160 with main.py:1396-1477): the step reads
- * table[*row] (device GlideStepParams rows, one per schedule step) and adds
- * its escape count to e->escaped[table[*row].step_index]. tmpl is a host row
+ * table[row[0]] (device GlideStepParams rows, one per schedule step), adds its
+ * escape count to e->escaped[table[row[0]].step_index], and the last
+ * workgroup to finish advances row[0] by one. row points to GLIDE_ROW_WORDS
+ * device int32: the cursor, then finished-workgroup counters that must be 0
+ * at launch (they are left 0; glide_row_set sets them). tmpl is a host row
  * with the flags every table row carries; it is used for argument checks. */
-int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, const int32_t* row,
+#define GLIDE_ROW_WORDS 66
+int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, int32_t* row,
                      const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
                      const GlideFootprintGrid* fg, const GlideEmit* e, void* stream);
 
-/* Device step cursor for glide_step_table: *row = value, or *row += 1. Both
- * are one-thread kernels, so they can sit inside a captured graph. */
+/* Device cursor for glide_step_table: row[0] = value, the counters 0 (a
+ * kernel, so it may sit inside a captured graph). */
 int glide_row_set(int32_t* row, int32_t value, void* stream);
-int glide_row_advance(int32_t* row, void* stream);
 
 /* FootprintGridder.accumulate index math on arbitrary inputs: positions
  * lon/lat/z (strided by pos_stride), active mask, weights, per-particle time
