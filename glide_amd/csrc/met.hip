@@ -13,11 +13,14 @@ constexpr double kMPerDegLatD = 110540.0;
 
 // Orientation of the source levels: top-of-atmosphere first when the bbox-mean
 // height of level 0 exceeds that of the last level (vertical_grid.py:195-196).
+// kOrientBlocks partial sums in float64, then one thread adds them in order.
+constexpr int kOrientBlocks = 64;
+
 __global__ __launch_bounds__(256) void level_orientation_kernel(const float* __restrict__ h, int32_t zp,
-                                                                int64_t ncol, int32_t* __restrict__ flipped) {
+                                                                int64_t ncol, double* __restrict__ part) {
     __shared__ double s0[256], s1[256];
     double a = 0.0, b = 0.0;
-    for (int64_t c = threadIdx.x; c < ncol; c += blockDim.x) {
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; c < ncol; c += 256 * kOrientBlocks) {
         a += static_cast<double>(h[c]);
         b += static_cast<double>(h[static_cast<int64_t>(zp - 1) * ncol + c]);
     }
@@ -31,7 +34,19 @@ __global__ __launch_bounds__(256) void level_orientation_kernel(const float* __r
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *flipped = (s0[0] > s1[0]) ? 1 : 0;
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s0[0];
+        part[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+
+__global__ void level_orientation_final(const double* __restrict__ part, int32_t* __restrict__ flipped) {
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < kOrientBlocks; ++k) {
+        a += part[2 * k];
+        b += part[2 * k + 1];
+    }
+    *flipped = (a > b) ? 1 : 0;
 }
 
 struct ResampleArgs {
@@ -60,11 +75,14 @@ __device__ __forceinline__ float grad1(const float* __restrict__ f, int64_t stri
     return static_cast<float>(static_cast<double>(f[stride] - f[-stride]) / (2.0 * d));
 }
 
+// One thread per (column, target level): blockIdx.y is the level, so each
+// block's pressure partial covers the same 256 columns at one level.
 __global__ __launch_bounds__(kColBlock) void resample_kernel(const ResampleArgs a) {
 #pragma clang fp contract(off)
     __shared__ double psum[kColBlock];
     const int64_t ncol = static_cast<int64_t>(a.ny) * a.nx;
     const int64_t col = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
     const bool live = col < ncol;
     const int zp = a.zp;
     const bool flip = *a.flipped != 0;
@@ -72,66 +90,60 @@ __global__ __launch_bounds__(kColBlock) void resample_kernel(const ResampleArgs 
     auto src_idx = [&](int asc) { return flip ? (zp - 1 - asc) : asc; };
     auto hasc = [&](int asc) { return static_cast<double>(a.h[static_cast<int64_t>(src_idx(asc)) * ncol + col]); };
 
-    int k0 = 0;
-    double dhdx = 0.0;
-    float dhdy = 0.0f;
+    double pval = 0.0;
     if (live) {
+        int k0 = 0;
         for (int k = 0; k < zp; ++k) k0 += (hasc(k) < 0.0) ? 1 : 0;
         const int y = static_cast<int>(col / a.nx);
         const int x = static_cast<int>(col % a.nx);
         const float* tc = a.terrain + col;
         const float dlat_g = grad1(tc, a.nx, y, a.ny, a.dlat);
         const float dlon_g = grad1(tc, 1, x, a.nx, a.dlon);
-        dhdx = static_cast<double>(dlon_g) / a.mlon[y];
-        dhdy = dlat_g / static_cast<float>(kMPerDegLatD);  // float32, as numpy keeps it
-    }
-    const double ztop = a.agl[a.za - 1];
-    const double ztop_c = ztop > 1.0 ? ztop : 1.0;
-    int idx = 0;
-    for (int j = 0; j < a.za; ++j) {
+        const double dhdx = static_cast<double>(dlon_g) / a.mlon[y];
+        const float dhdy = dlat_g / static_cast<float>(kMPerDegLatD);  // float32, as numpy keeps it
+        const double ztop = a.agl[a.za - 1];
+        const double ztop_c = ztop > 1.0 ? ztop : 1.0;
         const double tg = a.agl[j];
-        double pval = 0.0;
-        if (live) {
-            while (idx < zp && hasc(idx) < tg) ++idx;
-            int lower = idx - 1;
-            if (lower < 0) lower = 0;
-            if (lower < k0) lower = k0;
-            if (lower > zp - 2) lower = zp - 2;
-            const double hlo = hasc(lower), hhi = hasc(lower + 1);
-            double den = hhi - hlo;
-            if (den < 1e-6) den = 1e-6;
-            double fr = (tg - hlo) / den;
-            fr = fr < 0.0 ? 0.0 : (fr > 1.0 ? 1.0 : fr);
-            const int slo = src_idx(lower), shi = src_idx(lower + 1);
-            const float wf = static_cast<float>(fr);
-            float uu = 0.0f, vv = 0.0f;
-            for (int c = 0; c < a.c3; ++c) {
-                const float* fc = a.src3 + static_cast<int64_t>(c) * zp * ncol;
-                const float f0 = fc[static_cast<int64_t>(slo) * ncol + col];
-                const float f1 = fc[static_cast<int64_t>(shi) * ncol + col];
-                const float r = (fabsf(wf) < 0.5f) ? f0 + wf * (f1 - f0) : f1 - (f1 - f0) * (1.0f - wf);
-                if (c == a.iu) uu = r;
-                if (c == a.iv) vv = r;
-                a.dst3[(static_cast<int64_t>(c) * a.za + j) * ncol + col] = r;
-            }
-            // Slope correction of w in float64, rounded once (vertical_grid.py:296-316).
-            double taper = 1.0 - tg / ztop_c;
-            taper = taper < 0.0 ? 0.0 : (taper > 1.0 ? 1.0 : taper);
-            const double slope = static_cast<double>(uu) * dhdx + static_cast<double>(vv * dhdy);
-            float* wout = a.dst3 + (static_cast<int64_t>(a.iw) * a.za + j) * ncol + col;
-            *wout = static_cast<float>(static_cast<double>(*wout) - taper * slope);
-            const double p0 = a.p_hpa[slo], p1 = a.p_hpa[shi];
-            pval = (fabs(fr) < 0.5) ? p0 + fr * (p1 - p0) : p1 - (p1 - p0) * (1.0 - fr);
+        int idx = 0;  // searchsorted(h, tg, side="left")
+        while (idx < zp && hasc(idx) < tg) ++idx;
+        int lower = idx - 1;
+        if (lower < 0) lower = 0;
+        if (lower < k0) lower = k0;
+        if (lower > zp - 2) lower = zp - 2;
+        const double hlo = hasc(lower), hhi = hasc(lower + 1);
+        double den = hhi - hlo;
+        if (den < 1e-6) den = 1e-6;
+        double fr = (tg - hlo) / den;
+        fr = fr < 0.0 ? 0.0 : (fr > 1.0 ? 1.0 : fr);
+        const int slo = src_idx(lower), shi = src_idx(lower + 1);
+        const float wf = static_cast<float>(fr);
+        float uu = 0.0f, vv = 0.0f, ww = 0.0f;
+        for (int c = 0; c < a.c3; ++c) {
+            const float* fc = a.src3 + static_cast<int64_t>(c) * zp * ncol;
+            const float f0 = fc[static_cast<int64_t>(slo) * ncol + col];
+            const float f1 = fc[static_cast<int64_t>(shi) * ncol + col];
+            const float r = (fabsf(wf) < 0.5f) ? f0 + wf * (f1 - f0) : f1 - (f1 - f0) * (1.0f - wf);
+            if (c == a.iu) uu = r;
+            if (c == a.iv) vv = r;
+            if (c == a.iw) ww = r;
+            if (c != a.iw) a.dst3[(static_cast<int64_t>(c) * a.za + j) * ncol + col] = r;
         }
-        psum[threadIdx.x] = pval;
-        __syncthreads();
-        for (int off = kColBlock / 2; off > 0; off >>= 1) {
-            if (threadIdx.x < off) psum[threadIdx.x] += psum[threadIdx.x + off];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) a.p_part[static_cast<int64_t>(blockIdx.x) * a.za + j] = psum[0];
+        // Slope correction of w in float64, rounded once (vertical_grid.py:296-316).
+        double taper = 1.0 - tg / ztop_c;
+        taper = taper < 0.0 ? 0.0 : (taper > 1.0 ? 1.0 : taper);
+        const double slope = static_cast<double>(uu) * dhdx + static_cast<double>(vv * dhdy);
+        a.dst3[(static_cast<int64_t>(a.iw) * a.za + j) * ncol + col] =
+            static_cast<float>(static_cast<double>(ww) - taper * slope);
+        const double p0 = a.p_hpa[slo], p1 = a.p_hpa[shi];
+        pval = (fabs(fr) < 0.5) ? p0 + fr * (p1 - p0) : p1 - (p1 - p0) * (1.0 - fr);
+    }
+    psum[threadIdx.x] = pval;
+    __syncthreads();
+    for (int off = kColBlock / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) psum[threadIdx.x] += psum[threadIdx.x + off];
         __syncthreads();
     }
+    if (threadIdx.x == 0) a.p_part[static_cast<int64_t>(blockIdx.x) * a.za + j] = psum[0];
 }
 
 __global__ void pressure_mean_kernel(const double* __restrict__ part, int64_t nblk, int32_t za, int64_t ncol,
@@ -196,6 +208,8 @@ __device__ float local_std(const float* __restrict__ s, const float* __restrict_
     return sqrtf(ss / static_cast<float>(cnt));
 }
 
+// One thread per (column, level): blockIdx.y is the level. Level 0's threads
+// also write the surface record.
 template <bool BF16>
 __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
     const int64_t ncol = static_cast<int64_t>(a.ny) * a.nx;
@@ -204,21 +218,22 @@ __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
     const int y = static_cast<int>(col / a.nx);
     const int x = static_cast<int>(col % a.nx);
     const int nz = a.nz;
-    auto at = [&](const float* f, int ch, int z) {
-        return f[(static_cast<int64_t>(ch) * nz + z) * ncol + col];
+    const int z = blockIdx.y;
+    auto at = [&](const float* f, int ch, int zz) {
+        return f[(static_cast<int64_t>(ch) * nz + zz) * ncol + col];
     };
-    auto pres = [&](int z) { return static_cast<float>(a.p_hpa[z]) * 100.0f; };
-    auto theta = [&](int z) {
-        const float tm = 0.5f * (at(a.s3, 3, z) + at(a.e3, 3, z));
-        return tm * powf(100000.0f / pres(z), kKappa);
+    auto pres = [&](int zz) { return static_cast<float>(a.p_hpa[zz]) * 100.0f; };
+    auto theta = [&](int zz) {
+        const float tm = 0.5f * (at(a.s3, 3, zz) + at(a.e3, 3, zz));
+        return tm * powf(100000.0f / pres(zz), kKappa);
     };
-    auto umid = [&](int z) { return 0.5f * (at(a.s3, 0, z) + at(a.e3, 0, z)); };
-    auto vmid = [&](int z) { return 0.5f * (at(a.s3, 1, z) + at(a.e3, 1, z)); };
-    auto rho = [&](int z) {
-        const float tm = 0.5f * (at(a.s3, 3, z) + at(a.e3, 3, z));
-        return pres(z) / (287.05f * fmaxf(tm, 1.0f));
+    auto umid = [&](int zz) { return 0.5f * (at(a.s3, 0, zz) + at(a.e3, 0, zz)); };
+    auto vmid = [&](int zz) { return 0.5f * (at(a.s3, 1, zz) + at(a.e3, 1, zz)); };
+    auto rho = [&](int zz) {
+        const float tm = 0.5f * (at(a.s3, 3, zz) + at(a.e3, 3, zz));
+        return pres(zz) / (287.05f * fmaxf(tm, 1.0f));
     };
-    for (int z = 0; z < nz; ++z) {
+    {
         const int zl = z == 0 ? 0 : z - 1;
         const int zh = z == nz - 1 ? nz - 1 : z + 1;
         const float dh = safe_den(a.lev[zh] - a.lev[zl]);
@@ -255,6 +270,7 @@ __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
         Rec8<BF16>::store(a.wind, zoff + col, wr);
         Rec8<BF16>::store(a.aux, zoff + col, xr);
     }
+    if (z != 0) return;
     // Surface record: blh sp ustar shf t(level 0) at start then end.
     float sr[12];
     for (int c = 0; c < 4; ++c) {
@@ -299,7 +315,7 @@ extern "C" int glide_resample_hour(const float* src3, int32_t c3, int32_t zp, in
     // Small device scratch: orientation flag + per-block pressure sums.
     static thread_local void* scratch = nullptr;
     static thread_local size_t scratch_bytes = 0;
-    const size_t need = 256 + sizeof(double) * static_cast<size_t>(nblk) * za;
+    const size_t need = 256 + sizeof(double) * 2 * kOrientBlocks + sizeof(double) * static_cast<size_t>(nblk) * za;
     if (scratch_bytes < need) {
         if (scratch) GLIDE_HIP(hipFree(scratch));
         scratch = nullptr;
@@ -309,11 +325,14 @@ extern "C" int glide_resample_hour(const float* src3, int32_t c3, int32_t zp, in
         scratch_bytes = need;
     }
     int32_t* flipped = static_cast<int32_t*>(scratch);
-    double* part = reinterpret_cast<double*>(static_cast<char*>(scratch) + 256);
-    hipLaunchKernelGGL(level_orientation_kernel, dim3(1), dim3(256), 0, st, height_agl, zp, ncol, flipped);
+    double* opart = reinterpret_cast<double*>(static_cast<char*>(scratch) + 256);
+    double* part = opart + 2 * kOrientBlocks;
+    hipLaunchKernelGGL(level_orientation_kernel, dim3(kOrientBlocks), dim3(256), 0, st, height_agl, zp, ncol, opart);
+    hipLaunchKernelGGL(level_orientation_final, dim3(1), dim3(1), 0, st, opart, flipped);
     ResampleArgs a{src3, c3, zp, ny, nx, height_agl, agl_levels, za, terrain, m_per_deg_lon, dlat_deg, dlon_deg,
                    p_level_hpa, iu, iv, iw, dst3, part, flipped};
-    hipLaunchKernelGGL(resample_kernel, dim3(static_cast<unsigned>(nblk)), dim3(kColBlock), 0, st, a);
+    hipLaunchKernelGGL(resample_kernel, dim3(static_cast<unsigned>(nblk), static_cast<unsigned>(za)), dim3(kColBlock),
+                       0, st, a);
     hipLaunchKernelGGL(pressure_mean_kernel, dim3((za + 63) / 64), dim3(64), 0, st, part, nblk, za, ncol, p_out_hpa);
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
@@ -334,10 +353,11 @@ extern "C" int glide_build_window(const float* h3_s, const float* h3_e, const fl
     const int64_t ncol = static_cast<int64_t>(ny) * nx;
     const unsigned nblk = static_cast<unsigned>((ncol + kColBlock - 1) / kColBlock);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    const dim3 grid(nblk, static_cast<unsigned>(nz));
     if (dtype == GLIDE_MET_BF16) {
-        hipLaunchKernelGGL(window_kernel<true>, dim3(nblk), dim3(kColBlock), 0, st, a);
+        hipLaunchKernelGGL(window_kernel<true>, grid, dim3(kColBlock), 0, st, a);
     } else {
-        hipLaunchKernelGGL(window_kernel<false>, dim3(nblk), dim3(kColBlock), 0, st, a);
+        hipLaunchKernelGGL(window_kernel<false>, grid, dim3(kColBlock), 0, st, a);
     }
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
