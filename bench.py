@@ -54,6 +54,10 @@ def parse():
                     help="time the step kernel with HIP events on every Nth timed step (in graph "
                          "mode those steps launch eagerly)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the network-batch leg (batched-gpu-336r, bfloat16 met, compensated footprint)")
+    ap.add_argument("--batched-steps", type=int, default=0,
+                    help="time only the first N steps of the network batch (0 = all)")
     ap.add_argument("--cpu-particles", type=int, default=262144)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -100,6 +104,63 @@ def cpu_baseline(cfg, n_particles: int, n_steps: int) -> dict:
                    f"(float32 numpy restatement of the reference's plain method, "
                    f"{res.particle_steps} particle-steps in {res.step_seconds:.2f} s, one core)"),
     }
+
+
+BATCHED = "batched-gpu-336r"
+
+
+def batched_leg(args, sptr: int, plain_value: float) -> dict:
+    """One GPU's share of the network batch: 336 releases (7 sites x 48 hourly
+    releases, 2k-20k particles, 24-120 h back) in one engine, with the
+    precision layout of the batched method (bfloat16 met records, float32
+    footprint with Neumaier compensation, float64 release totals) and the live
+    set compacted every met hour. Times the whole schedule."""
+
+    import torch
+
+    from glide_amd import _lib as L
+    from glide_amd.runner import BatchEngine
+    from glide_amd.synthetic import SyntheticMet, workload_config
+
+    cfg = workload_config(BATCHED)
+    met = SyntheticMet(cfg["spec"])
+    eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=L.GLIDE_MET_BF16, compensated=True, compact=True)
+    n_sched = len(eng.sched)
+    K = n_sched if args.batched_steps <= 0 else min(args.batched_steps, n_sched)
+    eng.run(min(20, n_sched))  # warm-up, then back to the release state
+    torch.cuda.synchronize()
+    eng.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        eng.run_step(sptr)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    psteps = int(eng.active_steps.cpu().numpy()[:K].sum())
+    g = cfg["grid"]
+    value = psteps / elapsed
+    out = {
+        "workload": BATCHED,
+        "value": value,
+        "unit": "particle-steps/s",
+        "ms_per_step": elapsed * 1e3 / K,
+        "steps": K,
+        "steps_in_run": n_sched,
+        "active_particle_steps": psteps,
+        "particles": eng.n,
+        "releases": len(cfg["releases"]),
+        "met_storage": "bf16",
+        "footprint": f"{g['n_r']}x{g['n_t']}x1x{g['n_y']}x{g['n_x']} f32 + Neumaier compensation",
+        "release_totals": "f64",
+        "vs_plain": value / plain_value,
+        "error_bar": ("footprint within 1 % relative L2 per release of the float64 oracle on the same noise "
+                      "(tests/test_gpu_batch.py, 6-release network batch on the 64x48 grid: measured <= 1.4e-3 "
+                      "with bfloat16 met, <= 3e-4 with float32 met; residence totals within 8.5e-4 / 1.9e-4)"),
+    }
+    eng.close()
+    del eng
+    torch.cuda.empty_cache()
+    return out
 
 
 def main() -> int:
@@ -164,14 +225,8 @@ def main() -> int:
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
-    esc = eng.escaped.cpu().numpy()
-    active = []
-    cum_esc = 0
-    for i in range(K):
-        active.append(eng.active_count(eng.sched[i]) - cum_esc)
-        cum_esc += int(esc[i])
-    psteps = int(sum(active))
-    act = np.array(active, dtype=np.float64)
+    act = eng.active_steps.cpu().numpy()[:K].astype(np.float64)  # counted by the step kernel
+    psteps = int(act.sum())
     timed = [(i, a.elapsed_time(b)) for i, (a, b) in sorted(ev.items()) if act[i] > 0]
     t_ms = np.array([t for _, t in timed])
     t_act = np.array([act[i] for i, _ in timed])
@@ -233,6 +288,11 @@ def main() -> int:
         },
         "roofline": roofline,
     }
+    if not args.no_batched:
+        eng.close()
+        del eng
+        torch.cuda.empty_cache()
+        out["batched"] = batched_leg(args, sptr, out["value"])
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_particles, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -13,6 +13,8 @@ from pathlib import Path
 _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libglide_hip.so"
 
+ABI_VERSION = 2  # GLIDE_ABI_VERSION of include/glide.h
+
 GLIDE_OK = 0
 GLIDE_ERR_INVALID = -1
 GLIDE_ERR_MEMORY = -2
@@ -27,6 +29,7 @@ F_SL_OVERRIDE = 0x4
 F_EMIT = 0x8
 F_MASK_INPUT = 0x10
 F_NO_TURB = 0x20
+F_RELEASE_TERMINUS = 0x40
 
 GLIDE_ROW_WORDS = 66  # int32 words of the step cursor buffer (glide_step_table)
 
@@ -51,7 +54,7 @@ class GlideParticles(C.Structure):
         ("lon", vp), ("lat", vp), ("z", vp),
         ("up", vp), ("vp", vp), ("wp", vp), ("um", vp), ("vm", vp),
         ("weight", vp), ("r_off", vp), ("we_off", vp), ("rel_idx", vp), ("uid", vp),
-        ("alive", vp), ("active", vp),
+        ("alive", vp), ("active", vp), ("lo_off", vp),
     ]
 
 
@@ -77,7 +80,7 @@ class GlideStepParams(C.Structure):
 
 
 class GlideEmit(C.Structure):
-    _fields_ = [("keys", vp), ("vals", vp), ("escaped", vp)]
+    _fields_ = [("keys", vp), ("vals", vp), ("escaped", vp), ("active", vp)]
 
 
 P = C.POINTER
@@ -98,7 +101,7 @@ SIGNATURES = {
                                              C.c_size_t, vp]),
     "glide_sort_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "glide_sort_pairs": (C.c_int, [vp, vp, C.c_int64, C.c_int32, vp, vp, vp, C.c_size_t, vp]),
-    "glide_reorder_keys": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), vp, vp, vp]),
+    "glide_reorder_keys": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), vp, vp, vp, vp]),
     "glide_permute_particles": (C.c_int, [P(GlideParticles), P(GlideParticles), vp, vp]),
     "glide_resample_hour": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp, C.c_int32,
                                       vp, vp, C.c_double, C.c_double, vp, C.c_int32, C.c_int32, C.c_int32,
@@ -134,7 +137,7 @@ def lib() -> C.CDLL:
         fn = getattr(h, name)
         fn.restype = res
         fn.argtypes = args
-    if h.glide_abi_version() != 1:
+    if h.glide_abi_version() != ABI_VERSION:
         raise RuntimeError("libglide_hip.so ABI version mismatch")
     _LIB = h
     return h

@@ -84,6 +84,21 @@ __device__ __forceinline__ void sto(T* p, uint32_t off, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + off) = v;
 }
 
+// Active-mask test of main.py:1396-1401 with a per-particle backward length
+// when lo_off is given: r_off >= cursor and r_off - length <= cursor, alive.
+// With GLIDE_F_RELEASE_TERMINUS, lo_off is the release terminus te and the
+// test is r_off >= cursor > te (the release's own schedule).
+__device__ __forceinline__ bool in_window(const GlideParticles& s, uint32_t i1, float cursor, float length,
+                                          int32_t flags) {
+    const uint32_t i4 = i1 * 4u;
+    const float ro = ldo(s.r_off, i4);
+    if (flags & GLIDE_F_RELEASE_TERMINUS) {
+        return (ro >= cursor) && (cursor > ldo(s.lo_off, i4)) && (ldo(s.alive, i1) != 0);
+    }
+    const float lo = s.lo_off ? ldo(s.lo_off, i4) : ro - length;
+    return (ro >= cursor) && (lo <= cursor) && (ldo(s.alive, i1) != 0);
+}
+
 // ---------------------------------------------------------------------------
 // bfloat16 <-> float (round to nearest even)
 // ---------------------------------------------------------------------------
@@ -181,42 +196,18 @@ struct Rec8<true> {
     }
 };
 
-// Surface record of 12 values (blh sp ustar shf t0 at start, then at end).
-template <bool BF16>
-__device__ __forceinline__ void load_surf(const void* base, int64_t cell, float (&v)[10]) {
-    if constexpr (!BF16) {
-        const float4* p = static_cast<const float4*>(base) + cell * 3;
-        const float4 a = p[0], b = p[1], c = p[2];
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x;
-        v[5] = b.y; v[6] = b.z; v[7] = b.w; v[8] = c.x; v[9] = c.y;
-    } else {
-        const uint2* p = static_cast<const uint2*>(base) + cell * 3;
-        const uint2 a = p[0], b = p[1], c = p[2];
-        v[0] = bf16_lo(a.x); v[1] = bf16_hi(a.x); v[2] = bf16_lo(a.y); v[3] = bf16_hi(a.y);
-        v[4] = bf16_lo(b.x); v[5] = bf16_hi(b.x); v[6] = bf16_lo(b.y); v[7] = bf16_hi(b.y);
-        v[8] = bf16_lo(c.x); v[9] = bf16_hi(c.x);
-    }
-}
 
-// Surface record at a byte offset (48 B float, 24 B bfloat16).
-template <bool BF16>
+// Surface record at a byte offset: 12 floats (48 B) in both met layouts. The
+// surface scalars set the stability regime (air density from surface
+// pressure, Obukhov length, w*), so they are not rounded to bfloat16; only the
+// trilinear wind and support records are.
 __device__ __forceinline__ void load_surf_at(const void* base, uint32_t off, float (&v)[10]) {
-    const char* b = static_cast<const char*>(base) + off;
-    if constexpr (!BF16) {
-        const float4* p = reinterpret_cast<const float4*>(b);
-        const float4 a = p[0], c1 = p[1], c2 = p[2];
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = c1.x;
-        v[5] = c1.y; v[6] = c1.z; v[7] = c1.w; v[8] = c2.x; v[9] = c2.y;
-    } else {
-        const uint2* p = reinterpret_cast<const uint2*>(b);
-        const uint2 a = p[0], c1 = p[1], c2 = p[2];
-        v[0] = bf16_lo(a.x); v[1] = bf16_hi(a.x); v[2] = bf16_lo(a.y); v[3] = bf16_hi(a.y);
-        v[4] = bf16_lo(c1.x); v[5] = bf16_hi(c1.x); v[6] = bf16_lo(c1.y); v[7] = bf16_hi(c1.y);
-        v[8] = bf16_lo(c2.x); v[9] = bf16_hi(c2.x);
-    }
+    const float4* p = reinterpret_cast<const float4*>(static_cast<const char*>(base) + off);
+    const float4 a = p[0], c1 = p[1], c2 = p[2];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = c1.x;
+    v[5] = c1.y; v[6] = c1.z; v[7] = c1.w; v[8] = c2.x; v[9] = c2.y;
 }
-template <bool BF16>
-constexpr uint32_t kSurfBytes = BF16 ? 24u : 48u;
+constexpr uint32_t kSurfBytes = 48u;
 
 // torch.lerp(start, end, weight) with a scalar weight: two-sided form.
 __device__ __forceinline__ float torch_lerp(float a, float b, float w) {

@@ -154,8 +154,7 @@ __global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
     if (p.flags & GLIDE_F_MASK_INPUT) {
         active = s.active[i] != 0;
     } else {
-        const float ro = s.r_off[i];
-        active = (ro >= co) && (ro - static_cast<float>(p.length_s) <= co) && (s.alive[i] != 0);
+        active = in_window(s, static_cast<uint32_t>(i), co, static_cast<float>(p.length_s), p.flags);
     }
     if (!active) {
         if (emit) {
@@ -236,7 +235,7 @@ __global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
         a.e.vals[i] = s.weight[i] * static_cast<float>(p.dt);
         const bool esc = !within_domain(a.fp, lon, lat, z);
         if (esc) s.alive[i] = 0;
-        count_escapes(esc, a.e.escaped);
+        count_flags(esc, a.e.escaped);
     }
 }
 
@@ -380,6 +379,7 @@ extern "C" int glide_convect(const GlideStepParams* p, const GlideParticles* s, 
         GLIDE_REQUIRE(s->active, "active mask is NULL");
     } else {
         GLIDE_REQUIRE(s->r_off && s->alive, "r_off/alive are NULL (no explicit mask)");
+        GLIDE_REQUIRE(!(p->flags & GLIDE_F_RELEASE_TERMINUS) || s->lo_off, "GLIDE_F_RELEASE_TERMINUS needs lo_off");
     }
     ConvectArgs a{};
     a.p = *p;

@@ -81,8 +81,8 @@ __device__ __forceinline__ bool within_domain(const FpDev& g, float lon, float l
            lat <= g.dom_lat_max && z <= g.dom_alt_max;
 }
 
-// Wave-aggregated add of per-lane escape flags into *ctr.
-__device__ __forceinline__ void count_escapes(bool esc, unsigned long long* ctr) {
+// Adds the number of lanes with flag set to *ctr: wave ballot, one integer atomic per wave.
+__device__ __forceinline__ void count_flags(bool esc, unsigned long long* ctr) {
     const unsigned long long m = __ballot(esc);
     if (m == 0ull) return;
     const unsigned long long act = __ballot(1);

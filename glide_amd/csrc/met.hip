@@ -281,18 +281,11 @@ __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
     sr[9] = at(a.e3, 3, 0);
     sr[10] = 0.0f;
     sr[11] = 0.0f;
-    if constexpr (!BF16) {
-        float4* p = static_cast<float4*>(a.surf) + col * 3;
-        p[0] = make_float4(sr[0], sr[1], sr[2], sr[3]);
-        p[1] = make_float4(sr[4], sr[5], sr[6], sr[7]);
-        p[2] = make_float4(sr[8], sr[9], sr[10], sr[11]);
-    } else {
-        uint2* p = static_cast<uint2*>(a.surf) + col * 3;
-        for (int q = 0; q < 3; ++q) {
-            p[q] = make_uint2(f32_to_bf16_bits(sr[4 * q]) | (f32_to_bf16_bits(sr[4 * q + 1]) << 16),
-                              f32_to_bf16_bits(sr[4 * q + 2]) | (f32_to_bf16_bits(sr[4 * q + 3]) << 16));
-        }
-    }
+    // float in both layouts (the surface scalars are not rounded to bfloat16)
+    float4* p = static_cast<float4*>(a.surf) + col * 3;
+    p[0] = make_float4(sr[0], sr[1], sr[2], sr[3]);
+    p[1] = make_float4(sr[4], sr[5], sr[6], sr[7]);
+    p[2] = make_float4(sr[8], sr[9], sr[10], sr[11]);
 }
 
 }  // namespace glide

@@ -3,10 +3,12 @@
 // The fused step kernel gathers its met records per particle; when the 64
 // lanes of a wavefront hold particles in unrelated cells every lane touches its
 // own cache lines and the gathers become L2-bandwidth bound. Sorting the
-// particle state by met cell (in the records' memory order: level, row,
-// column) puts each wavefront's particles in a few neighbouring cells, so the
-// corner loads coalesce. Particles that can never be active again (escaped, or
-// past their backward window) sort to the end.
+// particle state by met cell puts each wavefront's particles in a few
+// neighbouring cells, so the corner loads coalesce. The key is column-major
+// (row, column, then level): particles mix vertically much faster than they
+// move horizontally, so a column key keeps its locality longer and a re-sort
+// moves most particles only a short way. Particles that can never be active
+// again (escaped, or past their backward window) sort to the end.
 //
 // Results do not depend on the order: noise is keyed by the particle id, which
 // moves with the particle.
@@ -20,9 +22,11 @@ struct ReorderKeyArgs {
     int nx, ny, nz;
     const float* levels;
     float cursor_off, length_s;
+    bool terminus;  // GLIDE_F_RELEASE_TERMINUS: lo_off holds the release terminus
     uint32_t sentinel;
     uint32_t* keys;
     uint32_t* idx;
+    uint32_t* live;  // optional count of particles that can still become active
 };
 
 constexpr int kMaxLev = 128;
@@ -34,17 +38,31 @@ __global__ __launch_bounds__(256) void reorder_key_kernel(const ReorderKeyArgs a
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.s.n) return;
     const GlideParticles& s = a.s;
-    const bool done = (s.alive[i] == 0) || (s.r_off[i] - a.length_s > a.cursor_off);
+    // can never be active again: the cursor only decreases
+    bool over;
+    if (a.terminus) {
+        over = s.lo_off[i] >= a.cursor_off;  // active needs cursor > te
+    } else {
+        const float lo = s.lo_off ? s.lo_off[i] : s.r_off[i] - a.length_s;
+        over = lo > a.cursor_off;
+    }
+    const bool done = (s.alive[i] == 0) || over;
     uint32_t key = a.sentinel;
     if (!done) {
         const int64_t ps = s.pos_stride;
         const int ix = static_cast<int>(frac_index(s.lon[i * ps], a.lon0, a.sx, a.nx));
         const int iy = static_cast<int>(frac_index(s.lat[i * ps], a.lat0, a.sy, a.ny));
         const int iz = static_cast<int>(level_index(lev, a.nz, s.z[i * ps]));
-        key = static_cast<uint32_t>((static_cast<int64_t>(iz) * a.ny + iy) * a.nx + ix);
+        key = static_cast<uint32_t>((static_cast<int64_t>(iy) * a.nx + ix) * a.nz + iz);
     }
     a.keys[i] = key;
     a.idx[i] = static_cast<uint32_t>(i);
+    if (a.live != nullptr) {
+        // one integer atomic per wavefront
+        const unsigned long long m = __ballot(!done);
+        if (m != 0ull && (threadIdx.x & 63) == static_cast<unsigned>(__ffsll(static_cast<long long>(m)) - 1))
+            atomicAdd(a.live, static_cast<uint32_t>(__popcll(m)));
+    }
 }
 
 struct PermuteArgs {
@@ -77,6 +95,7 @@ __global__ __launch_bounds__(256) void permute_kernel(const PermuteArgs a) {
     mv(s.rel_idx, const_cast<int32_t*>(d.rel_idx), j, i);
     mv(s.uid, const_cast<uint32_t*>(d.uid), j, i);
     mv(s.alive, d.alive, j, i);
+    mv(s.lo_off, const_cast<float*>(d.lo_off), j, i);
 }
 
 }  // namespace glide
@@ -84,7 +103,7 @@ __global__ __launch_bounds__(256) void permute_kernel(const PermuteArgs a) {
 using namespace glide;
 
 extern "C" int glide_reorder_keys(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
-                                  uint32_t* keys, uint32_t* idx, void* stream) {
+                                  uint32_t* keys, uint32_t* idx, uint32_t* live, void* stream) {
     GLIDE_REQUIRE(p && s && g && keys && idx, "glide_reorder_keys: NULL argument");
     GLIDE_REQUIRE(s->lon && s->lat && s->z && s->alive && s->r_off, "particle arrays are NULL");
     GLIDE_REQUIRE(s->pos_stride >= 1 && s->n >= 0, "bad particles");
@@ -103,9 +122,12 @@ extern "C" int glide_reorder_keys(const GlideStepParams* p, const GlideParticles
     a.levels = g->levels;
     a.cursor_off = static_cast<float>(p->cursor_off);
     a.length_s = static_cast<float>(p->length_s);
+    a.terminus = (p->flags & GLIDE_F_RELEASE_TERMINUS) != 0;
+    GLIDE_REQUIRE(!a.terminus || s->lo_off, "GLIDE_F_RELEASE_TERMINUS needs lo_off");
     a.sentinel = static_cast<uint32_t>(static_cast<int64_t>(g->nz) * g->ny * g->nx);
     a.keys = keys;
     a.idx = idx;
+    a.live = live;
     hipLaunchKernelGGL(reorder_key_kernel, dim3(static_cast<unsigned>((s->n + 255) / 256)), dim3(256), 0,
                        static_cast<hipStream_t>(stream), a);
     GLIDE_HIP(hipGetLastError());

@@ -32,9 +32,11 @@ size_t sort_layout(int64_t n, SortWork* w, char* base);
 
 // Sorts by the low key_bits of the key. compact_limit > 0 keeps only the pairs
 // with key < compact_limit (order-preserving; the first pass does it). The
-// sorted arrays are in the workspace (returned) and the item count is left in
-// w.count.
+// item count is left in w.count. The last pass writes into (dst_keys,
+// dst_vals) when both are given, else into the workspace; *out_keys /
+// *out_vals point at the result.
 void radix_sort_pairs(const uint32_t* keys, const uint32_t* vals, int64_t n, int key_bits, uint32_t compact_limit,
-                      const SortWork& w, hipStream_t st, const uint32_t** out_keys, const uint32_t** out_vals);
+                      const SortWork& w, hipStream_t st, const uint32_t** out_keys, const uint32_t** out_vals,
+                      uint32_t* dst_keys = nullptr, uint32_t* dst_vals = nullptr);
 
 }  // namespace glide

@@ -226,7 +226,8 @@ static void launch_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout
 }
 
 void radix_sort_pairs(const uint32_t* keys, const uint32_t* vals, int64_t n, int key_bits, uint32_t compact_limit,
-                      const SortWork& w, hipStream_t st, const uint32_t** out_keys, const uint32_t** out_vals) {
+                      const SortWork& w, hipStream_t st, const uint32_t** out_keys, const uint32_t** out_vals,
+                      uint32_t* dst_keys, uint32_t* dst_vals) {
     const int64_t ts = div_up(n, kSortTile);
     if (key_bits < 1) key_bits = 1;
     const int npass = static_cast<int>(div_up(key_bits, kMaxDigitBits));
@@ -240,6 +241,10 @@ void radix_sort_pairs(const uint32_t* keys, const uint32_t* vals, int64_t n, int
         const int rb = bits < kMinDigitBits ? kMinDigitBits : bits;
         uint32_t* kout = slot ? w.keys_b : w.keys_a;
         uint32_t* vout = slot ? w.vals_b : w.vals_a;
+        if (ps == npass - 1 && dst_keys != nullptr && dst_vals != nullptr) {
+            kout = dst_keys;
+            vout = dst_vals;
+        }
         const bool first = ps == 0;
         const uint32_t* n_dev = first ? nullptr : w.count;
         const uint32_t limit = first ? compact_limit : 0u;
@@ -281,11 +286,11 @@ extern "C" int glide_sort_pairs(const uint32_t* keys, const uint32_t* vals, int6
     SortWork sw{};
     sort_layout(n, &sw, static_cast<char*>(workspace));
     hipStream_t st = static_cast<hipStream_t>(stream);
+    GLIDE_REQUIRE(out_keys != keys && out_vals != vals && out_keys != vals && out_vals != keys,
+                  "glide_sort_pairs cannot sort in place");
     const uint32_t* sk = nullptr;
     const uint32_t* sv = nullptr;
-    radix_sort_pairs(keys, vals, n, key_bits, 0u, sw, st, &sk, &sv);
-    GLIDE_HIP(hipMemcpyAsync(out_keys, sk, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
-    GLIDE_HIP(hipMemcpyAsync(out_vals, sv, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+    radix_sort_pairs(keys, vals, n, key_bits, 0u, sw, st, &sk, &sv, out_keys, out_vals);
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
 }

@@ -32,10 +32,11 @@ struct StepArgs {
     FpDev fp;
     GlideEmit e;
     // Table mode: the step's scalars come from table[row[0]] on the device, the
-    // escape count goes to e.escaped[table[row[0]].step_index], and the last
-    // workgroup to finish advances row[0] (row[1..] count finished workgroups).
+    // escape count goes to e.escaped[row[0]], and the last workgroup to finish
+    // advances row[0] (row[1..] count finished workgroups).
     const GlideStepParams* table;
     int32_t* row;
+    unsigned long long* n_active;  // optional per-step active count (e->active)
 };
 
 constexpr int kMaxLevels = 128;
@@ -44,10 +45,10 @@ constexpr uint32_t kRowGroups = GLIDE_ROW_WORDS - 2;  // finished-workgroup coun
 
 // The step's row of the table, read into scalar registers (one readfirstlane
 // per 32-bit word), so everything derived from it stays scalar.
-__device__ __forceinline__ GlideStepParams load_row(const GlideStepParams* table, const int32_t* row) {
+__device__ __forceinline__ GlideStepParams load_row(const GlideStepParams* table, const int32_t* row, int& r) {
     static_assert(sizeof(GlideStepParams) % 4 == 0, "GlideStepParams is a whole number of words");
     constexpr int kWords = sizeof(GlideStepParams) / 4;
-    const int r = __builtin_amdgcn_readfirstlane(*row);
+    r = __builtin_amdgcn_readfirstlane(*row);
     const uint32_t* w = reinterpret_cast<const uint32_t*>(table + r);
     uint32_t buf[kWords];
 #pragma unroll
@@ -57,8 +58,9 @@ __device__ __forceinline__ GlideStepParams load_row(const GlideStepParams* table
     return p;
 }
 
+// Advances particle i; returns whether it was active.
 template <bool BF16>
-__device__ __forceinline__ void step_particle(const StepArgs& a, const float* lev, int64_t i,
+__device__ __forceinline__ bool step_particle(const StepArgs& a, const float* lev, int64_t i,
                                               const GlideStepParams& p, unsigned long long* escaped) {
     // 32-bit byte offsets of this particle (the host checks the arrays fit)
     const uint32_t i1 = static_cast<uint32_t>(i);
@@ -72,15 +74,14 @@ __device__ __forceinline__ void step_particle(const StepArgs& a, const float* le
     if (p.flags & GLIDE_F_MASK_INPUT) {
         active = ldo(s.active, i1) != 0;
     } else {
-        const float ro = ldo(s.r_off, i4);
-        active = (ro >= co) && (ro - static_cast<float>(p.length_s) <= co) && (ldo(s.alive, i1) != 0);
+        active = in_window(s, i1, co, static_cast<float>(p.length_s), p.flags);
     }
     if (!active) {
         if (emit) {
             sto(a.e.keys, i4, a.fp.n_cells);
             sto(a.e.vals, i4, 0.0f);
         }
-        return;
+        return false;
     }
 
     const int64_t ps = s.pos_stride;
@@ -135,14 +136,14 @@ __device__ __forceinline__ void step_particle(const StepArgs& a, const float* le
             const float w00 = (1.0f - fx) * (1.0f - fy), w01 = fx * (1.0f - fy);
             const float w10 = (1.0f - fx) * fy, w11 = fx * fy;
             float acc[5] = {0, 0, 0, 0, 0};
-            constexpr uint32_t sb = kSurfBytes<BF16>;
+            constexpr uint32_t sb = kSurfBytes;
             const uint32_t r0 = static_cast<uint32_t>(y0 * nx), r1 = static_cast<uint32_t>(y1 * nx);
             const uint32_t offs[4] = {(r0 + x0) * sb, (r0 + x1) * sb, (r1 + x0) * sb, (r1 + x1) * sb};
             const float ws[4] = {w00, w01, w10, w11};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 float r[10];
-                load_surf_at<BF16>(a.surf, offs[q], r);
+                load_surf_at(a.surf, offs[q], r);
 #pragma unroll
                 for (int ch = 0; ch < 5; ++ch) acc[ch] += ws[q] * torch_lerp(r[ch], r[5 + ch], al);
             }
@@ -295,8 +296,9 @@ __device__ __forceinline__ void step_particle(const StepArgs& a, const float* le
         sto(a.e.vals, i4, ldo(s.weight, i4) * dt);
         const bool esc = !within_domain(a.fp, lon, lat, z);
         if (esc) sto(s.alive, i1, static_cast<uint8_t>(0));
-        count_escapes(esc, escaped);
+        count_flags(esc, escaped);
     }
+    return true;
 }
 
 template <bool BF16>
@@ -306,12 +308,28 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
     __syncthreads();
     GlideStepParams p = a.p;
     unsigned long long* escaped = a.e.escaped;
+    unsigned long long* n_active = a.n_active;
     if (a.table != nullptr) {
-        p = load_row(a.table, a.row);
-        escaped += p.step_index;
+        int r = 0;
+        p = load_row(a.table, a.row, r);
+        escaped += r;
+        if (n_active != nullptr) n_active += r;
     }
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < a.s.n) step_particle<BF16>(a, lev, i, p, escaped);
+    bool act = false;
+    if (i < a.s.n) act = step_particle<BF16>(a, lev, i, p, escaped);
+    if (n_active != nullptr) {
+        // active count: wave ballots summed in LDS, one atomic per workgroup at the end
+        __shared__ uint32_t wact[kStepBlock / 64];
+        const unsigned long long m = __ballot(act);
+        if ((threadIdx.x & 63) == 0) wact[threadIdx.x >> 6] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t c = 0;
+            for (int w = 0; w < kStepBlock / 64; ++w) c += wact[w];
+            if (c) atomicAdd(n_active, static_cast<unsigned long long>(c));
+        }
+    }
     if (a.table != nullptr) {
         // Every workgroup has used row[0] before it counts itself finished, so
         // the last one to finish can advance the cursor (no waiting). The count
@@ -395,12 +413,14 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
         GLIDE_REQUIRE(s->active, "active mask is NULL");
     } else {
         GLIDE_REQUIRE(s->r_off && s->alive, "r_off/alive are NULL (no explicit mask)");
+        GLIDE_REQUIRE(!(p->flags & GLIDE_F_RELEASE_TERMINUS) || s->lo_off, "GLIDE_F_RELEASE_TERMINUS needs lo_off");
     }
     StepArgs a{};
     a.p = *p;
     a.s = *s;
     a.table = table;
     a.row = row;
+    a.n_active = e ? e->active : nullptr;
     int rc = fill_grid(a, *g, *w);
     if (rc) return rc;
     if (p->flags & GLIDE_F_EMIT) {

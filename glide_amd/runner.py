@@ -101,7 +101,7 @@ class DeviceMet:
         esize = 2 if met_dtype == L.GLIDE_MET_BF16 else 4
         self.wind = torch.empty(ncell * 8 * esize, dtype=torch.uint8, device=device)
         self.aux = torch.empty(ncell * 8 * esize, dtype=torch.uint8, device=device)
-        self.surf = torch.empty(self.ny * self.nx * 12 * esize, dtype=torch.uint8, device=device)
+        self.surf = torch.empty(self.ny * self.nx * 12 * 4, dtype=torch.uint8, device=device)  # float in both
         self.ring = {}
         self.window_hour = None
         self.grid = L.GlideGrid(nx=self.nx, ny=self.ny, nz=self.nz, reserved=0,
@@ -152,19 +152,32 @@ class BatchEngine:
     def __init__(self, cfg: dict, *, device="cuda", met: SyntheticMet | None = None,
                  met_dtype: int = L.GLIDE_MET_F32, hanna: HannaOptions | None = None,
                  convection: ConvectionOptions | None = None, compensated: bool = False,
-                 device_met: DeviceMet | None = None, reorder_every: int = 0, use_graph: bool = False):
+                 device_met: DeviceMet | None = None, reorder_every: int = 0, use_graph: bool = False,
+                 compact: bool = False, release_terminus: bool = False):
         self.cfg = cfg
         self.reorder_every = int(reorder_every)
+        # compact: on every new met hour, sort the particles by met cell with the
+        # ones that can never be active again (finished releases, escaped) last,
+        # and launch over the live prefix only
+        self.compact = bool(compact)
+        # release_terminus: each release is stepped only while the cursor is
+        # above its own terminus, as when it runs alone (its result does not
+        # depend on the batch); default: the reference's batch terminus
+        self.release_terminus = bool(release_terminus)
+        self.extra_flags = L.F_RELEASE_TERMINUS if self.release_terminus else 0
         self.use_graph = bool(use_graph)
-        self._graphs: dict[int, int] = {}
+        self._graphs: dict[tuple[int, int], int] = {}
         self.graph_launches = 0
         self.device = device
         self.h = hanna or HannaOptions()
         self.conv_opts = convection if convection is not None else (
             ConvectionOptions() if cfg.get("convection") else None)
         rels = cfg["releases"]
-        self.length_s = int(rels[0].length_s)
-        self.sched = cursor_schedule(rels, int(cfg["dt"]), self.length_s)
+        self.length_s = max(int(r.length_s) for r in rels)
+        # releases of different backward lengths pass each particle's earliest
+        # active offset (lo_off); one length keeps the scalar form
+        self.per_particle_length = len({int(r.length_s) for r in rels}) > 1 or self.release_terminus
+        self.sched = cursor_schedule(rels, int(cfg["dt"]))
         hours = sorted({s.bracket_hour for s in self.sched} | {s.bracket_hour + 1 for s in self.sched})
         self.met = device_met or DeviceMet(met or SyntheticMet(cfg["spec"]), hours, device=device,
                                            met_dtype=met_dtype,
@@ -173,6 +186,7 @@ class BatchEngine:
         self.seed = int(cfg["seed"])
         self.ra = release_arrays(rels)
         self.n = int(self.ra.lon.size)
+        self.n_live = self.n
         g = cfg["grid"]
         dom = cfg["domain"]
         self.grid_cfg = g
@@ -190,7 +204,7 @@ class BatchEngine:
         self.reset()
 
     STATE = ("lon", "lat", "z", "up", "vp", "wp", "um", "vm", "weight", "r_off", "we_off", "rel_idx",
-             "uid", "alive")
+             "uid", "alive", "lo_off")
 
     def _alloc(self):
         d = self.device
@@ -205,6 +219,7 @@ class BatchEngine:
         self.keys = torch.empty(n, dtype=torch.int32, device=d)
         self.vals = torch.empty(n, dtype=f32, device=d)
         self.escaped = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)
+        self.active_steps = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)  # per-step actives
         self.footprint = torch.zeros(self.n_cells, dtype=f32, device=d)
         self.comp = torch.zeros(self.n_cells, dtype=f32, device=d) if self.compensated else None
         self.release_total = torch.zeros(self.grid_cfg["n_r"], dtype=torch.float64, device=d)
@@ -223,6 +238,7 @@ class BatchEngine:
         self.m_b = torch.empty(nc, dtype=f32, device=d)
         self.layer_mass = torch.empty(self.met.nz, dtype=f32, device=d)
         self.colsum_err = torch.zeros(nc, dtype=torch.float64, device=d)
+        self.live = torch.zeros(1, dtype=torch.int32, device=d)
         # Step table for the captured step body: one GlideStepParams row per
         # schedule step (the flags of a plain emitting step), and the device
         # cursor that the graph advances.
@@ -238,11 +254,19 @@ class BatchEngine:
             up=b["up"].data_ptr(), vp=b["vp"].data_ptr(), wp=b["wp"].data_ptr(), um=b["um"].data_ptr(),
             vm=b["vm"].data_ptr(), weight=b["weight"].data_ptr(), r_off=b["r_off"].data_ptr(),
             we_off=b["we_off"].data_ptr(), rel_idx=b["rel_idx"].data_ptr(), uid=b["uid"].data_ptr(),
-            alive=b["alive"].data_ptr(), active=None)
+            alive=b["alive"].data_ptr(), active=None,
+            lo_off=b["lo_off"].data_ptr() if self.per_particle_length else None)
 
     @property
     def parts(self) -> L.GlideParticles:
-        return self._parts[self.cur]
+        """The current buffer's particles, limited to the live prefix."""
+
+        p = self._parts[self.cur]
+        if self.n_live == self.n:
+            return p
+        q = L.GlideParticles(**{f: getattr(p, f) for f, _ in L.GlideParticles._fields_})
+        q.n = self.n_live
+        return q
 
     def __getattr__(self, name):
         # current-buffer state arrays: eng.lon, eng.z, eng.alive, ...
@@ -258,29 +282,37 @@ class BatchEngine:
         b = self.bufs[0]
         ra = self.ra
         for k, v in (("lon", ra.lon), ("lat", ra.lat), ("z", ra.z), ("weight", ra.weight), ("r_off", ra.r_off),
-                     ("we_off", ra.we_off), ("rel_idx", ra.rel_idx), ("uid", ra.uid.view(np.int32))):
+                     ("we_off", ra.we_off), ("rel_idx", ra.rel_idx), ("uid", ra.uid.view(np.int32)),
+                     ("lo_off", ra.te_off if self.release_terminus else ra.lo_off)):
             b[k].copy_(torch.from_numpy(np.ascontiguousarray(v)))
         for k in ("up", "vp", "wp", "um", "vm"):
             b[k].zero_()
         b["alive"].fill_(1)
         self.escaped.zero_()
+        self.active_steps.zero_()
         self.footprint.zero_()
         if self.comp is not None:
             self.comp.zero_()
         self.release_total.zero_()
         self.step_i = 0
+        self.n_live = self.n
         self.last_bracket = None
         self.step_events = None
         self.row.zero_()
 
-    def reorder(self, st, stream: int) -> None:
-        """Sort the particle state by met cell (gather locality); particles that
-        can never be active again go to the end."""
+    def reorder(self, st, stream: int, count_live: bool = False) -> None:
+        """Sort the whole particle state by met cell (gather locality);
+        particles that can never be active again go to the end. With
+        ``count_live`` the launches that follow cover only the particles that
+        can still become active (one device-to-host read of the count)."""
 
         lib = L.lib()
         p = self.step_params(st, 0)
-        L.check(lib.glide_reorder_keys(C.byref(p), C.byref(self.parts), C.byref(self.met.grid),
-                                       self.ro_keys.data_ptr(), self.ro_idx.data_ptr(), stream))
+        if count_live:
+            self.live.zero_()
+        L.check(lib.glide_reorder_keys(C.byref(p), C.byref(self._parts[self.cur]), C.byref(self.met.grid),
+                                       self.ro_keys.data_ptr(), self.ro_idx.data_ptr(),
+                                       self.live.data_ptr() if count_live else None, stream))
         L.check(lib.glide_sort_pairs(self.ro_keys.data_ptr(), self.ro_idx.data_ptr(), self.n, self.ro_bits,
                                      self.ro_keys_s.data_ptr(), self.ro_idx_s.data_ptr(), self.sort_ws.data_ptr(),
                                      self.sort_ws.numel(), stream))
@@ -288,18 +320,21 @@ class BatchEngine:
         L.check(lib.glide_permute_particles(C.byref(self._parts[self.cur]), C.byref(self._parts[nxt]),
                                             self.ro_idx_s.data_ptr(), stream))
         self.cur = nxt
+        if count_live:
+            self.n_live = int(self.live.item())
 
     def step_params(self, st, flags: int) -> L.GlideStepParams:
         h = self.h
         return L.GlideStepParams(
             dt=float(st.delta_s), alpha=float(st.alpha), cursor_off=float(st.cursor_off),
             length_s=float(self.length_s), substep_c=h.substep_c, z_ubl_m=h.z_ubl_m,
-            meander_timescale_s=h.meander_timescale_s, seed=self.seed, step_index=st.index,
-            max_substeps=h.max_substeps, flags=flags, reserved=0)
+            meander_timescale_s=h.meander_timescale_s, seed=self.seed, step_index=st.noise_step,
+            max_substeps=h.max_substeps, flags=flags | self.extra_flags, reserved=0)
 
     def emit(self, st) -> L.GlideEmit:
-        esc = self.escaped.data_ptr() + 8 * st.index
-        return L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(), escaped=esc)
+        return L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(),
+                           escaped=self.escaped.data_ptr() + 8 * st.index,
+                           active=self.active_steps.data_ptr() + 8 * st.index)
 
     def _capture(self, stream: int) -> int:
         """Capture the plain step body (table-driven step, which also advances
@@ -309,7 +344,8 @@ class BatchEngine:
         lib = L.lib()
         if stream == 0:
             raise ValueError("graph capture needs a non-default stream")
-        em = L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(), escaped=self.escaped.data_ptr())
+        em = L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(), escaped=self.escaped.data_ptr(),
+                         active=self.active_steps.data_ptr())
         L.check(lib.glide_graph_begin(stream))
         try:
             L.check(lib.glide_step_table(C.byref(self.step_tmpl), self.step_table.data_ptr(), self.row.data_ptr(),
@@ -323,8 +359,10 @@ class BatchEngine:
         return h.value
 
     def _accumulate(self, stream: int) -> None:
+        if self.n_live == 0:
+            return
         L.check(L.lib().glide_footprint_accumulate(
-            self.keys.data_ptr(), self.vals.data_ptr(), self.n, self.n_cells, self.cells_per_release,
+            self.keys.data_ptr(), self.vals.data_ptr(), self.n_live, self.n_cells, self.cells_per_release,
             self.footprint.data_ptr(), self.comp.data_ptr() if self.comp is not None else None,
             self.release_total.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(), stream))
 
@@ -348,16 +386,21 @@ class BatchEngine:
         if stream is None:
             stream = L.stream_ptr()
         st = self.sched[self.step_i]
-        if self.reorder_every and self.step_i > 0 and self.step_i % self.reorder_every == 0:
-            self.reorder(st, stream)
         new_bracket = st.bracket_hour != self.last_bracket
+        if self.compact and new_bracket and self.step_i > 0:
+            self.reorder(st, stream, count_live=True)
+        elif self.reorder_every and self.step_i > 0 and self.step_i % self.reorder_every == 0:
+            self.reorder(st, stream)
         self.met.build_window(st.bracket_hour, stream)
         convect = new_bracket and self.conv_opts is not None
         ev = self.step_events.get(self.step_i) if self.step_events is not None else None
         if self.use_graph and not convect and ev is None:
-            g = self._graphs.get(self.cur)
+            key = (self.cur, self.n_live)
+            g = self._graphs.get(key)
             if g is None:
-                g = self._graphs[self.cur] = self._capture(stream)
+                for k in [k for k in self._graphs if k[1] != self.n_live]:  # a shrunken live set
+                    L.lib().glide_graph_destroy(self._graphs.pop(k))
+                g = self._graphs[key] = self._capture(stream)
             L.check(lib.glide_graph_launch(g, stream))
             self.graph_launches += 1
             self.last_bracket = st.bracket_hour
@@ -413,8 +456,13 @@ class BatchEngine:
             self.run_step()
 
     def active_count(self, st) -> int:
+        """Particles in their release window at step ``st`` (before escapes)."""
+
         co = np.float32(st.cursor_off)
-        act = (self.ra.r_off >= co) & (self.ra.r_off - np.float32(self.length_s) <= co)
+        if self.release_terminus:
+            act = (self.ra.r_off >= co) & (co > self.ra.te_off)
+        else:
+            act = (self.ra.r_off >= co) & (self.ra.lo_off <= co)
         return int(act.sum())
 
     def state_numpy(self) -> dict:
@@ -435,8 +483,10 @@ class BatchEngine:
         self.cur = 0
         b = self.bufs[0]
         ra = self.ra
+        self.n_live = self.n
         for k, v in (("weight", ra.weight), ("r_off", ra.r_off), ("we_off", ra.we_off), ("rel_idx", ra.rel_idx),
-                     ("uid", ra.uid.view(np.int32))):
+                     ("uid", ra.uid.view(np.int32)),
+                     ("lo_off", ra.te_off if self.release_terminus else ra.lo_off)):
             b[k].copy_(torch.from_numpy(np.ascontiguousarray(v)))
         for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm"):
             b[k].copy_(torch.from_numpy(np.asarray(S[k], np.float32)))

@@ -20,17 +20,20 @@ class StepInfo:
     cursor_off: float  # cursor - batch start
     bracket_hour: int  # hour_floor(cursor) as an hour index
     alpha: float
+    noise_step: int  # Philox counter word 1: cursor_s // dt (absolute time, not the batch's step count)
 
 
-def cursor_schedule(releases, dt: int, length_s: int) -> list[StepInfo]:
+def cursor_schedule(releases, dt: int, length_s: int | None = None) -> list[StepInfo]:
     """Steps of one batch: from the latest window end down to the earliest
-    window end minus the backward length, ``while t_cursor > terminus``."""
+    (window end - backward length) over the releases, ``while t_cursor >
+    terminus``. ``length_s`` overrides every release's own length."""
 
     if dt <= 0:
         raise ValueError("dt_seconds must be > 0")
     ends = [r.release_time_s + r.duration_s for r in releases]
+    lens = [int(length_s) if length_s is not None else int(r.length_s) for r in releases]
     start = max(ends)
-    term = min(ends) - int(length_s)
+    term = min(e - ln for e, ln in zip(ends, lens))
     batch_start = min(r.release_time_s for r in releases)
     out = []
     t = start
@@ -39,7 +42,7 @@ def cursor_schedule(releases, dt: int, length_s: int) -> list[StepInfo]:
         d = min(dt, t - term)
         bh = t // 3600
         alpha = max(0.0, min(1.0, ((t - 0.5 * d) - 3600.0 * bh) / 3600.0))
-        out.append(StepInfo(i, int(t), int(d), float(t - batch_start), int(bh), alpha))
+        out.append(StepInfo(i, int(t), int(d), float(t - batch_start), int(bh), alpha, int(t // dt)))
         t -= d
         i += 1
     return out
@@ -55,6 +58,8 @@ class ReleaseArrays:
     we_off: np.ndarray
     rel_idx: np.ndarray
     uid: np.ndarray
+    lo_off: np.ndarray  # r_off - backward length (float32): earliest active cursor offset
+    te_off: np.ndarray  # window end - backward length (float32): the release's own terminus
     batch_start_s: int
 
 
@@ -69,14 +74,31 @@ def release_offsets(seed: int, n: int, duration_s: float) -> np.ndarray:
     return torch.empty(n, dtype=torch.float32).uniform_(0.0, float(duration_s), generator=g).numpy()
 
 
-def release_arrays(releases, uid_base: int = 0) -> ReleaseArrays:
+NOISE_ID_BITS = 12  # particle id = (noise_id % 4096) << 20 | k
+
+
+def particle_ids(release) -> np.ndarray:
+    """Philox counter word 0 of a release's particles: (noise id % 4096) << 20 |
+    k for k < 2^20, so a release draws the same noise alone or in any batch of
+    up to 4,096 releases."""
+
+    n = int(release.n_particles)
+    if n > (1 << 20):
+        if n > (1 << 32) - 1:
+            raise ValueError("a release holds at most 2^32 - 1 particles")
+        return np.arange(n, dtype=np.uint32)  # one huge release: its own id space
+    nid = release.release_idx if getattr(release, "noise_id", None) is None else release.noise_id
+    return ((np.uint32(int(nid) % (1 << NOISE_ID_BITS)) << np.uint32(20)) + np.arange(n, dtype=np.uint32)).astype(np.uint32)
+
+
+def release_arrays(releases) -> ReleaseArrays:
     """Particle arrays for a batch of point releases (weight 1/N per release,
     offsets relative to the earliest release in the batch)."""
 
     if not releases:
         raise ValueError("ReleaseBatch must contain at least one release")
     batch_start = min(r.release_time_s for r in releases)
-    cols = {k: [] for k in ("lon", "lat", "z", "weight", "r_off", "we_off", "rel_idx")}
+    cols = {k: [] for k in ("lon", "lat", "z", "weight", "r_off", "we_off", "rel_idx", "lo_off", "te_off")}
     for r in releases:
         n = int(r.n_particles)
         if n <= 0:
@@ -86,10 +108,12 @@ def release_arrays(releases, uid_base: int = 0) -> ReleaseArrays:
         cols["lat"].append(np.full(n, r.lat, np.float32))
         cols["z"].append(np.full(n, r.alt_agl_m, np.float32))
         cols["weight"].append(np.full(n, 1.0 / n, np.float32))
-        cols["r_off"].append(release_offsets(r.seed, n, r.duration_s) + np.float32(off))
+        ro = release_offsets(r.seed, n, r.duration_s) + np.float32(off)
+        cols["r_off"].append(ro)
+        cols["lo_off"].append(ro - np.float32(r.length_s))
+        cols["te_off"].append(np.full(n, np.float32(off + r.duration_s) - np.float32(r.length_s), np.float32))
         cols["we_off"].append(np.full(n, off + r.duration_s, np.float32))
         cols["rel_idx"].append(np.full(n, r.release_idx, np.int32))
     a = {k: np.concatenate(v) for k, v in cols.items()}
-    n_tot = a["lon"].size
-    return ReleaseArrays(uid=np.arange(uid_base, uid_base + n_tot, dtype=np.uint32),
-                         batch_start_s=batch_start, **a)
+    uid = np.concatenate([particle_ids(r) for r in releases])
+    return ReleaseArrays(uid=uid, batch_start_s=batch_start, **a)

@@ -258,41 +258,92 @@ class Release:
     duration_s: int = 60
     length_s: int = 3 * 3600  # backward integration length
     seed: int = 0
-    release_idx: int = 0
+    release_idx: int = 0  # footprint slab of the release within its batch
+    noise_id: int | None = None  # particle ids are (noise_id % 4096) << 20 | k; None = release_idx
+
+
+def _domain_and_grid(spec: MetSpec, n_r: int, n_t: int):
+    lon_lo, lon_hi = float(spec.lon.min()), float(spec.lon.max())
+    lat_lo, lat_hi = float(spec.lat.min()), float(spec.lat.max())
+    grid = dict(
+        lon_min=lon_lo, lon_max=lon_hi + abs(spec.dlon), lat_min=lat_lo,
+        lat_max=lat_hi + abs(spec.dlat), n_x=spec.nx, n_y=spec.ny, n_t=n_t, n_r=n_r,
+        z_edges=[0.0, 100.0], n_z=1,
+    )
+    domain = dict(lon_min=lon_lo, lon_max=lon_hi, lat_min=lat_lo, lat_max=lat_hi,
+                  alt_max=spec.alt_max_m)
+    return grid, domain
+
+
+def network_releases(spec: MetSpec, *, n_sites: int, n_hourly: int, first_hour: int, lengths_h,
+                     n_range: tuple[int, int], seed: int, duration_s: int = 60) -> list[Release]:
+    """A measurement network's hourly releases (the reference's multi-site
+    batches, scripts/make_multisite_config.py): ``n_sites`` sites at seeded
+    interior positions and inlet heights, one release per site per hour for
+    ``n_hourly`` hours from ``first_hour``, particle counts log-uniform in
+    ``n_range``, backward lengths drawn from ``lengths_h`` (hours)."""
+
+    rng = np.random.default_rng(seed)
+    lon, lat = spec.lon, spec.lat
+    out = []
+    idx = 0
+    for s in range(n_sites):
+        slon = float(rng.uniform(lon[len(lon) // 4], lon[3 * len(lon) // 4]))
+        slat = float(rng.uniform(min(lat[len(lat) // 4], lat[3 * len(lat) // 4]),
+                                 max(lat[len(lat) // 4], lat[3 * len(lat) // 4])))
+        inlet = float(rng.choice([10.0, 50.0, 100.0, 150.0]))
+        for k in range(n_hourly):
+            n = int(round(float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))))
+            length_h = int(rng.choice(np.asarray(lengths_h)))
+            out.append(Release(lon=slon, lat=slat, alt_agl_m=inlet, n_particles=n,
+                               release_time_s=(first_hour + k) * 3600 - 120, duration_s=duration_s,
+                               length_s=length_h * 3600, seed=seed + 1 + idx, release_idx=idx))
+            idx += 1
+    return out
 
 
 def workload_config(name: str):
     """(MetSpec, [Release], grid dict, n_substeps, convection flag) for the named
-    single-release workloads:
+    workloads:
 
     * ``plain-cpu-10k-3h``: 10,000 particles at 50 m, 3 h back, dt = 60 s, 64x48
       grid at 0.25 deg, 40 AGL levels to 20 km, 4 hourly snapshots, convection
       off, footprint 64x48 x one 0-100 m layer x 3 hourly time-ago bins.
     * ``plain-gpu-1M-24h``: 2^20 particles, 24 h back, 240x160 grid, 60 levels,
       25 snapshots, convection on, footprint 24 time-ago bins x one layer.
+    * ``batched-gpu-336r``: one GPU's share (336 of 2,688) of the network batch:
+      7 sites x 48 hourly releases, 2k-20k particles each (log-uniform), 24 to
+      120 h back, 240x160x60 met over 169 hourly snapshots, convection on,
+      footprint 336 releases x 120 time-ago bins x one layer.
+    * ``batched-cpu-6r``: a small network batch on the 64x48x40 grid: 2 sites x
+      3 hourly releases of 8k-16k particles, 1 to 3 h back, convection off.
     """
 
-    if name == "plain-cpu-10k-3h":
-        spec = MetSpec(nx=64, ny=48, za=40, n_hours=4, seed=20240101)
-        n, hours, n_t, conv = 10_000, 3, 3, False
-    elif name == "plain-gpu-1M-24h":
-        spec = MetSpec(nx=240, ny=160, za=60, n_hours=25, seed=20240102, lon0=2.0, lat0=72.0)
-        n, hours, n_t, conv = 1 << 20, 24, 24, True
-    else:
-        raise KeyError(f"unknown workload {name!r}")
-    lon_c = float(spec.lon[(spec.nx - 1) // 2])
-    lat_c = float(spec.lat[(spec.ny - 1) // 2])
-    length = hours * 3600
-    rel = Release(lon=lon_c + 0.1, lat=lat_c + 0.1, alt_agl_m=50.0, n_particles=n,
-                  release_time_s=length - 120, duration_s=60, length_s=length, seed=7)
-    lon_lo, lon_hi = float(spec.lon.min()), float(spec.lon.max())
-    lat_lo, lat_hi = float(spec.lat.min()), float(spec.lat.max())
-    grid = dict(
-        lon_min=lon_lo, lon_max=lon_hi + abs(spec.dlon), lat_min=lat_lo,
-        lat_max=lat_hi + abs(spec.dlat), n_x=spec.nx, n_y=spec.ny, n_t=n_t, n_r=1,
-        z_edges=[0.0, 100.0], n_z=1,
-    )
-    domain = dict(lon_min=lon_lo, lon_max=lon_hi, lat_min=lat_lo, lat_max=lat_hi,
-                  alt_max=spec.alt_max_m)
-    return dict(spec=spec, releases=[rel], grid=grid, domain=domain, convection=conv,
-                dt=60, seed=rel.seed)
+    if name in ("plain-cpu-10k-3h", "plain-gpu-1M-24h"):
+        if name == "plain-cpu-10k-3h":
+            spec = MetSpec(nx=64, ny=48, za=40, n_hours=4, seed=20240101)
+            n, hours, n_t, conv = 10_000, 3, 3, False
+        else:
+            spec = MetSpec(nx=240, ny=160, za=60, n_hours=25, seed=20240102, lon0=2.0, lat0=72.0)
+            n, hours, n_t, conv = 1 << 20, 24, 24, True
+        lon_c = float(spec.lon[(spec.nx - 1) // 2])
+        lat_c = float(spec.lat[(spec.ny - 1) // 2])
+        length = hours * 3600
+        rel = Release(lon=lon_c + 0.1, lat=lat_c + 0.1, alt_agl_m=50.0, n_particles=n,
+                      release_time_s=length - 120, duration_s=60, length_s=length, seed=7)
+        grid, domain = _domain_and_grid(spec, 1, n_t)
+        return dict(spec=spec, releases=[rel], grid=grid, domain=domain, convection=conv,
+                    dt=60, seed=rel.seed)
+    if name == "batched-gpu-336r":
+        spec = MetSpec(nx=240, ny=160, za=60, n_hours=169, seed=20240103, lon0=2.0, lat0=72.0)
+        rels = network_releases(spec, n_sites=7, n_hourly=48, first_hour=121, lengths_h=(24, 48, 72, 96, 120),
+                                n_range=(2000, 20000), seed=20240103)
+        grid, domain = _domain_and_grid(spec, len(rels), 120)
+        return dict(spec=spec, releases=rels, grid=grid, domain=domain, convection=True, dt=60, seed=11)
+    if name == "batched-cpu-6r":
+        spec = MetSpec(nx=64, ny=48, za=40, n_hours=7, seed=20240104)
+        rels = network_releases(spec, n_sites=2, n_hourly=3, first_hour=4, lengths_h=(1, 2, 3),
+                                n_range=(8000, 16000), seed=20240104)
+        grid, domain = _domain_and_grid(spec, len(rels), 3)
+        return dict(spec=spec, releases=rels, grid=grid, domain=domain, convection=False, dt=60, seed=13)
+    raise KeyError(f"unknown workload {name!r}")

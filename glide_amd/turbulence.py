@@ -209,7 +209,7 @@ class HannaScheme(TurbulenceScheme):
         else:
             wind = torch.empty(Z * Y * X * 8 * es, dtype=torch.uint8, device=device)
             aux = torch.empty(Z * Y * X * 8 * es, dtype=torch.uint8, device=device)
-            surf = torch.empty(Y * X * 12 * es, dtype=torch.uint8, device=device)
+            surf = torch.empty(Y * X * 12 * 4, dtype=torch.uint8, device=device)  # float in both layouts
         L.check(L.lib().glide_build_window(
             s3.data_ptr(), e3.data_ptr(), s2.data_ptr(), e2.data_ptr(), p.data_ptr(), lev.data_ptr(), Z, Y, X,
             self.meander_coefficient, self.meander_stencil_radius, self.met_dtype, wind.data_ptr(),

@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define GLIDE_ABI_VERSION 1
+#define GLIDE_ABI_VERSION 2
 
 #define GLIDE_OK 0
 #define GLIDE_ERR_INVALID (-1)
@@ -59,6 +59,11 @@ extern "C" {
 #define GLIDE_F_EMIT 0x8           /* write footprint keys/values + escape kill */
 #define GLIDE_F_MASK_INPUT 0x10    /* active mask given explicitly (GlideParticles.active) */
 #define GLIDE_F_NO_TURB 0x20       /* skip the turbulence body (advection only) */
+#define GLIDE_F_RELEASE_TERMINUS 0x40 /* lo_off holds each particle's release terminus te (window end -
+                                         backward length): active iff r_off >= cursor > te, so a release
+                                         is stepped exactly as when it runs alone (no counterpart in the
+                                         reference, whose batch terminus gives non-earliest releases one
+                                         more step, main.py:1383) */
 
 /*
  * Met grid and the per-level AGL ladder (metadata.lon/lat/level,
@@ -76,7 +81,8 @@ typedef struct GlideGrid {
  *   wind: [nz][ny][nx] records of 8: u_s v_s w_s u_e v_e w_e 0 0
  *   aux : [nz][ny][nx] records of 8: sigma_w_ft T_Lw_ft rho drho_dz sigma_mu sigma_mv 0 0
  *   surf: [ny][nx] records of 12: blh sp ustar shf t0 (hour start), same (hour end), 0 0
- * Records hold float (GLIDE_MET_F32) or bfloat16 (GLIDE_MET_BF16) values.
+ * wind and aux records hold float (GLIDE_MET_F32) or bfloat16
+ * (GLIDE_MET_BF16) values; surf records are float in both.
  */
 typedef struct GlideWindow {
     const void* wind;
@@ -95,6 +101,10 @@ typedef struct GlideWindow {
  *   alive:   liveness (escape kill), active: explicit mask when
  *            GLIDE_F_MASK_INPUT is set (else active is computed from r_off,
  *            length and alive, main.py:1396-1401).
+ *   lo_off:  per-particle earliest active cursor offset, r_off - backward
+ *            length (releases of different lengths in one batch); NULL means
+ *            r_off - p->length_s for every particle. With
+ *            GLIDE_F_RELEASE_TERMINUS it holds the release terminus instead.
  * um/vm may be NULL when meander is off.
  */
 typedef struct GlideParticles {
@@ -115,6 +125,7 @@ typedef struct GlideParticles {
     const uint32_t* uid;
     uint8_t* alive;
     const uint8_t* active;
+    const float* lo_off;
 } GlideParticles;
 
 /*
@@ -144,11 +155,15 @@ typedef struct GlideStepParams {
 } GlideStepParams;
 
 /* Step outputs (device): per-particle footprint key (n_cells = dropped) and
- * value (weight * dt), and the step's escape count (added to *escaped). */
+ * value (weight * dt), and the step's escape count (added to *escaped).
+ * active (may be NULL): the step's count of active particles (its
+ * particle-steps) is added to *active. In table mode (glide_step_table) both
+ * counters point at per-row arrays indexed by the cursor. */
 typedef struct GlideEmit {
     uint32_t* keys;
     float* vals;
     unsigned long long* escaped;
+    unsigned long long* active;
 } GlideEmit;
 
 int glide_abi_version(void);
@@ -165,7 +180,7 @@ int glide_step(const GlideStepParams* p, const GlideParticles* s, const GlideGri
  * body (replaces the reference's static-buffer graph replay,
  * hanna.py:1117-1160 with main.py:1396-1477): the step reads
  * table[row[0]] (device GlideStepParams rows, one per schedule step), adds its
- * escape count to e->escaped[table[row[0]].step_index], and the last
+ * escape count to e->escaped[row[0]], and the last
  * workgroup to finish advances row[0] by one. row points to GLIDE_ROW_WORDS
  * device int32: the cursor, then finished-workgroup counters that must be 0
  * at launch (they are left 0; glide_row_set sets them). tmpl is a host row
@@ -214,14 +229,17 @@ int glide_sort_pairs(const uint32_t* keys, const uint32_t* vals, int64_t n, int3
                      size_t workspace_bytes, void* stream);
 
 /*
- * Particle reordering for gather locality (no counterpart in the reference,
- * which keeps particles in release order): keys[i] = met cell of particle i in
- * record memory order ((level * ny + row) * nx + column), or nz*ny*nx for a
- * particle that can never be active again (escaped, or r_off - length_s >
- * cursor_off); idx[i] = i. Sort with glide_sort_pairs, then permute.
+ * Particle reordering for gather locality and compaction of finished
+ * particles (no counterpart in the reference, which keeps particles in release
+ * order): keys[i] = met cell of particle i, column-major ((row * nx + column)
+ * * nz + level), or nz*ny*nx for a particle that can never be active again
+ * (escaped, or its earliest active offset lo_off (r_off - length_s) is after
+ * cursor_off); idx[i] = i. When live is not NULL, the number of particles that
+ * can still become active is added to *live (zero it first). Sort with
+ * glide_sort_pairs, then permute: those particles come first.
  */
 int glide_reorder_keys(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
-                       uint32_t* keys, uint32_t* idx, void* stream);
+                       uint32_t* keys, uint32_t* idx, uint32_t* live, void* stream);
 
 /* dst.array[j] = src.array[idx[j]] for every non-NULL array of the particle
  * state (structure of arrays, pos_stride 1, src and dst distinct). */

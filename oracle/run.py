@@ -74,9 +74,8 @@ def schedule(releases, dt: int):
     (cursor_s, delta_s) with cursor in seconds after the met epoch."""
 
     ends = [r.release_time_s + r.duration_s for r in releases]
-    length = releases[0].length_s
     start = max(ends)
-    term = min(ends) - length
+    term = min(e - r.length_s for e, r in zip(ends, releases))
     out = []
     t = start
     while t > term:
@@ -87,9 +86,13 @@ def schedule(releases, dt: int):
 
 
 def run(cfg, dty=np.float64, params: lpdm.HannaParams | None = None, met=None,
-        max_steps: int | None = None, hours_cache=None, convection=None) -> RunResult:
+        max_steps: int | None = None, hours_cache=None, convection=None,
+        release_terminus: bool = False) -> RunResult:
     """Run the plain method over the whole single-batch workload ``cfg``
-    (from ``glide_amd.synthetic.workload_config``)."""
+    (from ``glide_amd.synthetic.workload_config``). ``release_terminus``
+    steps each release only while the cursor is above its own terminus
+    (window end - length), as when it runs alone; the default is the
+    reference's mask (main.py:1396-1401) under the batch terminus."""
 
     from glide_amd.synthetic import SyntheticMet
 
@@ -104,8 +107,7 @@ def run(cfg, dty=np.float64, params: lpdm.HannaParams | None = None, met=None,
     seed = cfg["seed"]
     batch_start = min(r.release_time_s for r in rels)
     P = {k: [] for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm", "uid")}
-    w_l, roff_l, weoff_l, ridx_l = [], [], [], []
-    uid0 = 0
+    w_l, roff_l, weoff_l, ridx_l, lo_l, te_l = [], [], [], [], [], []
     for r in rels:
         n = r.n_particles
         P["lon"].append(np.full(n, r.lon, np.float32))
@@ -113,11 +115,16 @@ def run(cfg, dty=np.float64, params: lpdm.HannaParams | None = None, met=None,
         P["z"].append(np.full(n, r.alt_agl_m, np.float32))
         for k in ("up", "vp", "wp", "um", "vm"):
             P[k].append(np.zeros(n, np.float32))
-        P["uid"].append(np.arange(uid0, uid0 + n, dtype=np.uint32))
-        uid0 += n
+        # particle ids: (noise id % 4096) << 20 | k (one release of > 2^20: k alone)
+        nid = r.release_idx if getattr(r, "noise_id", None) is None else r.noise_id
+        base = 0 if n > (1 << 20) else (int(nid) % 4096) << 20
+        P["uid"].append((np.uint32(base) + np.arange(n, dtype=np.uint32)).astype(np.uint32))
         w_l.append(np.full(n, 1.0 / n, np.float32))
         off = float(r.release_time_s - batch_start)
-        roff_l.append(release_offsets(r.seed, n, r.duration_s) + np.float32(off))
+        ro = release_offsets(r.seed, n, r.duration_s) + np.float32(off)
+        roff_l.append(ro)
+        lo_l.append(ro - np.float32(r.length_s))  # each release's own backward length
+        te_l.append(np.full(n, np.float32(off + r.duration_s) - np.float32(r.length_s), np.float32))
         weoff_l.append(np.full(n, off + r.duration_s, np.float32))
         ridx_l.append(np.full(n, r.release_idx, np.int64))
     P = {k: np.concatenate(v) for k, v in P.items()}
@@ -127,7 +134,8 @@ def run(cfg, dty=np.float64, params: lpdm.HannaParams | None = None, met=None,
     roff = np.concatenate(roff_l)
     weoff = np.concatenate(weoff_l)
     ridx = np.concatenate(ridx_l)
-    length = float(rels[0].length_s)
+    lo = np.concatenate(lo_l)
+    te = np.concatenate(te_l)
     N = weights.size
     alive = np.ones(N, bool)
     fp = np.zeros(grid["n_r"] * grid["n_t"] * grid["n_z"] * grid["n_y"] * grid["n_x"], dty)
@@ -150,7 +158,10 @@ def run(cfg, dty=np.float64, params: lpdm.HannaParams | None = None, met=None,
     t_step = 0.0
     for si, (cur, delta) in enumerate(sched):
         cursor_off = float(cur - batch_start)
-        active = (roff >= np.float32(cursor_off)) & (roff - np.float32(length) <= np.float32(cursor_off)) & alive
+        if release_terminus:
+            active = (roff >= np.float32(cursor_off)) & (np.float32(cursor_off) > te) & alive
+        else:
+            active = (roff >= np.float32(cursor_off)) & (lo <= np.float32(cursor_off)) & alive
         bh = cur // 3600
         if bh not in windows:
             windows.clear()
@@ -158,10 +169,11 @@ def run(cfg, dty=np.float64, params: lpdm.HannaParams | None = None, met=None,
         win = windows[bh]
         alpha = max(0.0, min(1.0, ((cur - 0.5 * delta) - 3600.0 * bh) / 3600.0))
         t0 = time.perf_counter()
-        P = lpdm.step(P, active, win, alpha, float(delta), p, seed=seed, step_index=si)
+        noise_step = cur // dt  # Philox counter word 1: absolute time in steps
+        P = lpdm.step(P, active, win, alpha, float(delta), p, seed=seed, step_index=noise_step)
         if bh != last_bracket:
             if convection is not None:
-                P = convection(P, active, bh, alpha, si)
+                P = convection(P, active, bh, alpha, noise_step)
             last_bracket = bh
         t_idx = og.time_bins(weoff, cursor_off, grid["n_t"])
         og.accumulate(fp, P["lon"].astype(np.float32), P["lat"].astype(np.float32),

@@ -106,7 +106,7 @@ def test_resample_api_bit_exact_and_hanna_step_matches_engine_and_oracle():
     active_np[::7] = False  # some inactive particles must stay untouched
     active = torch.as_tensor(active_np, device="cuda")
     before = parts.clone()
-    scheme.step_index = st.index
+    scheme.step_index = st.noise_step
     out, state = scheme.step(parts, state, mw, st.alpha, float(st.delta_s), active, GPUEngine())
     got = out.cpu().numpy()
     assert np.array_equal(got[~active_np], before.cpu().numpy()[~active_np])
@@ -125,7 +125,7 @@ def test_resample_api_bit_exact_and_hanna_step_matches_engine_and_oracle():
         S[k] = np.zeros(n)
     S["uid"] = ra.uid
     ref = O.step(S, active_np, win, st.alpha, float(st.delta_s), O.HannaParams(), seed=cfg["seed"],
-                 step_index=st.index)
+                 step_index=st.noise_step)
     e = np.abs(got[active_np, 0] - ref["lon"][active_np]) / np.abs(ref["lon"][active_np])
     assert (e <= 1e-5).mean() >= 0.999
 

@@ -266,7 +266,7 @@ def test_step_parity_first_met_hour(cfg0):
         eng.last_bracket = bh
         eng.run_step()
         got = eng.state_numpy()
-        ref = O.step(S32, active, win, st.alpha, float(st.delta_s), p, seed=eng.seed, step_index=st.index)
+        ref = O.step(S32, active, win, st.alpha, float(st.delta_s), p, seed=eng.seed, step_index=st.noise_step)
         if active.any():
             n_checked += 1
             e_lon = np.abs(got["lon"][active] - ref["lon"][active]) / np.abs(ref["lon"][active])
@@ -337,7 +337,8 @@ def test_convection_parity():
     z_dev = eng.z.cpu().numpy()
     col = OC.nearest_column(lon, lat, met.lon[0], met.lon[-1], met.lat[0], met.lat[-1], nx, ny)
     z_ref = OC.convect_columns(z.astype(np.float64), active, col, olcl, olnb, omb.astype(np.float64),
-                               ph.cpu().numpy(), met.spec.agl_levels, eng.seed, eng.ra.uid, 5)
+                               ph.cpu().numpy(), met.spec.agl_levels, eng.seed, eng.ra.uid,
+                               eng.sched[5].noise_step)
     moved_dev = z_dev != z
     moved_ref = z_ref != z.astype(np.float64)
     assert moved_ref.mean() > 0.01
