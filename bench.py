@@ -58,6 +58,9 @@ def parse():
                     help="skip the network-batch leg (batched-gpu-336r, bfloat16 met, compensated footprint)")
     ap.add_argument("--batched-steps", type=int, default=0,
                     help="time only the first N steps of the network batch (0 = all)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=30.0,
+                    help="seconds of Poisson request arrivals (20 requests/s) in the serving leg")
     ap.add_argument("--cpu-particles", type=int, default=262144)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -107,6 +110,63 @@ def cpu_baseline(cfg, n_particles: int, n_steps: int) -> dict:
 
 
 BATCHED = "batched-gpu-336r"
+SERVING = "serving-gpu-10d"
+
+
+def serving_leg(args, plain_value: float) -> dict:
+    """Continuous-batching serving: Poisson requests (20/s) on a 240x160x60
+    met ring of 240 hourly brackets in bfloat16 (a 24 h synthetic cycle
+    repeated over the 10-day span), each request a release of 1k-50k particles,
+    6-240 h back, at one of 100 sites. Requests are admitted as slots and pool
+    blocks free up and are stepped together on their own cursors; latency is
+    arrival to footprint-on-host."""
+
+    import numpy as np
+    import torch
+
+    from glide_amd import _lib as L
+    from glide_amd.serving import MetRing, Server, latency_percentiles
+    from glide_amd.synthetic import MetSpec, SyntheticMet, serving_requests, workload_config
+
+    spec = MetSpec(nx=240, ny=160, za=60, n_hours=25, seed=20240105, lon0=2.0, lat0=72.0)
+    met = SyntheticMet(spec)
+    ring = MetRing(met, first_hour=0, n_hours=240, met_dtype=L.GLIDE_MET_BF16, source_hour=lambda h: h % 24)
+    base = workload_config("plain-gpu-1M-24h")
+    reqs = serving_requests(spec, rate_per_s=20.0, seconds=args.serve_seconds)
+    srv = Server(ring, base["grid"], base["domain"], max_slots=160, pool_particles=6 << 20, n_t_max=240,
+                 seed=17)
+    # warm-up: one request through, not timed
+    warm = serving_requests(spec, rate_per_s=20.0, seconds=0.2, seed=99)[:1]
+    for r in warm:
+        srv.admit(r)
+    srv.drain()
+    stats = srv.serve(reqs)
+    res = stats.results
+    lat = latency_percentiles(res)
+    day = [r for r in res if 18 * 3600 <= r.req.length_s <= 30 * 3600]
+    lat_day = latency_percentiles(day)
+    value = stats.particle_steps / stats.busy_s if stats.busy_s > 0 else float("nan")
+    out = {
+        "workload": SERVING,
+        "value": value,
+        "unit": "particle-steps/s",
+        "requests": len(reqs),
+        "completed": len(res),
+        "arrival_rate_per_s": 20.0,
+        "arrival_seconds": args.serve_seconds,
+        "busy_seconds": stats.busy_s,
+        "engine_steps": stats.steps,
+        "active_particle_steps": stats.particle_steps,
+        "latency_s": {"p50": lat["p50"], "p99": lat["p99"]},
+        "latency_18_30h_s": {"p50": lat_day["p50"], "p99": lat_day["p99"], "n": len(day)},
+        "met_storage": "bf16 ring, 240 brackets",
+        "slots": srv.max_slots,
+        "vs_plain": value / plain_value,
+        "mean_particles": float(np.mean([r.n_particles for r in reqs])) if reqs else 0.0,
+    }
+    del srv, ring
+    torch.cuda.empty_cache()
+    return out
 
 
 def batched_leg(args, sptr: int, plain_value: float) -> dict:
@@ -293,6 +353,8 @@ def main() -> int:
         del eng
         torch.cuda.empty_cache()
         out["batched"] = batched_leg(args, sptr, out["value"])
+    if not args.no_serving:
+        out["serving"] = serving_leg(args, out["value"])
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_particles, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -31,6 +31,9 @@ F_MASK_INPUT = 0x10
 F_NO_TURB = 0x20
 F_RELEASE_TERMINUS = 0x40
 
+GLIDE_SLOT_FREE = 0
+GLIDE_SLOT_RUNNING = 1
+GLIDE_SLOT_FINISHED = 2
 GLIDE_ROW_WORDS = 66  # int32 words of the step cursor buffer (glide_step_table)
 
 vp = C.c_void_p
@@ -79,6 +82,24 @@ class GlideStepParams(C.Structure):
     ]
 
 
+class GlideSlot(C.Structure):
+    _fields_ = [
+        ("t", C.c_double), ("term", C.c_double), ("rel_time", C.c_double),
+        ("state", C.c_int32), ("k", C.c_int32),
+        ("last_d", C.c_double), ("cursor_off", C.c_float), ("alpha", C.c_float), ("dt", C.c_float),
+        ("noise_step", C.c_int32), ("bracket", C.c_int32), ("stepping", C.c_int32),
+        ("escaped", C.c_uint64), ("active_steps", C.c_uint64),
+    ]
+
+
+class GlideMetRing(C.Structure):
+    _fields_ = [
+        ("wind", vp), ("aux", vp), ("surf", vp),
+        ("wind_stride", C.c_int64), ("aux_stride", C.c_int64), ("surf_stride", C.c_int64),
+        ("first_hour", C.c_int32), ("n_hours", C.c_int32), ("dtype", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
 class GlideEmit(C.Structure):
     _fields_ = [("keys", vp), ("vals", vp), ("escaped", vp), ("active", vp)]
 
@@ -94,6 +115,9 @@ SIGNATURES = {
     "glide_step_table": (C.c_int, [P(GlideStepParams), vp, vp, P(GlideParticles), P(GlideGrid), P(GlideWindow),
                                    P(GlideFootprintGrid), P(GlideEmit), vp]),
     "glide_row_set": (C.c_int, [vp, C.c_int32, vp]),
+    "glide_slots_advance": (C.c_int, [vp, C.c_int32, C.c_double, vp]),
+    "glide_step_slots": (C.c_int, [P(GlideStepParams), vp, vp, P(GlideParticles), P(GlideGrid), P(GlideMetRing),
+                                   P(GlideFootprintGrid), P(GlideEmit), vp]),
     "glide_footprint_keys": (C.c_int, [C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                        P(GlideFootprintGrid), vp, vp, vp]),
     "glide_footprint_workspace_bytes": (C.c_size_t, [C.c_int64]),

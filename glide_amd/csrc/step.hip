@@ -37,6 +37,10 @@ struct StepArgs {
     const GlideStepParams* table;
     int32_t* row;
     unsigned long long* n_active;  // optional per-step active count (e->active)
+    // Served slots (glide_step_slots): per-wavefront slot, scalars and window.
+    GlideSlot* slots;
+    const int32_t* wave_slot;
+    GlideMetRing ring;
 };
 
 constexpr int kMaxLevels = 128;
@@ -58,10 +62,19 @@ __device__ __forceinline__ GlideStepParams load_row(const GlideStepParams* table
     return p;
 }
 
+// Per-launch (or, for served slots, per-wavefront) inputs of the step body.
+struct StepCtx {
+    const void* wind;
+    const void* aux;
+    const void* surf;
+    unsigned long long* escaped;
+    int32_t slot;  // >= 0: footprint release index of a served slot; -1: rel_idx
+};
+
 // Advances particle i; returns whether it was active.
 template <bool BF16>
 __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* lev, int64_t i,
-                                              const GlideStepParams& p, unsigned long long* escaped) {
+                                              const GlideStepParams& p, const StepCtx& x) {
     // 32-bit byte offsets of this particle (the host checks the arrays fit)
     const uint32_t i1 = static_cast<uint32_t>(i);
     const uint32_t i4 = i1 * 4u;
@@ -99,7 +112,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         const float iz = level_index(lev, nz, zz);
         const Corner3 c = corners3(ix, iy, iz, nx, ny, nz, Rec8<BF16>::kBytes);
         float v[6];
-        sample_rec<BF16, 6>(a.wind, c, v);
+        sample_rec<BF16, 6>(x.wind, c, v);
         const float u = v[0] * (1.0f - al) + v[3] * al;
         const float vv = v[1] * (1.0f - al) + v[4] * al;
         const float w = v[2] * (1.0f - al) + v[5] * al;
@@ -143,7 +156,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 float r[10];
-                load_surf_at(a.surf, offs[q], r);
+                load_surf_at(x.surf, offs[q], r);
 #pragma unroll
                 for (int ch = 0; ch < 5; ++ch) acc[ch] += ws[q] * torch_lerp(r[ch], r[5 + ch], al);
             }
@@ -171,7 +184,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         float ax[6];
         {
             const Corner3 c = corners3(ix, iy, level_index(lev, nz, z_eval), nx, ny, nz, Rec8<BF16>::kBytes);
-            sample_rec<BF16, 6>(a.aux, c, ax);
+            sample_rec<BF16, 6>(x.aux, c, ax);
         }
 
         // Column turbulence at z_eval (hanna.py:1500-1596).
@@ -210,7 +223,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
             if (zq > sf.blh) {
                 const Corner3 c = corners3(ix, iy, level_index(lev, nz, zq), nx, ny, nz, Rec8<BF16>::kBytes);
                 float f0, f1;
-                sample_rec2<BF16>(a.aux, c, f0, f1);
+                sample_rec2<BF16>(x.aux, c, f0, f1);
                 sw = f0;
             }
             return fmaxf(sw, kSigmaMin);
@@ -291,12 +304,12 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
 
     if (emit) {
         const int64_t tb = time_bin(ldo(s.we_off, i4), co, a.fp.n_t);
-        const int64_t rb = s.rel_idx ? ldo(s.rel_idx, i4) : 0;
+        const int64_t rb = x.slot >= 0 ? x.slot : (s.rel_idx ? ldo(s.rel_idx, i4) : 0);
         sto(a.e.keys, i4, footprint_key(a.fp, true, lon, lat, z, tb, rb));
         sto(a.e.vals, i4, ldo(s.weight, i4) * dt);
         const bool esc = !within_domain(a.fp, lon, lat, z);
         if (esc) sto(s.alive, i1, static_cast<uint8_t>(0));
-        count_flags(esc, escaped);
+        count_flags(esc, x.escaped);
     }
     return true;
 }
@@ -307,18 +320,53 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
     for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
     __syncthreads();
     GlideStepParams p = a.p;
-    unsigned long long* escaped = a.e.escaped;
+    StepCtx x{a.wind, a.aux, a.surf, a.e.escaped, -1};
     unsigned long long* n_active = a.n_active;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool run = i < a.s.n;
     if (a.table != nullptr) {
         int r = 0;
         p = load_row(a.table, a.row, r);
-        escaped += r;
+        x.escaped += r;
         if (n_active != nullptr) n_active += r;
+    } else if (a.slots != nullptr) {
+        // Served slots: one slot per wavefront (64-particle blocks).
+        const int32_t slot = __builtin_amdgcn_readfirstlane(run ? a.wave_slot[i >> 6] : -1);
+        bool stepping = false;
+        if (slot >= 0) {
+            GlideSlot* sl = a.slots + slot;
+            stepping = __builtin_amdgcn_readfirstlane(sl->stepping) != 0;
+            const int32_t hb = __builtin_amdgcn_readfirstlane(sl->bracket) - a.ring.first_hour;
+            stepping = stepping && hb >= 0 && hb < a.ring.n_hours;  // never outside the ring
+            if (stepping) {
+                p.dt = sl->dt;
+                p.alpha = sl->alpha;
+                p.cursor_off = sl->cursor_off;
+                p.step_index = __builtin_amdgcn_readfirstlane(sl->noise_step);
+                const int64_t h = hb;
+                x.wind = static_cast<const char*>(a.ring.wind) + h * a.ring.wind_stride;
+                x.aux = static_cast<const char*>(a.ring.aux) + h * a.ring.aux_stride;
+                x.surf = static_cast<const char*>(a.ring.surf) + h * a.ring.surf_stride;
+                x.escaped = &sl->escaped;
+                x.slot = slot;
+                n_active = &sl->active_steps;
+            }
+        }
+        if (!stepping) {
+            if (run && (p.flags & GLIDE_F_EMIT)) {
+                const uint32_t i4 = static_cast<uint32_t>(i) * 4u;
+                sto(a.e.keys, i4, a.fp.n_cells);
+                sto(a.e.vals, i4, 0.0f);
+            }
+            run = false;
+            n_active = nullptr;
+        }
     }
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     bool act = false;
-    if (i < a.s.n) act = step_particle<BF16>(a, lev, i, p, escaped);
-    if (n_active != nullptr) {
+    if (run) act = step_particle<BF16>(a, lev, i, p, x);
+    if (a.slots != nullptr) {
+        if (n_active != nullptr) count_flags(act, n_active);  // per wavefront: its slot's counter
+    } else if (n_active != nullptr) {
         // active count: wave ballots summed in LDS, one atomic per workgroup at the end
         __shared__ uint32_t wact[kStepBlock / 64];
         const unsigned long long m = __ballot(act);
@@ -397,7 +445,9 @@ using namespace glide;
 
 static int launch_step(const GlideStepParams* p, const GlideStepParams* table, int32_t* row,
                        const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
-                       const GlideFootprintGrid* fg, const GlideEmit* e, void* stream) {
+                       const GlideFootprintGrid* fg, const GlideEmit* e, void* stream,
+                       GlideSlot* slots = nullptr, const int32_t* wave_slot = nullptr,
+                       const GlideMetRing* ring = nullptr) {
     GLIDE_REQUIRE(p && s && g && w, "glide_step: NULL argument");
     GLIDE_REQUIRE(s->n >= 0, "n must be >= 0");
     GLIDE_REQUIRE(s->lon && s->lat && s->z && s->up && s->vp && s->wp, "particle arrays are NULL");
@@ -421,6 +471,9 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
     a.table = table;
     a.row = row;
     a.n_active = e ? e->active : nullptr;
+    a.slots = slots;
+    a.wave_slot = wave_slot;
+    if (ring != nullptr) a.ring = *ring;
     int rc = fill_grid(a, *g, *w);
     if (rc) return rc;
     if (p->flags & GLIDE_F_EMIT) {
@@ -451,6 +504,57 @@ extern "C" int glide_step(const GlideStepParams* p, const GlideParticles* s, con
                           const GlideWindow* w, const GlideFootprintGrid* fg, const GlideEmit* e,
                           void* stream) {
     return launch_step(p, nullptr, nullptr, s, g, w, fg, e, stream);
+}
+
+__global__ void slots_advance_kernel(GlideSlot* slots, int32_t n, double dt) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    GlideSlot sl = slots[j];
+    sl.stepping = 0;
+    if (sl.state == GLIDE_SLOT_RUNNING) {
+        if (sl.k > 0) sl.t -= sl.last_d;
+        if (sl.t > sl.term) {
+            // the reference's cursor loop for a batch of this release alone
+            const double d = fmin(dt, sl.t - sl.term);
+            const double bh = floor(sl.t / 3600.0);
+            double al = ((sl.t - 0.5 * d) - 3600.0 * bh) / 3600.0;
+            al = al < 0.0 ? 0.0 : (al > 1.0 ? 1.0 : al);
+            sl.last_d = d;
+            sl.dt = static_cast<float>(d);
+            sl.alpha = static_cast<float>(al);
+            sl.cursor_off = static_cast<float>(sl.t - sl.rel_time);
+            sl.noise_step = static_cast<int32_t>(floor(sl.t / dt));
+            sl.bracket = static_cast<int32_t>(bh);
+            sl.stepping = 1;
+            sl.k += 1;
+        } else {
+            sl.state = GLIDE_SLOT_FINISHED;
+        }
+    }
+    slots[j] = sl;
+}
+
+extern "C" int glide_slots_advance(GlideSlot* slots, int32_t n_slots, double dt, void* stream) {
+    GLIDE_REQUIRE(slots != nullptr && n_slots >= 0, "glide_slots_advance: bad slots");
+    GLIDE_REQUIRE(dt > 0, "dt must be > 0");
+    if (n_slots == 0) return GLIDE_OK;
+    hipLaunchKernelGGL(slots_advance_kernel, dim3((n_slots + 255) / 256), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), slots, n_slots, dt);
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}
+
+extern "C" int glide_step_slots(const GlideStepParams* tmpl, GlideSlot* slots, const int32_t* wave_slot,
+                                const GlideParticles* s, const GlideGrid* g, const GlideMetRing* ring,
+                                const GlideFootprintGrid* fg, const GlideEmit* e, void* stream) {
+    GLIDE_REQUIRE(tmpl && slots && wave_slot && s && ring, "glide_step_slots: NULL argument");
+    GLIDE_REQUIRE(s->n % 64 == 0, "the particle pool must hold whole 64-particle blocks");
+    GLIDE_REQUIRE(tmpl->flags & GLIDE_F_RELEASE_TERMINUS, "served slots step with GLIDE_F_RELEASE_TERMINUS");
+    GLIDE_REQUIRE(tmpl->flags & GLIDE_F_EMIT, "served slots emit footprint keys (GLIDE_F_EMIT)");
+    GLIDE_REQUIRE(!(tmpl->flags & GLIDE_F_MASK_INPUT), "served slots compute their own mask");
+    GLIDE_REQUIRE(ring->wind && ring->aux && ring->surf && ring->n_hours >= 1, "empty met ring");
+    const GlideWindow w{ring->wind, ring->aux, ring->surf, ring->dtype, 0};
+    return launch_step(tmpl, nullptr, nullptr, s, g, &w, fg, e, stream, slots, wave_slot, ring);
 }
 
 extern "C" int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, int32_t* row,

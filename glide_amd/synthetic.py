@@ -347,3 +347,39 @@ def workload_config(name: str):
         grid, domain = _domain_and_grid(spec, len(rels), 3)
         return dict(spec=spec, releases=rels, grid=grid, domain=domain, convection=False, dt=60, seed=13)
     raise KeyError(f"unknown workload {name!r}")
+
+
+def serving_requests(spec: MetSpec, *, rate_per_s: float, seconds: float, n_sites: int = 100,
+                     span_hours: int = 240, length_h=(6.0, 240.0), n_range=(1000, 50000), seed: int = 20240105):
+    """A Poisson stream of footprint requests (the serving workload): arrivals
+    at ``rate_per_s`` over ``seconds``; each request picks one of ``n_sites``
+    seeded sites, an inlet height uniform on [10, 300] m, a backward length
+    uniform on ``length_h`` (whole minutes), a release time uniform within the
+    ``span_hours`` met span that leaves its backward window inside the span,
+    and a particle count log-uniform on ``n_range``."""
+
+    from glide_amd.serving import Request
+
+    rng = np.random.default_rng(seed)
+    lon, lat = spec.lon, spec.lat
+    s_lon = rng.uniform(lon[len(lon) // 8], lon[7 * len(lon) // 8], n_sites)
+    lat_lo, lat_hi = sorted((lat[len(lat) // 8], lat[7 * len(lat) // 8]))
+    s_lat = rng.uniform(lat_lo, lat_hi, n_sites)
+    out = []
+    t = 0.0
+    k = 0
+    span = span_hours * 3600
+    while True:
+        t += float(rng.exponential(1.0 / rate_per_s))
+        if t >= seconds:
+            break
+        site = int(rng.integers(n_sites))
+        length = int(round(float(rng.uniform(*length_h)) * 60.0)) * 60
+        length = min(length, span - 3 * 3600)
+        rel = int(rng.integers(length + 60, span - 120))
+        n = int(round(float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))))
+        out.append(Request(req_id=k, lon=float(s_lon[site]), lat=float(s_lat[site]),
+                           alt_agl_m=float(rng.uniform(10.0, 300.0)), n_particles=n, release_time_s=rel,
+                           length_s=length, duration_s=60, seed=seed + 1 + k, arrival_s=t))
+        k += 1
+    return out

@@ -313,6 +313,67 @@ int glide_convect(const GlideStepParams* p, const GlideParticles* s, const Glide
                   const GlideEmit* e, void* stream);
 
 /*
+ * Serving: many releases (requests), each with its own cursor, stepped
+ * together. No counterpart in the reference, which steps one batch on one
+ * cursor (main.py:1383-1477); each slot follows the reference's schedule for
+ * a batch holding only its release (cursor from the window end down while
+ * above the release terminus, main.py:1338-1389, bracket and alpha as
+ * main.py:892-938), so a request's result does not depend on the others.
+ *
+ * GlideSlot (device array, one per slot): the host fills the first block on
+ * admission (state = GLIDE_SLOT_RUNNING, t = window end, k = 0);
+ * glide_slots_advance fills the per-step block for every running slot, or sets
+ * state = GLIDE_SLOT_FINISHED once t <= term. Particle offsets (r_off,
+ * we_off, lo_off) are relative to the slot's release time.
+ */
+#define GLIDE_SLOT_FREE 0
+#define GLIDE_SLOT_RUNNING 1
+#define GLIDE_SLOT_FINISHED 2
+
+typedef struct GlideSlot {
+    /* set on admission */
+    double t;            /* cursor (s after the met epoch); starts at the window end */
+    double term;         /* release terminus: window end - backward length */
+    double rel_time;     /* release time: offsets are relative to it */
+    int32_t state;
+    int32_t k;           /* steps taken */
+    /* per step (glide_slots_advance) */
+    double last_d;       /* length of the step taken last */
+    float cursor_off, alpha, dt;
+    int32_t noise_step;  /* Philox counter word 1: floor(t / dt) */
+    int32_t bracket;     /* met hour floor(t / 3600) */
+    int32_t stepping;    /* 1 when the slot takes a step this launch */
+    /* counters (added by glide_step_slots) */
+    unsigned long long escaped;
+    unsigned long long active_steps;
+} GlideSlot;
+
+/* Met windows of consecutive brackets first_hour .. first_hour + n_hours - 1,
+ * each as glide_build_window writes it, stride bytes apart (device). */
+typedef struct GlideMetRing {
+    const void* wind;
+    const void* aux;
+    const void* surf;
+    int64_t wind_stride, aux_stride, surf_stride;
+    int32_t first_hour, n_hours;
+    int32_t dtype, reserved;
+} GlideMetRing;
+
+/* Advance every running slot one step of its schedule (dt = step length). */
+int glide_slots_advance(GlideSlot* slots, int32_t n_slots, double dt, void* stream);
+
+/* glide_step over a particle pool shared by the slots: wave_slot[j] is the slot
+ * of particles 64j .. 64j+63 (-1: unused), each slot's particles lie in whole
+ * such blocks. Each wavefront takes dt, alpha, cursor and noise step from its
+ * slot and its met window from the ring (the slot's bracket); flags, seed and
+ * the Hanna options from tmpl. Footprint keys use the slot as the release
+ * index (fg->n_r >= number of slots); escape and active counts go to the
+ * slot. Waves of non-stepping slots only write dropped keys. */
+int glide_step_slots(const GlideStepParams* tmpl, GlideSlot* slots, const int32_t* wave_slot,
+                     const GlideParticles* s, const GlideGrid* g, const GlideMetRing* ring,
+                     const GlideFootprintGrid* fg, const GlideEmit* e, void* stream);
+
+/*
  * Step-body graph capture on a caller stream: glide_graph_begin starts stream
  * capture, glide_graph_end instantiates the captured work into a handle,
  * glide_graph_launch replays it, glide_graph_destroy frees it. Captured

@@ -24,6 +24,8 @@ STRUCTS = {
     "GlideFootprintGrid": L.GlideFootprintGrid,
     "GlideStepParams": L.GlideStepParams,
     "GlideEmit": L.GlideEmit,
+    "GlideSlot": L.GlideSlot,
+    "GlideMetRing": L.GlideMetRing,
 }
 
 
