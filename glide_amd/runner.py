@@ -153,8 +153,11 @@ class BatchEngine:
                  met_dtype: int = L.GLIDE_MET_F32, hanna: HannaOptions | None = None,
                  convection: ConvectionOptions | None = None, compensated: bool = False,
                  device_met: DeviceMet | None = None, reorder_every: int = 0, use_graph: bool = False,
-                 compact: bool = False, release_terminus: bool = False):
+                 compact: bool = False, release_terminus: bool = False, diagnostics: bool = False):
         self.cfg = cfg
+        # diagnostics: per-step mean particle position (inside the step body)
+        # and hourly grid-mean winds, for the trajectory table
+        self.diagnostics = bool(diagnostics)
         self.reorder_every = int(reorder_every)
         # compact: on every new met hour, sort the particles by met cell with the
         # ones that can never be active again (finished releases, escaped) last,
@@ -247,6 +250,10 @@ class BatchEngine:
         self.step_table = torch.frombuffer(bytearray(bytes(rows)), dtype=torch.uint8).to(d)
         self.step_tmpl = self.step_params(self.sched[0], flags) if self.sched else None
         self.row = torch.zeros(L.GLIDE_ROW_WORDS, dtype=torch.int32, device=d)  # cursor, finished-workgroup counts
+        if self.diagnostics:
+            self.pos_means = torch.zeros((max(len(self.sched), 1), 3), dtype=torch.float64, device=d)
+            self.means_ws = torch.empty(L.lib().glide_means_workspace_bytes(), dtype=torch.uint8, device=d)
+            self.hour_wind: dict[int, torch.Tensor] = {}
 
     def _make_parts(self, b) -> L.GlideParticles:
         return L.GlideParticles(
@@ -294,6 +301,8 @@ class BatchEngine:
         if self.comp is not None:
             self.comp.zero_()
         self.release_total.zero_()
+        if self.diagnostics:
+            self.pos_means.zero_()
         self.step_i = 0
         self.n_live = self.n
         self.last_bracket = None
@@ -352,6 +361,8 @@ class BatchEngine:
                                          C.byref(self.parts), C.byref(self.met.grid), C.byref(self.met.window),
                                          C.byref(self.fg), C.byref(em), stream))
             self._accumulate(stream)
+            if self.diagnostics:
+                self._position_means(stream, row=True)
         finally:
             h = C.c_void_p()
             rc = lib.glide_graph_end(stream, C.byref(h))
@@ -365,6 +376,107 @@ class BatchEngine:
             self.keys.data_ptr(), self.vals.data_ptr(), self.n_live, self.n_cells, self.cells_per_release,
             self.footprint.data_ptr(), self.comp.data_ptr() if self.comp is not None else None,
             self.release_total.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(), stream))
+
+    def _position_means(self, stream: int, row: bool = False) -> None:
+        """Mean lon/lat/z over all particles of the batch into pos_means[step]
+        (the step is read from the device cursor inside a captured body)."""
+
+        n_rows = self.pos_means.shape[0]
+        out = self.pos_means.data_ptr() if row else self.pos_means.data_ptr() + 24 * self.step_i
+        L.check(L.lib().glide_particle_means(
+            C.byref(self._parts[self.cur]), self.row.data_ptr() if row else None, n_rows if row else 0, out,
+            self.means_ws.data_ptr(), self.means_ws.numel(), stream))
+
+    def _wind_means(self, h: int, stream: int) -> None:
+        if h in self.hour_wind:
+            return
+        out3, _, _ = self.met.resampled(h, stream)
+        m = torch.empty(3, dtype=torch.float64, device=self.device)
+        nlev = self.met.nz * self.met.ny * self.met.nx
+        L.check(L.lib().glide_field_means(out3.data_ptr(), nlev, 3, m.data_ptr(), self.means_ws.data_ptr(),
+                                          self.means_ws.numel(), stream))
+        self.hour_wind[h] = m
+
+    def trajectory_rows(self, *, batch_idx: int = 0, step_offset: int = 0,
+                        epoch_iso: str = "2024-01-01T00:00:00") -> list[dict]:
+        """Per-step diagnostics rows of the steps taken, with the reference's
+        columns (main.py:1614-1643): one device-to-host read at the end."""
+
+        if not self.diagnostics:
+            raise ValueError("engine built without diagnostics=True")
+        torch.cuda.synchronize()
+        k = self.step_i
+        pos = self.pos_means.cpu().numpy()[:k]
+        act = self.active_steps.cpu().numpy()[:k]
+        esc = self.escaped.cpu().numpy()[:k]
+        alive = self.n - np.cumsum(esc)
+        wind = {h: m.cpu().numpy() for h, m in self.hour_wind.items()}
+        epoch = np.datetime64(epoch_iso, "s")
+        rows = []
+        for i in range(k):
+            st = self.sched[i]
+            a = float(st.alpha)
+            w = wind[st.bracket_hour] * (1.0 - a) + wind[st.bracket_hour + 1] * a
+            rows.append({
+                "step": step_offset + i + 1, "batch_idx": batch_idx,
+                "time_hour_end": str(epoch + np.timedelta64(int(st.cursor_s), "s")) + "+00:00",
+                "mean_lon": float(pos[i, 0]), "mean_lat": float(pos[i, 1]), "mean_alt_agl_m": float(pos[i, 2]),
+                "u_mean_m_s": float(w[0]), "v_mean_m_s": float(w[1]), "w_mean_m_s": float(w[2]),
+                "active_particles": int(act[i]), "escaped_this_step": int(esc[i]),
+                "alive_particles": int(alive[i]),
+            })
+        return rows
+
+    def endpoints(self) -> dict:
+        """Particle endpoints (lon, lat, alt, weight, release_idx) in particle-id
+        order, for the endpoint table."""
+
+        torch.cuda.synchronize()
+        b = self.bufs[self.cur]
+        uid = b["uid"].cpu().numpy().view(np.uint32)
+        order = np.argsort(uid, kind="stable")
+        return {"lon": b["lon"].cpu().numpy()[order], "lat": b["lat"].cpu().numpy()[order],
+                "alt": b["z"].cpu().numpy()[order], "weight": b["weight"].cpu().numpy()[order],
+                "release_idx": b["rel_idx"].cpu().numpy()[order].astype(np.int64)}
+
+    def footprint_numpy(self) -> np.ndarray:
+        """The footprint as (R, T, Z, Y, X) float32 on the host (the
+        compensation terms added when the footprint is compensated)."""
+
+        torch.cuda.synchronize()
+        g = self.grid_cfg
+        f = self.footprint if self.comp is None else self.footprint + self.comp
+        return f.cpu().numpy().reshape(g["n_r"], g["n_t"], g["n_z"], g["n_y"], g["n_x"])
+
+    def write_outputs(self, out_dir, *, epoch_iso: str = "2024-01-01T00:00:00", release_start: int = 0,
+                      store=None) -> dict:
+        """Write the batch's outputs as the reference does at batch end
+        (main.py:1648-1692): its footprint slabs into the Zarr store (created
+        here for the batch's releases unless ``store`` is given, then written at
+        ``release_start``), endpoints and, with diagnostics, the trajectory
+        table as Parquet. Returns the paths."""
+
+        from pathlib import Path
+
+        from .output import (ZarrFootprintStore, footprint_coords, write_particles_parquet,
+                             write_trajectory_parquet)
+
+        out = Path(out_dir)
+        fp = self.footprint_numpy()
+        if store is None:
+            store = ZarrFootprintStore(out / "footprint.zarr")
+            store.create(fp.shape, coords=footprint_coords(self.grid_cfg, self.cfg["releases"], length_s=self.length_s,
+                                                           epoch_iso=epoch_iso))
+        store.write_region(release_start, fp)
+        ep = self.endpoints()
+        write_particles_parquet(out / "endpoints.parquet", ep["lon"], ep["lat"], ep["alt"], ep["weight"],
+                                ep["release_idx"] + release_start)
+        paths = {"footprint": str(store.path), "endpoints": str(out / "endpoints.parquet")}
+        if self.diagnostics:
+            write_trajectory_parquet(out / "trajectory.parquet", int(self.cfg["dt"]),
+                                     self.trajectory_rows(epoch_iso=epoch_iso))
+            paths["trajectory"] = str(out / "trajectory.parquet")
+        return paths
 
     def close(self) -> None:
         """Free the captured graphs."""
@@ -392,6 +504,9 @@ class BatchEngine:
         elif self.reorder_every and self.step_i > 0 and self.step_i % self.reorder_every == 0:
             self.reorder(st, stream)
         self.met.build_window(st.bracket_hour, stream)
+        if self.diagnostics and new_bracket:
+            self._wind_means(st.bracket_hour, stream)
+            self._wind_means(st.bracket_hour + 1, stream)
         convect = new_bracket and self.conv_opts is not None
         ev = self.step_events.get(self.step_i) if self.step_events is not None else None
         if self.use_graph and not convect and ev is None:
@@ -433,6 +548,8 @@ class BatchEngine:
                                       self.layer_mass.data_ptr(), self.colsum_err.data_ptr(),
                                       C.byref(self.fg), C.byref(em), stream))
         self._accumulate(stream)
+        if self.diagnostics:
+            self._position_means(stream)
         self.last_bracket = st.bracket_hour
         self.step_i += 1
         if self.use_graph:

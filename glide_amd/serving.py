@@ -179,6 +179,7 @@ class Server:
                  stream: int | None = None, poll_every: int = 8, pinned_slabs: int = 16,
                  keep_state: bool = False):
         self.ring = ring
+        self.grid = dict(grid)
         self.dt = int(dt)
         self.seed = int(seed)
         self.h = hanna or HannaOptions()
@@ -433,6 +434,30 @@ class Server:
                          for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm", "uid", "alive")}
         self.particle_steps += int(sv.active_steps)
         self.pending.append(_Pending(res, ev, host, total, n_t))
+
+    def write_result(self, res: Result, out_dir, *, epoch_iso: str = "2024-01-01T00:00:00") -> str:
+        """Write a finished request's footprint as a one-release footprint store
+        (time-ago bins of one hour, as many as the request's length needs) and,
+        when the server keeps final states, its endpoints as Parquet."""
+
+        from pathlib import Path
+
+        from .output import ZarrFootprintStore, footprint_coords, write_particles_parquet
+
+        out = Path(out_dir) / f"request_{res.req.req_id:06d}"
+        fp = res.footprint
+        grid = dict(self.grid, n_t=fp.shape[0])
+        store = ZarrFootprintStore(out / "footprint.zarr")
+        store.create((1,) + fp.shape, coords=footprint_coords(grid, [res.req], length_s=res.req.length_s,
+                                                              epoch_iso=epoch_iso),
+                     attrs={"request_id": int(res.req.req_id), "residence_total": float(res.total),
+                            "escaped": int(res.escaped), "steps": int(res.steps)})
+        store.write_region(0, fp[None])
+        if res.state is not None:
+            S = res.state
+            w = np.full(S["lon"].shape, 1.0 / max(1, res.req.n_particles), np.float32)
+            write_particles_parquet(out / "endpoints.parquet", S["lon"], S["lat"], S["z"], w)
+        return str(out)
 
     def take_completed(self) -> list[Result]:
         out, self.completed = self.completed, []

@@ -279,7 +279,8 @@ def network_releases(spec: MetSpec, *, n_sites: int, n_hourly: int, first_hour: 
                      n_range: tuple[int, int], seed: int, duration_s: int = 60) -> list[Release]:
     """A measurement network's hourly releases (the reference's multi-site
     batches, scripts/make_multisite_config.py): ``n_sites`` sites at seeded
-    interior positions and inlet heights, one release per site per hour for
+    interior positions with inlet heights uniform on [10, 300] m, one release
+    per site per hour for
     ``n_hourly`` hours from ``first_hour``, particle counts log-uniform in
     ``n_range``, backward lengths drawn from ``lengths_h`` (hours)."""
 
@@ -291,7 +292,7 @@ def network_releases(spec: MetSpec, *, n_sites: int, n_hourly: int, first_hour: 
         slon = float(rng.uniform(lon[len(lon) // 4], lon[3 * len(lon) // 4]))
         slat = float(rng.uniform(min(lat[len(lat) // 4], lat[3 * len(lat) // 4]),
                                  max(lat[len(lat) // 4], lat[3 * len(lat) // 4])))
-        inlet = float(rng.choice([10.0, 50.0, 100.0, 150.0]))
+        inlet = float(rng.uniform(10.0, 300.0))
         for k in range(n_hourly):
             n = int(round(float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))))
             length_h = int(rng.choice(np.asarray(lengths_h)))

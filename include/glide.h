@@ -194,6 +194,25 @@ int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, 
  * kernel, so it may sit inside a captured graph). */
 int glide_row_set(int32_t* row, int32_t value, void* stream);
 
+/*
+ * Trajectory diagnostics (replace main.py:1565, the per-step
+ * particles[:, :3].mean, and main.py:892-937, the hourly grid-mean wind).
+ * Fixed-order float64 reductions: the same inputs give the same bits.
+ *
+ * glide_particle_means: mean lon, lat, z over all s->n particles (pos_stride
+ * 1) into out[3r .. 3r+2]; r = row[0] - 1 when row is given (the step that
+ * glide_step_table has just finished; nothing is written unless
+ * 0 <= r < n_rows), else r = 0.
+ * glide_field_means: out[c] = mean of fields[c * n_per_channel ..] over
+ * n_per_channel elements, c < n_channels <= 4.
+ * Both need glide_means_workspace_bytes() of device workspace.
+ */
+size_t glide_means_workspace_bytes(void);
+int glide_particle_means(const GlideParticles* s, const int32_t* row, int32_t n_rows, double* out,
+                         void* workspace, size_t workspace_bytes, void* stream);
+int glide_field_means(const float* fields, int64_t n_per_channel, int32_t n_channels, double* out,
+                      void* workspace, size_t workspace_bytes, void* stream);
+
 /* FootprintGridder.accumulate index math on arbitrary inputs: positions
  * lon/lat/z (strided by pos_stride), active mask, weights, per-particle time
  * bin t_idx and release index (int64, as the reference passes them). */

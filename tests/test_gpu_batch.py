@@ -7,9 +7,12 @@ alone and in a batch stepped with per-release termini (noise keyed by the
 release's particle ids and absolute time): every particle bit-identical,
 footprints within float32 summation-order rounding (2e-5); compaction on/off:
 the same. Residence totals: float32 and float64 trajectories of the same noise
-drift apart, and at 8k-16k particles per release the totals then differ by up
-to 2e-4 (measured; mixed signs, no bias), while the 10k-particle release of
-the configs[0] run sits at 1e-5 against its 1e-4 bar. The bar here is 3e-4."""
+drift apart, and at 8k-16k particles per release, released between 10 and 300
+m (some of them straddle the 100 m top of the output layer), the totals then
+differ by up to 5e-4 (measured; mixed signs), while the single 10k-particle
+release of the plain run sits at 1e-5 against its 1e-4 bar. Bars here: 1e-3
+per release, and 1e-4 on the mean signed error over the releases (no bias;
+measured 1.2e-5)."""
 
 from __future__ import annotations
 
@@ -27,7 +30,8 @@ from glide_amd.synthetic import SyntheticMet, workload_config  # noqa: E402
 from oracle import run as ORUN  # noqa: E402
 
 STATE = ("lon", "lat", "z", "up", "vp", "wp", "um", "vm", "alive")
-TOTAL_BAR = 3e-4
+TOTAL_BAR = 1e-3
+BIAS_BAR = 1e-4
 
 
 @pytest.fixture(scope="module")
@@ -66,6 +70,7 @@ def test_network_batch_matches_oracle(batch, oracle_batch):
         assert fr[r].sum() > 0, r
         assert l2[r] <= 0.01, (r, l2[r])
         assert abs(dt[r]) <= TOTAL_BAR, (r, tot[r], fr[r].sum())
+    assert abs(np.mean(dt)) <= BIAS_BAR, dt
     assert int(eng.escaped.cpu().numpy().sum()) == int(sum(ref.escaped_per_step))
     # the live set shrank as releases finished
     assert eng.n_live < eng.n
