@@ -115,6 +115,8 @@ SIGNATURES = {
     "glide_step_table": (C.c_int, [P(GlideStepParams), vp, vp, P(GlideParticles), P(GlideGrid), P(GlideWindow),
                                    P(GlideFootprintGrid), P(GlideEmit), vp]),
     "glide_row_set": (C.c_int, [vp, C.c_int32, vp]),
+    "glide_column_cape": (C.c_int, [vp, vp, vp, vp, C.c_double, vp, vp, C.c_int32, C.c_int32, C.c_int32,
+                                    vp, vp, vp, vp, vp, vp]),
     "glide_means_workspace_bytes": (C.c_size_t, []),
     "glide_particle_means": (C.c_int, [P(GlideParticles), vp, C.c_int32, vp, vp, C.c_size_t, vp]),
     "glide_field_means": (C.c_int, [vp, C.c_int64, C.c_int32, vp, vp, C.c_size_t, vp]),

@@ -66,6 +66,141 @@ __global__ void convect_columns_kernel(const ColumnArgs a) {
     a.m_b[c] = mb;
 }
 
+// ---------------------------------------------------------------------------
+// Surface-parcel lift per column (compute_lcl_lnb_cape, emanuel.py:282-399,
+// with the free functions of emanuel.py:124-255), in float64. One wavefront
+// per column: lanes take levels for the fixed 30-step moist-adiabat bisection,
+// then lane 0 scans the levels in order (LCL, LNB, the CAPE sum), so the
+// result does not depend on the launch.
+// ---------------------------------------------------------------------------
+namespace thermo {
+constexpr double kG = 9.80665, kRd = 287.05, kRv = 461.5, kCp = 1005.0, kLv = 2.5e6, kP0 = 100000.0;
+constexpr double kEps = kRd / kRv, kKappa = kRd / kCp, kPmin = 1.0, kTmin = 150.0;
+
+__device__ __forceinline__ double esat(double t) {  // Bolton (1980) eq. 10, Pa
+    const double tc = t - 273.15;
+    return 611.2 * exp(17.67 * tc / (tc + 243.5));
+}
+__device__ __forceinline__ double q_to_e(double q, double p) { return q * p / (kEps + (1.0 - kEps) * q); }
+__device__ __forceinline__ double e_to_q(double e, double p) { return kEps * e / (fmax(p, kPmin) - (1.0 - kEps) * e); }
+__device__ __forceinline__ double tv(double t, double q) { return t * (1.0 + (1.0 / kEps - 1.0) * q); }
+__device__ __forceinline__ double t_lcl(double t, double q, double p) {  // Bolton (1980) eq. 22
+    const double e_hpa = fmax(q_to_e(q, p), 1.0) / 100.0;
+    return 2840.0 / fmax(3.5 * log(t) - log(e_hpa) - 4.805, 1e-3) + 55.0;
+}
+__device__ __forceinline__ double theta(double t, double p) { return t * pow(kP0 / fmax(p, kPmin), kKappa); }
+__device__ __forceinline__ double theta_e(double t, double q, double p) {
+    return theta(t, p) * exp(kLv * q / (kCp * fmax(t_lcl(t, q, p), kTmin)));
+}
+__device__ __forceinline__ double qsat(double t, double p) { return e_to_q(esat(t), p); }
+// Temperature on the moist adiabat of theta_e at pressure p: 30 halvings of
+// [180, 350] K, no early exit (emanuel.py:215-255).
+__device__ double lift_moist(double the, double p) {
+    double lo = 180.0, hi = 350.0;
+    for (int it = 0; it < 30; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (theta_e(mid, qsat(mid, p), p) > the) {
+            hi = mid;
+        } else {
+            lo = mid;
+        }
+    }
+    return 0.5 * (lo + hi);
+}
+}  // namespace thermo
+
+constexpr int kCapeWaves = 4;
+
+struct CapeArgs {
+    const float *t_s, *t_e, *q_s, *q_e;  // [nz][ncol]
+    double alpha;
+    const double* p_hpa;
+    const float* lev;
+    int nz;
+    int64_t ncol;
+    int32_t* i_lcl;
+    int32_t* i_lnb;
+    double* cape;
+    double* dtv;
+    float* conv;  // [4][ncol] or NULL
+};
+
+__global__ __launch_bounds__(64 * kCapeWaves) void column_cape_kernel(const CapeArgs a) {
+    using namespace thermo;
+    __shared__ double s_t[kCapeWaves][kConvLevels], s_lp[kCapeWaves][kConvLevels];
+    __shared__ double s_b[kCapeWaves][kConvLevels], s_d[kCapeWaves][kConvLevels];
+    __shared__ uint8_t s_below[kCapeWaves][kConvLevels];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t col = static_cast<int64_t>(blockIdx.x) * kCapeWaves + wave;
+    if (col >= a.ncol) return;  // whole waves leave together; no block barrier below
+    const double al = a.alpha;
+    auto lerp_at = [&](const float* s, const float* e, int k) {
+        const int64_t o = static_cast<int64_t>(k) * a.ncol + col;
+        return static_cast<double>(s[o]) * (1.0 - al) + static_cast<double>(e[o]) * al;
+    };
+    // surface parcel (level 0), evaluated by every lane
+    const double t0 = lerp_at(a.t_s, a.t_e, 0);
+    const double q0 = fmax(lerp_at(a.q_s, a.q_e, 0), 0.0);
+    const double p0 = a.p_hpa[0] * 100.0;
+    const double tl = t_lcl(t0, q0, p0);
+    const double p_lcl = p0 * pow(tl / fmax(t0, kTmin), kCp / kRd);
+    const double the = theta_e(t0, q0, p0);
+    const double th0 = theta(t0, p0);
+    for (int k = lane; k < a.nz; k += 64) {
+        const double t = lerp_at(a.t_s, a.t_e, k);
+        const double q = fmax(lerp_at(a.q_s, a.q_e, k), 0.0);
+        const double p = a.p_hpa[k] * 100.0;
+        const bool below = p >= p_lcl;
+        const double tp = below ? th0 * pow(p / kP0, kKappa) : lift_moist(the, p);
+        const double qp = below ? q0 : qsat(tp, p);
+        const double tve = tv(t, q), tvp = tv(tp, qp);
+        s_t[wave][k] = t;
+        s_lp[wave][k] = log(fmax(p, kPmin));
+        s_b[wave][k] = (tvp - tve) / fmax(tve, kTmin);
+        s_d[wave][k] = tvp - tve;
+        s_below[wave][k] = below ? 1 : 0;
+    }
+    // the wave's level rows are in LDS before lane 0 reads them
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane != 0) return;
+    int lcl = -1;
+    for (int k = 0; k < a.nz; ++k) {
+        if (!s_below[wave][k]) {
+            lcl = k;
+            break;
+        }
+    }
+    int lnb = -1;
+    double cape = 0.0, dtv = -1.0;
+    if (lcl >= 0) {
+        if (lcl + 1 < a.nz) dtv = s_d[wave][lcl + 1];
+        // sum over the positive-buoyancy layers from the LCL up, dz from the
+        // hypsometric equation on the environment temperature
+        for (int k = (lcl > 1 ? lcl : 1); k < a.nz; ++k) {
+            const double dz = kRd * 0.5 * (s_t[wave][k - 1] + s_t[wave][k]) * (s_lp[wave][k - 1] - s_lp[wave][k]) / kG;
+            cape += kG * fmax(s_b[wave][k], 0.0) * dz;
+        }
+        lnb = a.nz - 1;
+        for (int k = lcl + 1; k < a.nz; ++k) {
+            if (s_b[wave][k] <= 0.0) {
+                lnb = k;
+                break;
+            }
+        }
+    }
+    if (a.i_lcl) a.i_lcl[col] = lcl;
+    if (a.i_lnb) a.i_lnb[col] = lnb;
+    if (a.cape) a.cape[col] = cape;
+    if (a.dtv) a.dtv[col] = dtv;
+    if (a.conv) {
+        a.conv[col] = static_cast<float>(cape);
+        a.conv[a.ncol + col] = lcl >= 0 ? a.lev[lcl] : __builtin_huge_valf();
+        a.conv[2 * a.ncol + col] = lnb >= 0 ? a.lev[lnb] : __builtin_huge_valf();
+        a.conv[3 * a.ncol + col] = static_cast<float>(dtv);
+    }
+}
+
 // Layer masses |dp|/g with half layers at the ends (emanuel.py:263-279).
 __global__ void layer_mass_kernel(const double* p_hpa, int nz, float* lm) {
     const int k = threadIdx.x;
@@ -363,6 +498,24 @@ extern "C" int glide_convect_columns(const float* conv, const float* t_s, const 
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(convect_columns_kernel, dim3(static_cast<unsigned>((ncol + 255) / 256)), dim3(256), 0, st, a);
     hipLaunchKernelGGL(layer_mass_kernel, dim3(1), dim3(kConvLevels), 0, st, p_level_hpa, nz, layer_mass);
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}
+
+extern "C" int glide_column_cape(const float* t_s, const float* t_e, const float* q_s, const float* q_e, double alpha,
+                                 const double* p_level_hpa, const float* levels, int32_t nz, int32_t ny, int32_t nx,
+                                 int32_t* i_lcl, int32_t* i_lnb, double* cape, double* dtv, float* conv,
+                                 void* stream) {
+    GLIDE_REQUIRE(t_s && t_e && q_s && q_e && p_level_hpa, "glide_column_cape: NULL input");
+    GLIDE_REQUIRE(i_lcl || i_lnb || cape || dtv || conv, "glide_column_cape: no output");
+    GLIDE_REQUIRE(conv == nullptr || levels != nullptr, "glide_column_cape: conv needs levels");
+    GLIDE_REQUIRE(nz >= 2 && nz <= kConvLevels && ny >= 1 && nx >= 1, "bad column dimensions");
+    GLIDE_REQUIRE(alpha >= 0.0 && alpha <= 1.0, "alpha must be in [0, 1]");
+    CapeArgs a{t_s, t_e, q_s, q_e, alpha, p_level_hpa, levels, nz, static_cast<int64_t>(ny) * nx,
+               i_lcl, i_lnb, cape, dtv, conv};
+    const int64_t blocks = (a.ncol + kCapeWaves - 1) / kCapeWaves;
+    hipLaunchKernelGGL(column_cape_kernel, dim3(static_cast<unsigned>(blocks)), dim3(64 * kCapeWaves), 0,
+                       static_cast<hipStream_t>(stream), a);
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
 }

@@ -56,6 +56,10 @@ class ConvectionOptions:
     trigger_dtv_k: float = 0.9
     min_cape_j_kg: float = 50.0
     min_cloud_depth_m: float = 500.0
+    # parcel_lift: column CAPE / LCL / cloud top / T_v excess from a surface
+    # parcel lifted through each column's own temperature and humidity
+    # (glide_column_cape); otherwise the met supplies them per column
+    parcel_lift: bool = False
 
 
 def _dev(a, dtype=None, device="cuda"):
@@ -241,6 +245,7 @@ class BatchEngine:
         self.m_b = torch.empty(nc, dtype=f32, device=d)
         self.layer_mass = torch.empty(self.met.nz, dtype=f32, device=d)
         self.colsum_err = torch.zeros(nc, dtype=torch.float64, device=d)
+        self.conv_lift = torch.empty((4, self.met.ny, self.met.nx), dtype=f32, device=d)
         self.live = torch.zeros(1, dtype=torch.int32, device=d)
         # Step table for the captured step body: one GlideStepParams row per
         # schedule step (the flags of a plain emitting step), and the device
@@ -534,8 +539,15 @@ class BatchEngine:
             s3, _, phpa = self.met.resampled(st.bracket_hour, stream)
             e3, _, _ = self.met.resampled(st.bracket_hour + 1, stream)
             co = self.conv_opts
-            conv = self.met.src[st.bracket_hour]["conv"]
             nlev = self.met.nz * self.met.ny * self.met.nx
+            if co.parcel_lift:
+                conv = self.conv_lift
+                L.check(lib.glide_column_cape(
+                    s3.data_ptr() + 3 * nlev * 4, e3.data_ptr() + 3 * nlev * 4, s3.data_ptr() + 4 * nlev * 4,
+                    e3.data_ptr() + 4 * nlev * 4, float(st.alpha), phpa.data_ptr(), self.met.levels_f.data_ptr(),
+                    self.met.nz, self.met.ny, self.met.nx, None, None, None, None, conv.data_ptr(), stream))
+            else:
+                conv = self.met.src[st.bracket_hour]["conv"]
             L.check(lib.glide_convect_columns(
                 conv.data_ptr(), s3.data_ptr() + 3 * nlev * 4, e3.data_ptr() + 3 * nlev * 4,
                 float(st.alpha), phpa.data_ptr(), self.met.levels_f.data_ptr(), self.met.nz,

@@ -23,6 +23,8 @@
  *                              (met_reader.py:641-693; vertical_grid.py:166-316)
  *   glide_build_window         HannaScheme window stacks + raw met staging
  *                              (hanna.py:747-853, 1694-1830)
+ *   glide_column_cape          compute_lcl_lnb_cape per met column
+ *                              (convection/emanuel.py:282-399)
  *   glide_convect_columns      EmanuelReducedConvection.maybe_convect trigger +
  *                              closure (convection/emanuel.py:629-679)
  *   glide_convect              EmanuelReducedConvection matrix build +
@@ -296,6 +298,23 @@ int glide_build_window(const float* h3_s, const float* h3_e, const float* h2_s,
                        int32_t nz, int32_t ny, int32_t nx, double meander_coeff,
                        int32_t meander_radius, int32_t dtype, void* wind, void* aux, void* surf,
                        void* stream);
+
+/*
+ * Surface-parcel lift of every met column (compute_lcl_lnb_cape,
+ * emanuel.py:282-399, which the reference applies to the bbox-mean column,
+ * emanuel.py:593-627), float64 throughout. Per column: t = t_s (1 - alpha) +
+ * t_e alpha and q likewise, clamped >= 0, from [nz][ny][nx] float fields with
+ * level 0 the surface; pressure p_level_hpa [nz] double. Writes (each output
+ * optional): i_lcl (first level with p < p_LCL, -1 when the parcel never
+ * saturates), i_lnb (first level above the LCL with buoyancy <= 0, else
+ * nz - 1; -1 without LCL), cape (J/kg), dtv (T_v excess at LCL+1, K; -1 when
+ * off the column) and conv [4][ny][nx] float, the descriptors that
+ * glide_convect_columns takes: (cape, levels[i_lcl], levels[i_lnb], dtv),
+ * heights +inf when the index is -1.
+ */
+int glide_column_cape(const float* t_s, const float* t_e, const float* q_s, const float* q_e, double alpha,
+                      const double* p_level_hpa, const float* levels, int32_t nz, int32_t ny, int32_t nx,
+                      int32_t* i_lcl, int32_t* i_lnb, double* cape, double* dtv, float* conv, void* stream);
 
 /*
  * Per-column convection descriptors for one met bracket (device):

@@ -81,7 +81,8 @@ def lift_parcel_moist(theta_e, p, max_iter=30, t_min=180.0, t_max=350.0):
     hi = np.full_like(p, t_max)
     for _ in range(max_iter):
         mid = 0.5 * (lo + hi)
-        th = equivalent_potential_temperature(mid, saturation_specific_humidity(mid, p), p)
+        with np.errstate(over="ignore"):  # q_sat blows up near p = e_sat: theta_e = inf is "too high"
+            th = equivalent_potential_temperature(mid, saturation_specific_humidity(mid, p), p)
         too_high = th > theta_e
         hi = np.where(too_high, mid, hi)
         lo = np.where(too_high, lo, mid)
@@ -130,6 +131,37 @@ def compute_lcl_lnb_cape(t, q, p):
     neg = np.nonzero((buoy <= 0.0) & (np.arange(n) > i_lcl))[0]
     i_lnb = int(neg[0]) if neg.size else n - 1
     return i_lcl, i_lnb, cape, dtv
+
+
+def column_cape(t_s, t_e, q_s, q_e, alpha, p_hpa):
+    """compute_lcl_lnb_cape applied to every met column (the reference lifts
+    the bbox-mean column, emanuel.py:593-627): t and q lerped at alpha in
+    float64 from the [Z, Y, X] float32 fields, q clamped >= Q_MIN_KG_KG = 0
+    (emanuel.py:108, 611), pressure p_hpa [Z] * 100. Returns i_lcl, i_lnb
+    (int32), cape, dtv (float64), each [Y, X]."""
+
+    al = float(alpha)
+    t = np.asarray(t_s, np.float64) * (1.0 - al) + np.asarray(t_e, np.float64) * al
+    q = np.maximum(np.asarray(q_s, np.float64) * (1.0 - al) + np.asarray(q_e, np.float64) * al, 0.0)
+    p = np.asarray(p_hpa, np.float64) * 100.0
+    ny, nx = t.shape[1:]
+    out = [np.zeros((ny, nx), np.int32), np.zeros((ny, nx), np.int32), np.zeros((ny, nx)), np.zeros((ny, nx))]
+    for j in range(ny):
+        for i in range(nx):
+            for arr, v in zip(out, compute_lcl_lnb_cape(t[:, j, i], q[:, j, i], p)):
+                arr[j, i] = v
+    return tuple(out)
+
+
+def conv_descriptors_from_lift(i_lcl, i_lnb, cape, dtv, levels):
+    """The [4, Y, X] float32 (CAPE, LCL height, top height, T_v excess) fields
+    column_descriptors takes, from a parcel lift; heights +inf for -1."""
+
+    lev = np.asarray(levels, np.float32)
+    inf = np.float32(np.inf)
+    lcl_h = np.where(i_lcl >= 0, lev[np.maximum(i_lcl, 0)], inf)
+    top_h = np.where(i_lnb >= 0, lev[np.maximum(i_lnb, 0)], inf)
+    return np.stack([np.asarray(cape, np.float32), lcl_h, top_h, np.asarray(dtv, np.float32)]).astype(np.float32)
 
 
 def detrainment_weights(i_lcl, i_lnb, n):
