@@ -314,7 +314,13 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     return true;
 }
 
-template <bool BF16>
+// Launch modes, fixed at compile time so each instantiation keeps only its
+// own scalars live (the kernel is otherwise short of scalar registers).
+constexpr int kModePlain = 0;  // scalars from the launch
+constexpr int kModeTable = 1;  // scalars from the device table row (captured step body)
+constexpr int kModeSlots = 2;  // served slots: scalars and window per wavefront
+
+template <bool BF16, int MODE>
 __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
     __shared__ float lev[kMaxLevels];
     for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
@@ -324,12 +330,12 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
     unsigned long long* n_active = a.n_active;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     bool run = i < a.s.n;
-    if (a.table != nullptr) {
+    if constexpr (MODE == kModeTable) {
         int r = 0;
         p = load_row(a.table, a.row, r);
         x.escaped += r;
         if (n_active != nullptr) n_active += r;
-    } else if (a.slots != nullptr) {
+    } else if constexpr (MODE == kModeSlots) {
         // Served slots: one slot per wavefront (64-particle blocks).
         const int32_t slot = __builtin_amdgcn_readfirstlane(run ? a.wave_slot[i >> 6] : -1);
         bool stepping = false;
@@ -364,7 +370,7 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
     }
     bool act = false;
     if (run) act = step_particle<BF16>(a, lev, i, p, x);
-    if (a.slots != nullptr) {
+    if constexpr (MODE == kModeSlots) {
         if (n_active != nullptr) count_flags(act, n_active);  // per wavefront: its slot's counter
     } else if (n_active != nullptr) {
         // active count: wave ballots summed in LDS, one atomic per workgroup at the end
@@ -378,7 +384,7 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
             if (c) atomicAdd(n_active, static_cast<unsigned long long>(c));
         }
     }
-    if (a.table != nullptr) {
+    if constexpr (MODE == kModeTable) {
         // Every workgroup has used row[0] before it counts itself finished, so
         // the last one to finish can advance the cursor (no waiting). The count
         // is split over kRowGroups counters and a top counter, so no single
@@ -491,10 +497,17 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
     const int64_t blocks = (s->n + kStepBlock - 1) / kStepBlock;
     GLIDE_REQUIRE(blocks < (1ll << 31), "too many particles");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (w->dtype == GLIDE_MET_BF16) {
-        hipLaunchKernelGGL(step_kernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(kStepBlock), 0, st, a);
+    const dim3 grid(static_cast<unsigned>(blocks)), block(kStepBlock);
+    const bool bf16 = w->dtype == GLIDE_MET_BF16;
+    if (table != nullptr) {
+        if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeTable>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((step_kernel<false, kModeTable>), grid, block, 0, st, a);
+    } else if (slots != nullptr) {
+        if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeSlots>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((step_kernel<false, kModeSlots>), grid, block, 0, st, a);
     } else {
-        hipLaunchKernelGGL(step_kernel<false>, dim3(static_cast<unsigned>(blocks)), dim3(kStepBlock), 0, st, a);
+        if (bf16) hipLaunchKernelGGL((step_kernel<true, kModePlain>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((step_kernel<false, kModePlain>), grid, block, 0, st, a);
     }
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;

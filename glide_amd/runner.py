@@ -74,8 +74,11 @@ class DeviceMet:
     window, for a :class:`SyntheticMet` grid."""
 
     def __init__(self, met: SyntheticMet, hours, *, device="cuda", met_dtype: int = L.GLIDE_MET_F32,
-                 meander_coeff: float = 0.16, meander_radius: int = 1):
+                 meander_coeff: float = 0.16, meander_radius: int = 1, ingest=None):
         self.met = met
+        # ingest (glide_amd.ingest.MetIngest): source hours stream in ahead of
+        # the cursor instead of being made resident up front
+        self.ingest = ingest
         spec = met.spec
         self.device = device
         self.nx, self.ny, self.nz = spec.nx, spec.ny, spec.za
@@ -92,15 +95,19 @@ class DeviceMet:
         mlon = np.clip(111320.0 * np.cos(np.deg2rad(self.lat)), 1.0, None)
         self.mlon = _dev(mlon, torch.float64, device)
         self.src = {}
-        for h in hours:
-            sh = met.hour(h)
-            self.src[h] = dict(
-                src3=_dev(sh.src3, None, device), src2=_dev(sh.src2, None, device),
-                height=_dev(sh.height_agl, None, device),
-                p_hpa=_dev(sh.p_level_pa / 100.0, torch.float64, device),
-                conv=_dev(sh.conv, None, device),
-            )
-        self.zp = next(iter(self.src.values()))["src3"].shape[1]
+        if ingest is None:
+            for h in hours:
+                sh = met.hour(h)
+                self.src[h] = dict(
+                    src3=_dev(sh.src3, None, device), src2=_dev(sh.src2, None, device),
+                    height=_dev(sh.height_agl, None, device),
+                    p_hpa=_dev(sh.p_level_pa / 100.0, torch.float64, device),
+                    conv=_dev(sh.conv, None, device),
+                )
+            self.zp = next(iter(self.src.values()))["src3"].shape[1]
+        else:
+            first = max(hours) if ingest.direction < 0 else min(hours)
+            self.zp = ingest.get(first)["src3"].shape[1]
         ncell = self.nz * self.ny * self.nx
         esize = 2 if met_dtype == L.GLIDE_MET_BF16 else 4
         self.wind = torch.empty(ncell * 8 * esize, dtype=torch.uint8, device=device)
@@ -120,9 +127,7 @@ class DeviceMet:
 
         if h in self.ring:
             return self.ring[h]
-        if h not in self.src:
-            raise ValueError(f"met hour {h} not resident (have {sorted(self.src)})")
-        s = self.src[h]
+        s = self.source(h, stream)
         out3 = torch.empty((5, self.nz, self.ny, self.nx), dtype=torch.float32, device=self.device)
         p_out = torch.empty(self.nz, dtype=torch.float64, device=self.device)
         dlat = float(self.lat[1] - self.lat[0])
@@ -136,6 +141,15 @@ class DeviceMet:
             del self.ring[old]
         self.ring[h] = (out3, s["src2"], p_out)
         return self.ring[h]
+
+    def source(self, h: int, stream: int) -> dict:
+        """Device arrays of source hour ``h``, ordered before work on ``stream``."""
+
+        if self.ingest is not None:
+            return self.ingest.get(h, stream)
+        if h not in self.src:
+            raise ValueError(f"met hour {h} not resident (have {sorted(self.src)})")
+        return self.src[h]
 
     def build_window(self, h: int, stream: int) -> None:
         if self.window_hour == h:
@@ -157,7 +171,8 @@ class BatchEngine:
                  met_dtype: int = L.GLIDE_MET_F32, hanna: HannaOptions | None = None,
                  convection: ConvectionOptions | None = None, compensated: bool = False,
                  device_met: DeviceMet | None = None, reorder_every: int = 0, use_graph: bool = False,
-                 compact: bool = False, release_terminus: bool = False, diagnostics: bool = False):
+                 compact: bool = False, release_terminus: bool = False, diagnostics: bool = False,
+                 ingest=None):
         self.cfg = cfg
         # diagnostics: per-step mean particle position (inside the step body)
         # and hourly grid-mean winds, for the trajectory table
@@ -187,7 +202,7 @@ class BatchEngine:
         self.sched = cursor_schedule(rels, int(cfg["dt"]))
         hours = sorted({s.bracket_hour for s in self.sched} | {s.bracket_hour + 1 for s in self.sched})
         self.met = device_met or DeviceMet(met or SyntheticMet(cfg["spec"]), hours, device=device,
-                                           met_dtype=met_dtype,
+                                           met_dtype=met_dtype, ingest=ingest,
                                            meander_coeff=self.h.meander_coefficient,
                                            meander_radius=self.h.meander_stencil_radius)
         self.seed = int(cfg["seed"])
@@ -547,7 +562,7 @@ class BatchEngine:
                     e3.data_ptr() + 4 * nlev * 4, float(st.alpha), phpa.data_ptr(), self.met.levels_f.data_ptr(),
                     self.met.nz, self.met.ny, self.met.nx, None, None, None, None, conv.data_ptr(), stream))
             else:
-                conv = self.met.src[st.bracket_hour]["conv"]
+                conv = self.met.source(st.bracket_hour, stream)["conv"]
             L.check(lib.glide_convect_columns(
                 conv.data_ptr(), s3.data_ptr() + 3 * nlev * 4, e3.data_ptr() + 3 * nlev * 4,
                 float(st.alpha), phpa.data_ptr(), self.met.levels_f.data_ptr(), self.met.nz,

@@ -96,3 +96,11 @@ def test_oracle_release_terminus_equals_release_alone():
     # under the reference's batch terminus release a is stepped once more
     assert not np.array_equal(ref.particles["z"][sel], alone.particles["z"])
     assert sum(ref.active_per_step) > sum(both.active_per_step)
+
+
+def test_met_ingest_rejects_bad_arguments():
+    from glide_amd.ingest import MetIngest
+
+    for kw in (dict(workers=0), dict(lookahead=-1), dict(max_resident=1)):
+        with pytest.raises(ValueError, match="need workers"):
+            MetIngest(object(), **kw)
