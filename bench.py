@@ -60,6 +60,8 @@ def parse():
                     help="time only the first N steps of the network batch (0 = all)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-ingest", action="store_true", help="skip the streamed-met leg")
+    ap.add_argument("--no-plain-torch", action="store_true", help="skip the PyTorch-ops plain method leg")
+    ap.add_argument("--torch-steps", type=int, default=60, help="timed steps of the PyTorch-ops plain method")
     ap.add_argument("--ingest-workers", type=int, default=8, help="host decode threads of the streamed-met leg")
     ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="seconds of Poisson request arrivals (20 requests/s) in the serving leg")
@@ -221,6 +223,38 @@ def batched_leg(args, sptr: int, plain_value: float) -> dict:
     }
     eng.close()
     del eng
+    torch.cuda.empty_cache()
+    return out
+
+
+def plain_torch_leg(args, sptr: int) -> dict:
+    """The plain FP32 method as PyTorch ops on the same GPU (the reference's
+    GPU path restated: grid_sample, searchsorted, randn, the fixed substep
+    loop, scatter_add_), on the same workload: the first --torch-steps steps
+    of the schedule, after 3 warm-up steps on a separate instance."""
+
+    import torch
+
+    from glide_amd.runner import BatchEngine
+    from glide_amd.synthetic import SyntheticMet, workload_config
+    from glide_amd.torch_plain import TorchPlain
+
+    cfg = workload_config(WORKLOAD)
+    met = SyntheticMet(cfg["spec"])
+    eng = BatchEngine(cfg, device="cuda", met=met)
+    TorchPlain(eng, seed=1).run(3, sptr)
+    torch.cuda.synchronize()
+    tp = TorchPlain(eng, seed=2)
+    K = max(1, min(args.torch_steps, len(eng.sched)))
+    t0 = time.perf_counter()
+    tp.run(K, sptr)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    psteps = int(tp.active_steps.item())
+    out = {"workload": WORKLOAD, "value": psteps / elapsed, "unit": "particle-steps/s", "steps": K,
+           "ms_per_step": elapsed * 1e3 / K, "active_particle_steps": psteps,
+           "method": "plain FP32 as PyTorch ops (grid_sample, randn, scatter_add_), eager"}
+    del tp, eng
     torch.cuda.empty_cache()
     return out
 
@@ -395,11 +429,16 @@ def main() -> int:
         },
         "roofline": roofline,
     }
+    eng.close()
+    del eng
+    torch.cuda.empty_cache()
+    if not args.no_plain_torch:
+        out["plain_torch"] = plain_torch_leg(args, sptr)
+        out["vs_plain_torch"] = out["value"] / out["plain_torch"]["value"]
     if not args.no_batched:
-        eng.close()
-        del eng
-        torch.cuda.empty_cache()
         out["batched"] = batched_leg(args, sptr, out["value"])
+        if "plain_torch" in out:
+            out["batched"]["vs_plain_torch"] = out["batched"]["value"] / out["plain_torch"]["value"]
     if not args.no_serving:
         out["serving"] = serving_leg(args, out["value"])
     if not args.no_ingest:
