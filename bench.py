@@ -60,6 +60,7 @@ def parse():
                     help="time only the first N steps of the network batch (0 = all)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-ingest", action="store_true", help="skip the streamed-met leg")
+    ap.add_argument("--serve-out", default="", help="keep the serving leg's Zarr stores in this directory")
     ap.add_argument("--no-plain-torch", action="store_true", help="skip the PyTorch-ops plain method leg")
     ap.add_argument("--torch-steps", type=int, default=60, help="timed steps of the PyTorch-ops plain method")
     ap.add_argument("--ingest-workers", type=int, default=8, help="host decode threads of the streamed-met leg")
@@ -122,14 +123,19 @@ def serving_leg(args, plain_value: float) -> dict:
     met ring of 240 hourly brackets in bfloat16 (a 24 h synthetic cycle
     repeated over the 10-day span), each request a release of 1k-50k particles,
     6-240 h back, at one of 100 sites. Requests are admitted as slots and pool
-    blocks free up and are stepped together on their own cursors; latency is
-    arrival to footprint-on-host."""
+    blocks free up and are stepped together on their own cursors; finished
+    footprints are written to one zlib Zarr store per request on host threads,
+    and latency is arrival to footprint store on disk (the stores go to a
+    temporary directory, removed afterwards, unless --serve-out names one)."""
 
     import numpy as np
     import torch
 
     from glide_amd import _lib as L
-    from glide_amd.serving import MetRing, Server, latency_percentiles
+    import shutil
+    import tempfile
+
+    from glide_amd.serving import MetRing, ResultWriter, Server, latency_percentiles
     from glide_amd.synthetic import MetSpec, SyntheticMet, serving_requests, workload_config
 
     spec = MetSpec(nx=240, ny=160, za=60, n_hours=25, seed=20240105, lon0=2.0, lat0=72.0)
@@ -137,8 +143,10 @@ def serving_leg(args, plain_value: float) -> dict:
     ring = MetRing(met, first_hour=0, n_hours=240, met_dtype=L.GLIDE_MET_BF16, source_hour=lambda h: h % 24)
     base = workload_config("plain-gpu-1M-24h")
     reqs = serving_requests(spec, rate_per_s=20.0, seconds=args.serve_seconds)
+    out_dir = args.serve_out or tempfile.mkdtemp(prefix="glide_serve_")
+    writer = ResultWriter(out_dir, workers=4)
     srv = Server(ring, base["grid"], base["domain"], max_slots=160, pool_particles=6 << 20, n_t_max=240,
-                 seed=17)
+                 seed=17, writer=writer)
     # warm-up: one request through, not timed
     warm = serving_requests(spec, rate_per_s=20.0, seconds=0.2, seed=99)[:1]
     for r in warm:
@@ -164,10 +172,16 @@ def serving_leg(args, plain_value: float) -> dict:
         "latency_s": {"p50": lat["p50"], "p99": lat["p99"]},
         "latency_18_30h_s": {"p50": lat_day["p50"], "p99": lat_day["p99"], "n": len(day)},
         "met_storage": "bf16 ring, 240 brackets",
+        "latency_to": "footprint Zarr store (zlib) on disk",
+        "zarr_stores": len(writer.paths),
+        "zarr_bytes": sum(f.stat().st_size for p in writer.paths for f in Path(p).rglob("*") if f.is_file()),
         "slots": srv.max_slots,
         "vs_plain": value / plain_value,
         "mean_particles": float(np.mean([r.n_particles for r in reqs])) if reqs else 0.0,
     }
+    writer.close()
+    if not args.serve_out:
+        shutil.rmtree(out_dir, ignore_errors=True)
     del srv, ring
     torch.cuda.empty_cache()
     return out

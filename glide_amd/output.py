@@ -5,8 +5,8 @@
   (release, time_ago, z_bin, latitude, longitude), chunked one release per
   chunk so each finished release is written on its own, and the coordinates of
   main._build_footprint_dataset_metadata (main.py:394-445). The store is
-  written directly (JSON metadata, raw little-endian C-order chunks, no
-  compressor) in the encoding xarray reads back: ``_ARRAY_DIMENSIONS`` on
+  written directly (JSON metadata, little-endian C-order chunks, raw or
+  zlib-deflated) in the encoding xarray reads back: ``_ARRAY_DIMENSIONS`` on
   every array, non-index coordinates named in the data variable's
   ``coordinates`` attribute, datetimes as int64 nanoseconds with CF units, NaN
   as the footprint's fill value (so a release never written reads as NaN, as
@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import json
 import os
+import zlib
 from pathlib import Path
 from typing import Any, Mapping, Sequence
 
@@ -42,9 +43,19 @@ def _write_json(path: Path, obj) -> None:
     path.write_text(json.dumps(obj, indent=2, sort_keys=True, allow_nan=False))
 
 
-def _array_meta(shape, chunks, dtype: str, fill) -> dict:
+def _array_meta(shape, chunks, dtype: str, fill, compressor=None) -> dict:
     return {"zarr_format": 2, "shape": list(shape), "chunks": list(chunks), "dtype": dtype,
-            "compressor": None, "fill_value": fill, "order": "C", "filters": None, "dimension_separator": "."}
+            "compressor": compressor, "fill_value": fill, "order": "C", "filters": None, "dimension_separator": "."}
+
+
+def _compressor(name: str | None, level: int = 1):
+    """Zarr v2 compressor metadata: None, or zlib (numcodecs' Zlib codec id)."""
+
+    if name is None:
+        return None
+    if name != "zlib":
+        raise ValueError(f"unsupported compressor {name!r} (None or 'zlib')")
+    return {"id": "zlib", "level": int(level)}
 
 
 def _coord_dims(name: str, value) -> tuple[tuple[str, ...], Any]:
@@ -104,7 +115,11 @@ class ZarrFootprintStore:
         self.path = Path(path)
 
     def create(self, shape: tuple[int, ...], *, coords: Mapping[str, Any] | None = None,
-               attrs: Mapping[str, Any] | None = None, dims: tuple[str, ...] = FOOTPRINT_DIMS) -> None:
+               attrs: Mapping[str, Any] | None = None, dims: tuple[str, ...] = FOOTPRINT_DIMS,
+               compressor: str | None = None, level: int = 1) -> None:
+        """Write the store's metadata and coordinates. ``compressor="zlib"``
+        deflates the footprint chunks (sparse footprints shrink a hundredfold);
+        default: raw chunks."""
         if len(shape) != 5 or len(dims) != 5:
             raise ValueError("footprint store shape must be (R, T, Z, Y, X)")
         shape = tuple(int(s) for s in shape)
@@ -139,7 +154,7 @@ class ZarrFootprintStore:
             meta[f"{name}/.zarray"], meta[f"{name}/.zattrs"] = cm, ca
         fp_dir = self.path / "footprint"
         fp_dir.mkdir(exist_ok=True)
-        am = _array_meta(shape, (1,) + shape[1:], "<f4", "NaN")
+        am = _array_meta(shape, (1,) + shape[1:], "<f4", "NaN", _compressor(compressor, level))
         aa: dict[str, Any] = {"_ARRAY_DIMENSIONS": list(dims)}
         if aux:
             aa["coordinates"] = " ".join(sorted(aux))
@@ -150,8 +165,11 @@ class ZarrFootprintStore:
         _write_json(self.path / ".zattrs", meta[".zattrs"])
         _write_json(self.path / ".zmetadata", {"zarr_consolidated_format": 1, "metadata": meta})
 
+    def _meta(self) -> dict:
+        return json.loads((self.path / "footprint" / ".zarray").read_text())
+
     def shape(self) -> tuple[int, ...]:
-        return tuple(json.loads((self.path / "footprint" / ".zarray").read_text())["shape"])
+        return tuple(self._meta()["shape"])
 
     def write_region(self, release_start: int, footprint) -> None:
         """Write ``footprint`` (n, T, Z, Y, X) into releases
@@ -161,7 +179,9 @@ class ZarrFootprintStore:
         if arr.ndim != 5:
             raise ValueError("footprint must have shape (R, T, Z, Y, X)")
         arr = arr.astype("<f4", copy=False)
-        shape = self.shape()
+        meta = self._meta()
+        shape = tuple(meta["shape"])
+        comp = meta.get("compressor")
         if tuple(arr.shape[1:]) != tuple(shape[1:]):
             raise ValueError(f"footprint block {arr.shape[1:]} does not match the store's {tuple(shape[1:])}")
         if release_start < 0 or release_start + arr.shape[0] > shape[0]:
@@ -169,17 +189,22 @@ class ZarrFootprintStore:
         for j in range(arr.shape[0]):
             name = ".".join([str(release_start + j)] + ["0"] * 4)
             tmp = self.path / "footprint" / (name + ".tmp")
-            tmp.write_bytes(np.ascontiguousarray(arr[j]).tobytes())
+            raw = np.ascontiguousarray(arr[j]).tobytes()
+            tmp.write_bytes(zlib.compress(raw, comp["level"]) if comp else raw)
             os.replace(tmp, self.path / "footprint" / name)
 
     def read(self, release: int) -> np.ndarray:
         """One release's footprint (T, Z, Y, X); NaN if it was never written."""
 
-        shape = self.shape()
+        meta = self._meta()
+        shape = meta["shape"]
         f = self.path / "footprint" / ".".join([str(release)] + ["0"] * 4)
         if not f.exists():
             return np.full(shape[1:], np.nan, np.float32)
-        return np.frombuffer(f.read_bytes(), dtype="<f4").reshape(shape[1:]).copy()
+        raw = f.read_bytes()
+        if meta.get("compressor"):
+            raw = zlib.decompress(raw)
+        return np.frombuffer(raw, dtype="<f4").reshape(shape[1:]).copy()
 
     def read_coord(self, name: str) -> np.ndarray:
         m = json.loads((self.path / name / ".zarray").read_text())

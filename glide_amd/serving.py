@@ -169,6 +169,48 @@ class ServeStats:
     busy_s: float = 0.0
 
 
+class ResultWriter:
+    """Writes finished requests' footprints to Zarr stores on host threads, off
+    the serving loop (one zlib-deflated one-release store per request; the
+    deflate releases the interpreter lock). A result is handed back, with its
+    latency measured to the moment its store is on disk, once written; a write
+    error re-raises in the serving loop."""
+
+    def __init__(self, out_dir, *, workers: int = 4, compressor: str | None = "zlib"):
+        from concurrent.futures import ThreadPoolExecutor
+
+        self.out_dir = out_dir
+        self.compressor = compressor
+        self.pool = ThreadPoolExecutor(max_workers=int(workers), thread_name_prefix="glide-zarr")
+        self.inflight: list = []
+        self.bytes_written = 0
+        self.paths: list[str] = []
+
+    def submit(self, server: "Server", res: Result) -> None:
+        self.inflight.append((res, self.pool.submit(server.write_result, res, self.out_dir,
+                                                    compressor=self.compressor)))
+
+    def take_done(self, now_fn, block: bool = False) -> list[Result]:
+        done, keep = [], []
+        for res, fut in self.inflight:
+            if block or fut.done():
+                path = fut.result()  # re-raises a write error here
+                res.latency_s = now_fn() - res.req.arrival_s
+                self.paths.append(path)
+                done.append(res)
+            else:
+                keep.append((res, fut))
+        self.inflight = keep
+        return done
+
+    @property
+    def busy(self) -> bool:
+        return bool(self.inflight)
+
+    def close(self) -> None:
+        self.pool.shutdown(wait=True)
+
+
 class Server:
     """Continuous-batching footprint server on one GPU. All of its device work
     goes to the stream that is current when it is constructed; use it under
@@ -177,9 +219,10 @@ class Server:
     def __init__(self, ring: MetRing, grid: dict, domain: dict, *, max_slots: int, pool_particles: int,
                  n_t_max: int, seed: int, dt: int = 60, hanna: HannaOptions | None = None, device="cuda",
                  stream: int | None = None, poll_every: int = 8, pinned_slabs: int = 16,
-                 keep_state: bool = False):
+                 keep_state: bool = False, writer: ResultWriter | None = None):
         self.ring = ring
         self.grid = dict(grid)
+        self.writer = writer  # finished footprints go to Zarr stores before they count as done
         self.dt = int(dt)
         self.seed = int(seed)
         self.h = hanna or HannaOptions()
@@ -404,6 +447,10 @@ class Server:
             else:
                 keep.append(p)
         self.pending = keep
+        if self.writer is not None:
+            for res in done:
+                self.writer.submit(self, res)
+            done = self.writer.take_done(self.now, block=block and not self.pending)
         self.completed.extend(done)
         return done
 
@@ -435,7 +482,8 @@ class Server:
         self.particle_steps += int(sv.active_steps)
         self.pending.append(_Pending(res, ev, host, total, n_t))
 
-    def write_result(self, res: Result, out_dir, *, epoch_iso: str = "2024-01-01T00:00:00") -> str:
+    def write_result(self, res: Result, out_dir, *, epoch_iso: str = "2024-01-01T00:00:00",
+                     compressor: str | None = "zlib") -> str:
         """Write a finished request's footprint as a one-release footprint store
         (time-ago bins of one hour, as many as the request's length needs) and,
         when the server keeps final states, its endpoints as Parquet."""
@@ -451,7 +499,7 @@ class Server:
         store.create((1,) + fp.shape, coords=footprint_coords(grid, [res.req], length_s=res.req.length_s,
                                                               epoch_iso=epoch_iso),
                      attrs={"request_id": int(res.req.req_id), "residence_total": float(res.total),
-                            "escaped": int(res.escaped), "steps": int(res.steps)})
+                            "escaped": int(res.escaped), "steps": int(res.steps)}, compressor=compressor)
         store.write_region(0, fp[None])
         if res.state is not None:
             S = res.state
@@ -467,7 +515,7 @@ class Server:
         """Step until every admitted request has finished; return the results
         finished since the last hand-out."""
 
-        while self.running or self.pending:
+        while self.running or self.pending or (self.writer is not None and self.writer.busy):
             if self.running:
                 self.step()
             self.poll(block=not self.running)
@@ -485,7 +533,8 @@ class Server:
         waiting: list[Request] = []
         self.t_origin = time.perf_counter()
         t_busy0 = None
-        while qi < len(queue) or waiting or self.running or self.pending:
+        while qi < len(queue) or waiting or self.running or self.pending or (
+                self.writer is not None and self.writer.busy):
             now = self.now()
             while qi < len(queue) and queue[qi].arrival_s <= now:
                 waiting.append(queue[qi])
@@ -498,6 +547,9 @@ class Server:
                 self.step()
             elif self.pending:
                 self.poll(block=True)
+            elif self.writer is not None and self.writer.busy:
+                self.poll()  # writes finish on their threads; keep admitting arrivals meanwhile
+                time.sleep(0.001)
             elif qi < len(queue):
                 time.sleep(max(0.0, min(0.01, queue[qi].arrival_s - self.now())))
             stats.results.extend(self.take_completed())

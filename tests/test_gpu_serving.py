@@ -115,3 +115,29 @@ def test_admission_waits_for_free_slots(setup):
     assert [r.req.req_id for r in out] == [reqs[1].req_id]
     assert srv.admit(reqs[0])  # free again
     assert [r.req.req_id for r in srv.drain()] == [reqs[0].req_id]
+
+
+def test_finished_footprints_stream_to_zarr(setup, mixed, tmp_path):
+    """With a ResultWriter, every finished request lands in its own Zarr store
+    (zlib chunks) holding exactly the footprint the server produced, with the
+    request's coordinates; results count as done only once written."""
+    from glide_amd.output import ZarrFootprintStore
+    from glide_amd.serving import ResultWriter
+
+    cfg, met, ring, reqs = setup
+    writer = ResultWriter(tmp_path, workers=2)
+    srv = Server(ring, cfg["grid"], cfg["domain"], max_slots=8, pool_particles=8192, n_t_max=3, seed=SEED,
+                 writer=writer)
+    for r in reqs:
+        assert srv.admit(r)
+    out = {r.req.req_id: r for r in srv.drain()}
+    writer.close()
+    assert sorted(out) == [r.req_id for r in reqs] and not writer.busy
+    assert len(writer.paths) == len(reqs)
+    for rid, res in out.items():
+        st = ZarrFootprintStore(tmp_path / f"request_{rid:06d}" / "footprint.zarr")
+        assert st.shape() == (1,) + res.footprint.shape
+        assert np.array_equal(st.read(0), res.footprint)
+        np.testing.assert_allclose(res.footprint, mixed[rid].footprint, rtol=2e-5, atol=0)
+        assert st.read_coord("release_lon")[0] == pytest.approx(res.req.lon)
+        assert np.isfinite(res.latency_s)

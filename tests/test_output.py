@@ -30,7 +30,9 @@ def _decode_array(root: Path, name: str):
 
     meta = json.loads((root / name / ".zarray").read_text())
     attrs = json.loads((root / name / ".zattrs").read_text())
-    assert meta["zarr_format"] == 2 and meta["order"] == "C" and meta["compressor"] is None
+    assert meta["zarr_format"] == 2 and meta["order"] == "C"
+    comp = meta["compressor"]
+    assert comp is None or comp["id"] == "zlib"  # numcodecs Zlib: a zlib stream per chunk
     dt = np.dtype(meta["dtype"])
     shape, chunks = meta["shape"], meta["chunks"]
     fill = meta["fill_value"]
@@ -43,7 +45,12 @@ def _decode_array(root: Path, name: str):
         f = root / name / sep.join(str(i) for i in idx)
         if not f.exists():
             continue
-        block = np.frombuffer(f.read_bytes(), dtype=dt).reshape(chunks)
+        raw = f.read_bytes()
+        if comp is not None:
+            import zlib
+
+            raw = zlib.decompress(raw)
+        block = np.frombuffer(raw, dtype=dt).reshape(chunks)
         sl = tuple(slice(i * c, min((i + 1) * c, s)) for i, c, s in zip(idx, chunks, shape))
         out[sl] = block[tuple(slice(0, x.stop - x.start) for x in sl)]
     if fill is not None and dt.kind == "f" and not (isinstance(fill, float) and np.isnan(fill)):
@@ -174,3 +181,20 @@ def test_particles_and_trajectory_parquet(tmp_path):
     assert list(pd.read_parquet(tmp_path / "t.parquet")["elapsed_seconds"]) == [300, 600]
     w.write_metadata_json(str(tmp_path / "m.json"), {"hello": "world", "n": 2})
     assert json.loads((tmp_path / "m.json").read_text()) == {"hello": "world", "n": 2}
+
+
+def test_zlib_store_roundtrips_and_shrinks_sparse_footprints(tmp_path):
+    st = ZarrFootprintStore(tmp_path / "z.zarr")
+    st.create((2, 24, 1, 160, 240), compressor="zlib")
+    fp = np.zeros((2, 24, 1, 160, 240), np.float32)
+    rng = np.random.default_rng(1)
+    idx = rng.integers(0, fp[0].size, 5000)
+    fp[0].reshape(-1)[idx] = rng.random(5000).astype(np.float32)
+    st.write_region(0, fp[:1])
+    v, _ = _open(tmp_path / "z.zarr")["footprint"]
+    assert np.array_equal(v[0], fp[0]) and np.all(np.isnan(v[1]))
+    assert np.array_equal(st.read(0), fp[0])
+    size = (tmp_path / "z.zarr" / "footprint" / "0.0.0.0.0").stat().st_size
+    assert size < fp[0].nbytes / 20
+    with pytest.raises(ValueError, match="unsupported compressor"):
+        ZarrFootprintStore(tmp_path / "b.zarr").create((1, 1, 1, 1, 1), compressor="lz4")
