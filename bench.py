@@ -60,6 +60,7 @@ def parse():
                     help="time only the first N steps of the network batch (0 = all)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-ingest", action="store_true", help="skip the streamed-met leg")
+    ap.add_argument("--no-autotune", action="store_true", help="batched leg with the default launch parameters")
     ap.add_argument("--serve-out", default="", help="keep the serving leg's Zarr stores in this directory")
     ap.add_argument("--no-plain-torch", action="store_true", help="skip the PyTorch-ops plain method leg")
     ap.add_argument("--torch-steps", type=int, default=60, help="timed steps of the PyTorch-ops plain method")
@@ -200,9 +201,13 @@ def batched_leg(args, sptr: int, plain_value: float) -> dict:
     from glide_amd.runner import BatchEngine
     from glide_amd.synthetic import SyntheticMet, workload_config
 
+    from glide_amd.autotune import tune
+
     cfg = workload_config(BATCHED)
     met = SyntheticMet(cfg["spec"])
-    eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=L.GLIDE_MET_BF16, compensated=True, compact=True)
+    kw = dict(met_dtype=L.GLIDE_MET_BF16, compensated=True, compact=True)
+    tuned = tune(cfg, met, use_cache=False, **kw) if not args.no_autotune else {"best": {}, "trials": []}
+    eng = BatchEngine(cfg, device="cuda", met=met, **kw, **tuned["best"])
     n_sched = len(eng.sched)
     K = n_sched if args.batched_steps <= 0 else min(args.batched_steps, n_sched)
     eng.run(min(20, n_sched))  # warm-up, then back to the release state
@@ -231,6 +236,7 @@ def batched_leg(args, sptr: int, plain_value: float) -> dict:
         "footprint": f"{g['n_r']}x{g['n_t']}x1x{g['n_y']}x{g['n_x']} f32 + Neumaier compensation",
         "release_totals": "f64",
         "vs_plain": value / plain_value,
+        "autotune": {"chosen": tuned["best"], "trials": tuned["trials"]},
         "error_bar": ("footprint within 1 % relative L2 per release of the float64 oracle on the same noise "
                       "(tests/test_gpu_batch.py, 6-release network batch on the 64x48 grid: measured <= 1.4e-3 "
                       "with bfloat16 met, <= 3e-4 with float32 met; residence totals within 8.5e-4 / 1.9e-4)"),

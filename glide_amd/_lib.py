@@ -78,7 +78,7 @@ class GlideStepParams(C.Structure):
         ("length_s", C.c_double), ("substep_c", C.c_double), ("z_ubl_m", C.c_double),
         ("meander_timescale_s", C.c_double), ("seed", C.c_uint64),
         ("step_index", C.c_int32), ("max_substeps", C.c_int32),
-        ("flags", C.c_int32), ("reserved", C.c_int32),
+        ("flags", C.c_int32), ("block", C.c_int32),
     ]
 
 

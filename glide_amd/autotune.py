@@ -1,0 +1,98 @@
+"""Autotuner for the engine's launch and binning parameters.
+
+Two parameters change speed and nothing else:
+
+* ``step_block``: threads per workgroup of the fused step kernel (64 .. 512).
+  Smaller groups finish their tail sooner and stage the level ladder more
+  often; larger ones hold more waves per group.
+* ``reorder_every``: every how many steps the particles are re-binned by met
+  cell (sorted by cell, state permuted), so that the 64 lanes of a wavefront
+  gather from few cache lines; 0 = only the hourly compaction.
+
+Results do not depend on either (particles are independent, noise is keyed by
+particle id, counts are integers), so the tuner only times. It runs each
+candidate on its own engine over the same steps of the workload (after a
+warm-up share of the schedule, so the particles have dispersed as they will in
+the run), keeps the fastest, and caches the choice per device and workload
+shape in a JSON file, so a later run of the same shape skips the trials.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import time
+from pathlib import Path
+
+import torch
+
+DEFAULT_CANDIDATES = [
+    dict(step_block=sb, reorder_every=ro) for sb in (128, 256, 512) for ro in (0, 32)
+]
+
+
+def cache_path() -> Path:
+    return Path(os.environ.get("GLIDE_AUTOTUNE_CACHE", Path.home() / ".cache" / "glide_amd" / "autotune.json"))
+
+
+def _signature(cfg: dict, engine_kw: dict) -> str:
+    spec, g = cfg["spec"], cfg["grid"]
+    n = sum(int(r.n_particles) for r in cfg["releases"])
+    dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
+    fixed = {k: engine_kw[k] for k in sorted(engine_kw) if k in ("met_dtype", "compensated", "compact")}
+    return json.dumps([dev, spec.nx, spec.ny, spec.za, n, len(cfg["releases"]), g["n_t"], fixed], sort_keys=True)
+
+
+def tune(cfg: dict, met, *, candidates=None, steps: int = 48, warm_fraction: float = 0.25, use_cache: bool = True,
+         stream=None, **engine_kw) -> dict:
+    """Time each candidate on ``steps`` steps of ``cfg`` (after
+    ``warm_fraction`` of the schedule, capped at 1,500 steps) and return
+    ``{"best": params, "trials": [...], "cached": bool}``. ``engine_kw`` are
+    the other BatchEngine arguments of the run being tuned."""
+
+    from .runner import BatchEngine, DeviceMet
+
+    candidates = list(candidates or DEFAULT_CANDIDATES)
+    key = _signature(cfg, engine_kw)
+    path = cache_path()
+    table = {}
+    if use_cache and path.exists():
+        try:
+            table = json.loads(path.read_text())
+        except (OSError, ValueError):
+            table = {}
+        hit = table.get(key)
+        if hit is not None and hit.get("best") in candidates:
+            return {"best": hit["best"], "trials": hit.get("trials", []), "cached": True}
+    s = stream if stream is not None else torch.cuda.current_stream()
+    sptr = int(s.cuda_stream)
+    shared_met: DeviceMet | None = None
+    trials = []
+    for cand in candidates:
+        eng = BatchEngine(cfg, device="cuda", met=met, device_met=shared_met, **engine_kw, **cand)
+        shared_met = eng.met  # one met upload for all candidates
+        n_sched = len(eng.sched)
+        warm = min(int(warm_fraction * n_sched), 1500, max(0, n_sched - steps))
+        k = min(steps, n_sched - warm)
+        for _ in range(warm):
+            eng.run_step(sptr)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            eng.run_step(sptr)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        act = int(eng.active_steps.cpu().numpy()[warm:warm + k].sum())
+        trials.append(dict(cand, ms_per_step=dt * 1e3 / max(k, 1), particle_steps_per_s=act / dt if dt > 0 else 0.0))
+        eng.close()
+        del eng
+    best = max(trials, key=lambda t: t["particle_steps_per_s"])
+    best = {k: best[k] for k in candidates[0]}
+    if use_cache:
+        table[key] = {"best": best, "trials": trials}
+        try:
+            path.parent.mkdir(parents=True, exist_ok=True)
+            path.write_text(json.dumps(table, indent=1))
+        except OSError:
+            pass
+    return {"best": best, "trials": trials, "cached": False}

@@ -44,7 +44,8 @@ struct StepArgs {
 };
 
 constexpr int kMaxLevels = 128;
-constexpr int kStepBlock = 256;
+constexpr int kStepBlock = 256;     // default threads per workgroup
+constexpr int kStepBlockMax = 512;  // GlideStepParams.block may choose 64 .. 512
 constexpr uint32_t kRowGroups = GLIDE_ROW_WORDS - 2;  // finished-workgroup counters of the step cursor
 
 // The step's row of the table, read into scalar registers (one readfirstlane
@@ -321,7 +322,7 @@ constexpr int kModeTable = 1;  // scalars from the device table row (captured st
 constexpr int kModeSlots = 2;  // served slots: scalars and window per wavefront
 
 template <bool BF16, int MODE>
-__global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
+__global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
     __shared__ float lev[kMaxLevels];
     for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
     __syncthreads();
@@ -374,13 +375,13 @@ __global__ __launch_bounds__(kStepBlock) void step_kernel(const StepArgs a) {
         if (n_active != nullptr) count_flags(act, n_active);  // per wavefront: its slot's counter
     } else if (n_active != nullptr) {
         // active count: wave ballots summed in LDS, one atomic per workgroup at the end
-        __shared__ uint32_t wact[kStepBlock / 64];
+        __shared__ uint32_t wact[kStepBlockMax / 64];
         const unsigned long long m = __ballot(act);
         if ((threadIdx.x & 63) == 0) wact[threadIdx.x >> 6] = static_cast<uint32_t>(__popcll(m));
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t c = 0;
-            for (int w = 0; w < kStepBlock / 64; ++w) c += wact[w];
+            for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) c += wact[w];
             if (c) atomicAdd(n_active, static_cast<unsigned long long>(c));
         }
     }
@@ -463,6 +464,9 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
     GLIDE_REQUIRE(p->max_substeps >= 1, "max_substeps must be >= 1");
     GLIDE_REQUIRE(p->substep_c > 0, "substep_c must be > 0");
     GLIDE_REQUIRE(p->dt >= 0, "dt must be >= 0");
+    const int bs = p->block == 0 ? kStepBlock : p->block;
+    GLIDE_REQUIRE(bs >= 64 && bs <= kStepBlockMax && bs % 64 == 0,
+                  "step block %d: must be a multiple of 64 in [64, %d] (0 = %d)", p->block, kStepBlockMax, kStepBlock);
     GLIDE_REQUIRE(w->dtype == GLIDE_MET_F32 || w->dtype == GLIDE_MET_BF16, "unknown met dtype");
     if (p->flags & GLIDE_F_MEANDER) GLIDE_REQUIRE(s->um && s->vm, "meander state arrays are NULL");
     if (p->flags & GLIDE_F_MASK_INPUT) {
@@ -494,10 +498,10 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
         a.e = *e;
     }
     if (s->n == 0) return GLIDE_OK;
-    const int64_t blocks = (s->n + kStepBlock - 1) / kStepBlock;
+    const int64_t blocks = (s->n + bs - 1) / bs;
     GLIDE_REQUIRE(blocks < (1ll << 31), "too many particles");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const dim3 grid(static_cast<unsigned>(blocks)), block(kStepBlock);
+    const dim3 grid(static_cast<unsigned>(blocks)), block(static_cast<unsigned>(bs));
     const bool bf16 = w->dtype == GLIDE_MET_BF16;
     if (table != nullptr) {
         if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeTable>), grid, block, 0, st, a);

@@ -172,8 +172,11 @@ class BatchEngine:
                  convection: ConvectionOptions | None = None, compensated: bool = False,
                  device_met: DeviceMet | None = None, reorder_every: int = 0, use_graph: bool = False,
                  compact: bool = False, release_terminus: bool = False, diagnostics: bool = False,
-                 ingest=None):
+                 ingest=None, step_block: int = 0):
         self.cfg = cfg
+        # threads per workgroup of the step kernel (0 = the library's default);
+        # a tuning parameter (glide_amd.autotune), results do not depend on it
+        self.step_block = int(step_block)
         # diagnostics: per-step mean particle position (inside the step body)
         # and hourly grid-mean winds, for the trajectory table
         self.diagnostics = bool(diagnostics)
@@ -358,7 +361,7 @@ class BatchEngine:
             dt=float(st.delta_s), alpha=float(st.alpha), cursor_off=float(st.cursor_off),
             length_s=float(self.length_s), substep_c=h.substep_c, z_ubl_m=h.z_ubl_m,
             meander_timescale_s=h.meander_timescale_s, seed=self.seed, step_index=st.noise_step,
-            max_substeps=h.max_substeps, flags=flags | self.extra_flags, reserved=0)
+            max_substeps=h.max_substeps, flags=flags | self.extra_flags, block=self.step_block)
 
     def emit(self, st) -> L.GlideEmit:
         return L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(),

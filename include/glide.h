@@ -147,13 +147,15 @@ typedef struct GlideFootprintGrid {
  *   dt: this step's delta_s; alpha: the RK2-midpoint time weight;
  *   cursor_off: pre-step cursor (s after batch start); length_s: backward
  *   window length; seed/step_index: Philox key and counter word 1.
+ *   block: threads per workgroup of the step kernel launch, a multiple of 64
+ *   in [64, 512]; 0 = 256 (a tuning parameter: results do not depend on it).
  */
 typedef struct GlideStepParams {
     double dt, alpha, cursor_off, length_s;
     double substep_c, z_ubl_m, meander_timescale_s;
     uint64_t seed;
     int32_t step_index, max_substeps;
-    int32_t flags, reserved;
+    int32_t flags, block;
 } GlideStepParams;
 
 /* Step outputs (device): per-particle footprint key (n_cells = dropped) and

@@ -89,3 +89,20 @@ def test_errors_map_to_reference_exceptions():
         L.check(lib.glide_footprint_keys(1, 1, 1, 1, 1, 1, 1, 1, 1, 60.0, C.byref(g), 1, 1, None))
     assert lib.glide_footprint_workspace_bytes(0) > 0
     assert lib.glide_footprint_workspace_bytes(1 << 20) >= 16 * (1 << 20)
+
+
+def test_step_block_is_validated_before_any_launch():
+    """GlideStepParams.block (threads per workgroup of the step launch) must be
+    0 or a multiple of 64 in [64, 512]; a bad value is an argument error raised
+    before anything reaches the device (the pointers here are never read)."""
+
+    lib = L.lib()
+    fake = 0x1000
+    s = L.GlideParticles(n=64, pos_stride=1, lon=fake, lat=fake, z=fake, up=fake, vp=fake, wp=fake, um=fake,
+                         vm=fake, r_off=fake, alive=fake)
+    g = L.GlideGrid(nx=4, ny=4, nz=4, lon0=0.0, lon1=1.0, lat0=0.0, lat1=1.0, levels=fake)
+    w = L.GlideWindow(wind=fake, aux=fake, surf=fake, dtype=L.GLIDE_MET_F32)
+    for bad in (32, 100, 576, 1024, -64):
+        p = L.GlideStepParams(dt=60.0, substep_c=0.5, max_substeps=6, block=bad)
+        with pytest.raises(ValueError, match="step block"):
+            L.check(lib.glide_step(C.byref(p), C.byref(s), C.byref(g), C.byref(w), None, None, None))
