@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-ingest", action="store_true", help="skip the streamed-met leg")
     ap.add_argument("--no-autotune", action="store_true", help="batched leg with the default launch parameters")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="accumulate each step's footprint on the step's own stream (no second stream)")
     ap.add_argument("--serve-out", default="", help="keep the serving leg's Zarr stores in this directory")
     ap.add_argument("--no-plain-torch", action="store_true", help="skip the PyTorch-ops plain method leg")
     ap.add_argument("--torch-steps", type=int, default=60, help="timed steps of the PyTorch-ops plain method")
@@ -205,7 +207,7 @@ def batched_leg(args, sptr: int, plain_value: float) -> dict:
 
     cfg = workload_config(BATCHED)
     met = SyntheticMet(cfg["spec"])
-    kw = dict(met_dtype=L.GLIDE_MET_BF16, compensated=True, compact=True)
+    kw = dict(met_dtype=L.GLIDE_MET_BF16, compensated=True, compact=True, overlap=not args.no_overlap)
     tuned = tune(cfg, met, use_cache=False, **kw) if not args.no_autotune else {"best": {}, "trials": []}
     eng = BatchEngine(cfg, device="cuda", met=met, **kw, **tuned["best"])
     n_sched = len(eng.sched)
@@ -350,7 +352,7 @@ def main() -> int:
     sptr = int(stream.cuda_stream)
     torch.cuda.set_stream(stream)
     eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=met_dtype, reorder_every=args.reorder_every,
-                      use_graph=True)
+                      use_graph=True, overlap=not args.no_overlap)
     n_sched = len(eng.sched)
     K = max(1, min(args.steps, n_sched))
     W = max(0, min(args.warmup, n_sched))
@@ -440,6 +442,7 @@ def main() -> int:
             "max_substeps": eng.h.max_substeps,
             "reorder_every": args.reorder_every,
             "step_graph": bool(args.graph),
+            "accumulate_on_second_stream": not args.no_overlap,
             "footprint": f"1x{cfg['grid']['n_t']}x1x{cfg['grid']['n_y']}x{cfg['grid']['n_x']} f32",
         },
         "active_particle_steps": psteps,

@@ -39,7 +39,7 @@ def _signature(cfg: dict, engine_kw: dict) -> str:
     spec, g = cfg["spec"], cfg["grid"]
     n = sum(int(r.n_particles) for r in cfg["releases"])
     dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
-    fixed = {k: engine_kw[k] for k in sorted(engine_kw) if k in ("met_dtype", "compensated", "compact")}
+    fixed = {k: engine_kw[k] for k in sorted(engine_kw) if k in ("met_dtype", "compensated", "compact", "overlap")}
     return json.dumps([dev, spec.nx, spec.ny, spec.za, n, len(cfg["releases"]), g["n_t"], fixed], sort_keys=True)
 
 

@@ -172,8 +172,12 @@ class BatchEngine:
                  convection: ConvectionOptions | None = None, compensated: bool = False,
                  device_met: DeviceMet | None = None, reorder_every: int = 0, use_graph: bool = False,
                  compact: bool = False, release_terminus: bool = False, diagnostics: bool = False,
-                 ingest=None, step_block: int = 0):
+                 ingest=None, step_block: int = 0, overlap: bool = False):
         self.cfg = cfg
+        # overlap: each eager step's footprint accumulation runs on a second
+        # stream, concurrently with the next step's kernel (keys and values
+        # double-buffered; a buffer is reused only after its accumulation)
+        self.overlap = bool(overlap)
         # threads per workgroup of the step kernel (0 = the library's default);
         # a tuning parameter (glide_amd.autotune), results do not depend on it
         self.step_block = int(step_block)
@@ -243,6 +247,14 @@ class BatchEngine:
         self._parts = [self._make_parts(b) for b in self.bufs]
         self.keys = torch.empty(n, dtype=torch.int32, device=d)
         self.vals = torch.empty(n, dtype=f32, device=d)
+        self.emit_bufs = [(self.keys, self.vals)]
+        if self.overlap:
+            self.emit_bufs.append((torch.empty(n, dtype=torch.int32, device=d), torch.empty(n, dtype=f32, device=d)))
+            self.acc_stream = torch.cuda.Stream(device=d)  # default priority
+            self.ev_step = [torch.cuda.Event(), torch.cuda.Event()]
+            self.ev_acc = [torch.cuda.Event(), torch.cuda.Event()]
+            self.acc_pending = [False, False]
+            self._ext: dict[int, torch.cuda.Stream] = {}
         self.escaped = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)
         self.active_steps = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)  # per-step actives
         self.footprint = torch.zeros(self.n_cells, dtype=f32, device=d)
@@ -308,6 +320,8 @@ class BatchEngine:
         """Back to the release state (positions at the sources, state zero,
         release order)."""
 
+        if getattr(self, "acc_pending", None) is not None:
+            self.join()  # no accumulation may still be writing the footprint being cleared
         self.cur = 0
         b = self.bufs[0]
         ra = self.ra
@@ -363,10 +377,30 @@ class BatchEngine:
             meander_timescale_s=h.meander_timescale_s, seed=self.seed, step_index=st.noise_step,
             max_substeps=h.max_substeps, flags=flags | self.extra_flags, block=self.step_block)
 
-    def emit(self, st) -> L.GlideEmit:
-        return L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(),
+    def emit(self, st, buf: int = 0) -> L.GlideEmit:
+        keys, vals = self.emit_bufs[buf]
+        return L.GlideEmit(keys=keys.data_ptr(), vals=vals.data_ptr(),
                            escaped=self.escaped.data_ptr() + 8 * st.index,
                            active=self.active_steps.data_ptr() + 8 * st.index)
+
+    def _stream_obj(self, ptr: int):
+        s = self._ext.get(ptr)
+        if s is None:
+            s = self._ext[ptr] = (torch.cuda.ExternalStream(ptr) if ptr else torch.cuda.default_stream())
+        return s
+
+    def join(self, stream: int | None = None) -> None:
+        """Order the work that follows on ``stream`` (default: the current
+        stream) after every footprint accumulation still running on the
+        accumulation stream (overlap mode; a no-op otherwise)."""
+
+        if not self.overlap:
+            return
+        main = self._stream_obj(L.stream_ptr() if stream is None else stream)
+        for b in (0, 1):
+            if self.acc_pending[b]:
+                main.wait_event(self.ev_acc[b])
+                self.acc_pending[b] = False
 
     def _capture(self, stream: int) -> int:
         """Capture the plain step body (table-driven step, which also advances
@@ -392,11 +426,12 @@ class BatchEngine:
         L.check(rc)
         return h.value
 
-    def _accumulate(self, stream: int) -> None:
+    def _accumulate(self, stream: int, buf: int = 0) -> None:
         if self.n_live == 0:
             return
+        keys, vals = self.emit_bufs[buf]
         L.check(L.lib().glide_footprint_accumulate(
-            self.keys.data_ptr(), self.vals.data_ptr(), self.n_live, self.n_cells, self.cells_per_release,
+            keys.data_ptr(), vals.data_ptr(), self.n_live, self.n_cells, self.cells_per_release,
             self.footprint.data_ptr(), self.comp.data_ptr() if self.comp is not None else None,
             self.release_total.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(), stream))
 
@@ -533,6 +568,7 @@ class BatchEngine:
         convect = new_bracket and self.conv_opts is not None
         ev = self.step_events.get(self.step_i) if self.step_events is not None else None
         if self.use_graph and not convect and ev is None:
+            self.join(stream)  # the captured body emits into buffer 0 and accumulates in line
             key = (self.cur, self.n_live)
             g = self._graphs.get(key)
             if g is None:
@@ -546,7 +582,12 @@ class BatchEngine:
             return
         flags = self.h.flags() | (0 if convect else L.F_EMIT)
         p = self.step_params(st, flags)
-        em = self.emit(st)
+        buf = self.step_i & 1 if self.overlap else 0
+        if self.overlap and self.acc_pending[buf]:
+            # the accumulation of two steps ago still reads this buffer
+            self._stream_obj(stream).wait_event(self.ev_acc[buf])
+            self.acc_pending[buf] = False
+        em = self.emit(st, buf)
         if ev is not None:
             ev[0].record(self._torch_stream)
         L.check(lib.glide_step(C.byref(p), C.byref(self.parts), C.byref(self.met.grid),
@@ -577,7 +618,14 @@ class BatchEngine:
                                       self.i_lcl.data_ptr(), self.i_lnb.data_ptr(), self.m_b.data_ptr(),
                                       self.layer_mass.data_ptr(), self.colsum_err.data_ptr(),
                                       C.byref(self.fg), C.byref(em), stream))
-        self._accumulate(stream)
+        if self.overlap:
+            self.ev_step[buf].record(self._stream_obj(stream))
+            self.acc_stream.wait_event(self.ev_step[buf])
+            self._accumulate(int(self.acc_stream.cuda_stream), buf)
+            self.ev_acc[buf].record(self.acc_stream)
+            self.acc_pending[buf] = True
+        else:
+            self._accumulate(stream)
         if self.diagnostics:
             self._position_means(stream)
         self.last_bracket = st.bracket_hour
@@ -601,6 +649,7 @@ class BatchEngine:
         end = len(self.sched) if n_steps is None else min(len(self.sched), self.step_i + n_steps)
         while self.step_i < end:
             self.run_step()
+        self.join()
 
     def active_count(self, st) -> int:
         """Particles in their release window at step ``st`` (before escapes)."""
