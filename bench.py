@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-ingest", action="store_true", help="skip the streamed-met leg")
     ap.add_argument("--no-autotune", action="store_true", help="batched leg with the default launch parameters")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the headline run's launch parameters too (its trials then show in a kernel trace)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="accumulate each step's footprint on the step's own stream (no second stream)")
     ap.add_argument("--serve-out", default="", help="keep the serving leg's Zarr stores in this directory")
@@ -351,8 +353,14 @@ def main() -> int:
     stream = torch.cuda.Stream()  # default priority; graph capture needs a non-default stream
     sptr = int(stream.cuda_stream)
     torch.cuda.set_stream(stream)
-    eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=met_dtype, reorder_every=args.reorder_every,
-                      use_graph=True, overlap=not args.no_overlap)
+    from glide_amd.autotune import tune
+
+    kw = dict(met_dtype=met_dtype, overlap=not args.no_overlap)
+    if args.autotune and not args.reorder_every:
+        tuned = tune(cfg, met, use_cache=False, **kw)
+    else:
+        tuned = {"best": {"reorder_every": args.reorder_every}, "trials": []}
+    eng = BatchEngine(cfg, device="cuda", met=met, use_graph=True, **kw, **tuned["best"])
     n_sched = len(eng.sched)
     K = max(1, min(args.steps, n_sched))
     W = max(0, min(args.warmup, n_sched))
@@ -404,7 +412,7 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / PEAK_HBM_GBS, 5),
         "traffic": None,
-        "kernel": "glide::step_kernel<false>" if met_dtype == L.GLIDE_MET_F32 else "glide::step_kernel<true>",
+        "kernel": "glide::step_kernel<false, 0>" if met_dtype == L.GLIDE_MET_F32 else "glide::step_kernel<true, 0>",
         "kernel_avg_us": round(avg_ms * 1e3, 3),
         "kernel_launches_timed": len(t_ms),
         "bytes_per_launch": round(bytes_launch),
@@ -440,7 +448,9 @@ def main() -> int:
             "convection": "per-column emanuel",
             "meander": True,
             "max_substeps": eng.h.max_substeps,
-            "reorder_every": args.reorder_every,
+            "reorder_every": eng.reorder_every,
+            "step_block": eng.step_block or 256,
+            "autotune_trials": tuned["trials"],
             "step_graph": bool(args.graph),
             "accumulate_on_second_stream": not args.no_overlap,
             "footprint": f"1x{cfg['grid']['n_t']}x1x{cfg['grid']['n_y']}x{cfg['grid']['n_x']} f32",
