@@ -22,8 +22,9 @@ struct ReorderKeyArgs {
     int nx, ny, nz;
     const float* levels;
     float cursor_off, length_s;
+    float horizon;  // particles with r_off < horizon become active only after it: second bucket
     bool terminus;  // GLIDE_F_RELEASE_TERMINUS: lo_off holds the release terminus
-    uint32_t sentinel;
+    uint32_t ncell, sentinel;
     uint32_t* keys;
     uint32_t* idx;
     uint32_t* live;  // optional count of particles that can still become active
@@ -47,19 +48,21 @@ __global__ __launch_bounds__(256) void reorder_key_kernel(const ReorderKeyArgs a
         over = lo > a.cursor_off;
     }
     const bool done = (s.alive[i] == 0) || over;
+    // not active before the cursor passes the horizon (released further back)
+    const bool later = !done && s.r_off[i] < a.horizon;
     uint32_t key = a.sentinel;
     if (!done) {
         const int64_t ps = s.pos_stride;
         const int ix = static_cast<int>(frac_index(s.lon[i * ps], a.lon0, a.sx, a.nx));
         const int iy = static_cast<int>(frac_index(s.lat[i * ps], a.lat0, a.sy, a.ny));
         const int iz = static_cast<int>(level_index(lev, a.nz, s.z[i * ps]));
-        key = static_cast<uint32_t>((static_cast<int64_t>(iy) * a.nx + ix) * a.nz + iz);
+        key = static_cast<uint32_t>((static_cast<int64_t>(iy) * a.nx + ix) * a.nz + iz) + (later ? a.ncell : 0u);
     }
     a.keys[i] = key;
     a.idx[i] = static_cast<uint32_t>(i);
     if (a.live != nullptr) {
         // one integer atomic per wavefront
-        const unsigned long long m = __ballot(!done);
+        const unsigned long long m = __ballot(!done && !later);
         if (m != 0ull && (threadIdx.x & 63) == static_cast<unsigned>(__ffsll(static_cast<long long>(m)) - 1))
             atomicAdd(a.live, static_cast<uint32_t>(__popcll(m)));
     }
@@ -103,7 +106,8 @@ __global__ __launch_bounds__(256) void permute_kernel(const PermuteArgs a) {
 using namespace glide;
 
 extern "C" int glide_reorder_keys(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
-                                  uint32_t* keys, uint32_t* idx, uint32_t* live, void* stream) {
+                                  double horizon_off, uint32_t* keys, uint32_t* idx, uint32_t* live,
+                                  void* stream) {
     GLIDE_REQUIRE(p && s && g && keys && idx, "glide_reorder_keys: NULL argument");
     GLIDE_REQUIRE(s->lon && s->lat && s->z && s->alive && s->r_off, "particle arrays are NULL");
     GLIDE_REQUIRE(s->pos_stride >= 1 && s->n >= 0, "bad particles");
@@ -124,7 +128,11 @@ extern "C" int glide_reorder_keys(const GlideStepParams* p, const GlideParticles
     a.length_s = static_cast<float>(p->length_s);
     a.terminus = (p->flags & GLIDE_F_RELEASE_TERMINUS) != 0;
     GLIDE_REQUIRE(!a.terminus || s->lo_off, "GLIDE_F_RELEASE_TERMINUS needs lo_off");
-    a.sentinel = static_cast<uint32_t>(static_cast<int64_t>(g->nz) * g->ny * g->nx);
+    const int64_t ncell = static_cast<int64_t>(g->nz) * g->ny * g->nx;
+    GLIDE_REQUIRE(2 * ncell < 0xffffffffll, "met grid too large for the reorder keys");
+    a.ncell = static_cast<uint32_t>(ncell);
+    a.sentinel = static_cast<uint32_t>(2 * ncell);
+    a.horizon = static_cast<float>(horizon_off);
     a.keys = keys;
     a.idx = idx;
     a.live = live;

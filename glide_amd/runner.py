@@ -207,6 +207,11 @@ class BatchEngine:
         # active offset (lo_off); one length keeps the scalar form
         self.per_particle_length = len({int(r.length_s) for r in rels}) > 1 or self.release_terminus
         self.sched = cursor_schedule(rels, int(cfg["dt"]))
+        # index of the last step of each step's met hour (the hourly compaction's horizon)
+        self._bracket_end = list(range(len(self.sched)))
+        for i in range(len(self.sched) - 2, -1, -1):
+            if self.sched[i + 1].bracket_hour == self.sched[i].bracket_hour:
+                self._bracket_end[i] = self._bracket_end[i + 1]
         hours = sorted({s.bracket_hour for s in self.sched} | {s.bracket_hour + 1 for s in self.sched})
         self.met = device_met or DeviceMet(met or SyntheticMet(cfg["spec"]), hours, device=device,
                                            met_dtype=met_dtype, ingest=ingest,
@@ -268,7 +273,7 @@ class BatchEngine:
         self.ro_keys_s = torch.empty(n, dtype=torch.int32, device=d)
         self.ro_idx_s = torch.empty(n, dtype=torch.int32, device=d)
         ncell = self.met.nz * self.met.ny * self.met.nx
-        self.ro_bits = max(1, int(ncell).bit_length())
+        self.ro_bits = max(1, int(2 * ncell).bit_length())  # cells, cells of the later bucket, the sentinel
         nc = self.met.ny * self.met.nx
         self.i_lcl = torch.empty(nc, dtype=torch.int32, device=d)
         self.i_lnb = torch.empty(nc, dtype=torch.int32, device=d)
@@ -342,6 +347,7 @@ class BatchEngine:
             self.pos_means.zero_()
         self.step_i = 0
         self.n_live = self.n
+        self.horizon = -1e30  # no later bucket until the first hourly compaction
         self.last_bracket = None
         self.step_events = None
         self.row.zero_()
@@ -349,15 +355,19 @@ class BatchEngine:
     def reorder(self, st, stream: int, count_live: bool = False) -> None:
         """Sort the whole particle state by met cell (gather locality);
         particles that can never be active again go to the end. With
-        ``count_live`` the launches that follow cover only the particles that
-        can still become active (one device-to-host read of the count)."""
+        ``count_live`` (the hourly compaction) the particles that cannot be
+        active before the last step of this met hour (released further back)
+        go after the ones that can, and the launches that follow cover only
+        those (one device-to-host read of the count); a reorder within the hour
+        keeps the same split."""
 
         lib = L.lib()
         p = self.step_params(st, 0)
         if count_live:
             self.live.zero_()
+            self.horizon = float(np.float32(self.sched[self._bracket_end[st.index]].cursor_off))
         L.check(lib.glide_reorder_keys(C.byref(p), C.byref(self._parts[self.cur]), C.byref(self.met.grid),
-                                       self.ro_keys.data_ptr(), self.ro_idx.data_ptr(),
+                                       self.horizon, self.ro_keys.data_ptr(), self.ro_idx.data_ptr(),
                                        self.live.data_ptr() if count_live else None, stream))
         L.check(lib.glide_sort_pairs(self.ro_keys.data_ptr(), self.ro_idx.data_ptr(), self.n, self.ro_bits,
                                      self.ro_keys_s.data_ptr(), self.ro_idx_s.data_ptr(), self.sort_ws.data_ptr(),
@@ -557,7 +567,7 @@ class BatchEngine:
             stream = L.stream_ptr()
         st = self.sched[self.step_i]
         new_bracket = st.bracket_hour != self.last_bracket
-        if self.compact and new_bracket and self.step_i > 0:
+        if self.compact and new_bracket:
             self.reorder(st, stream, count_live=True)
         elif self.reorder_every and self.step_i > 0 and self.step_i % self.reorder_every == 0:
             self.reorder(st, stream)
@@ -680,6 +690,7 @@ class BatchEngine:
         b = self.bufs[0]
         ra = self.ra
         self.n_live = self.n
+        self.horizon = -1e30
         for k, v in (("weight", ra.weight), ("r_off", ra.r_off), ("we_off", ra.we_off), ("rel_idx", ra.rel_idx),
                      ("uid", ra.uid.view(np.int32)),
                      ("lo_off", ra.te_off if self.release_terminus else ra.lo_off)):

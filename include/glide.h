@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define GLIDE_ABI_VERSION 2
+#define GLIDE_ABI_VERSION 3
 
 #define GLIDE_OK 0
 #define GLIDE_ERR_INVALID (-1)
@@ -255,14 +255,17 @@ int glide_sort_pairs(const uint32_t* keys, const uint32_t* vals, int64_t n, int3
  * Particle reordering for gather locality and compaction of finished
  * particles (no counterpart in the reference, which keeps particles in release
  * order): keys[i] = met cell of particle i, column-major ((row * nx + column)
- * * nz + level), or nz*ny*nx for a particle that can never be active again
- * (escaped, or its earliest active offset lo_off (r_off - length_s) is after
- * cursor_off); idx[i] = i. When live is not NULL, the number of particles that
- * can still become active is added to *live (zero it first). Sort with
- * glide_sort_pairs, then permute: those particles come first.
+ * * nz + level); plus ncell = nz*ny*nx for a particle released further back
+ * than horizon_off (r_off < horizon_off: not active before the cursor passes
+ * it); 2*ncell for a particle that can never be active again (escaped, or its
+ * earliest active offset lo_off (r_off - length_s) is after cursor_off);
+ * idx[i] = i. When live is not NULL, the number of particles that can be
+ * active before the horizon is added to *live (zero it first). Sort with
+ * glide_sort_pairs, then permute: those particles come first, then the ones
+ * released further back. horizon_off below every r_off keeps one bucket.
  */
 int glide_reorder_keys(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
-                       uint32_t* keys, uint32_t* idx, uint32_t* live, void* stream);
+                       double horizon_off, uint32_t* keys, uint32_t* idx, uint32_t* live, void* stream);
 
 /* dst.array[j] = src.array[idx[j]] for every non-NULL array of the particle
  * state (structure of arrays, pos_stride 1, src and dst distinct). */
