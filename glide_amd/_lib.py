@@ -68,7 +68,7 @@ class GlideFootprintGrid(C.Structure):
         ("n_r", C.c_int32), ("n_t", C.c_int32), ("n_z", C.c_int32), ("n_y", C.c_int32),
         ("n_x", C.c_int32), ("reserved", C.c_int32),
         ("dom_lon_min", C.c_double), ("dom_lon_max", C.c_double), ("dom_lat_min", C.c_double),
-        ("dom_lat_max", C.c_double), ("dom_alt_max", C.c_double),
+        ("dom_lat_max", C.c_double), ("dom_alt_max", C.c_double), ("t_bin", vp),
     ]
 
 
@@ -117,6 +117,8 @@ SIGNATURES = {
     "glide_row_set": (C.c_int, [vp, C.c_int32, vp]),
     "glide_column_cape": (C.c_int, [vp, vp, vp, vp, C.c_double, vp, vp, C.c_int32, C.c_int32, C.c_int32,
                                     vp, vp, vp, vp, vp, vp]),
+    "glide_footprint_accumulate_binned": (C.c_int, [vp, vp, C.c_int64, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp,
+                                                    vp, vp, vp, C.c_size_t, vp]),
     "glide_means_workspace_bytes": (C.c_size_t, []),
     "glide_particle_means": (C.c_int, [P(GlideParticles), vp, C.c_int32, vp, vp, C.c_size_t, vp]),
     "glide_field_means": (C.c_int, [vp, C.c_int64, C.c_int32, vp, vp, C.c_size_t, vp]),

@@ -293,7 +293,7 @@ __global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
     }
     if (!active) {
         if (emit) {
-            a.e.keys[i] = a.fp.n_cells;
+            a.e.keys[i] = a.fp.n_keys;
             a.e.vals[i] = 0.0f;
         }
         return;

@@ -50,13 +50,22 @@ struct RedWork {
 
 struct CellTarget {
     using Acc = float;
-    uint32_t n_cells;
+    uint32_t n_cells;  // key space: flat cells, or n_r * cells_per_bin for release-binned keys
     float* slab;
     float* comp;
+    const int32_t* t_bin;  // release-binned keys: slab cell (r * n_t + t_bin[r]) * cpb + k % cpb
+    uint32_t cpb, n_t;
     __device__ __forceinline__ uint32_t seg(uint32_t key) const { return key; }
     __device__ __forceinline__ bool valid(uint32_t s) const { return s < n_cells; }
-    __device__ __forceinline__ void emit(uint32_t key, float x) const {
+    __device__ __forceinline__ void emit(uint32_t k, float x) const {
 #pragma clang fp contract(off)
+        uint32_t key = k;
+        if (t_bin != nullptr) {
+            const uint32_t r = k / cpb;
+            const int32_t tb = t_bin[r];
+            if (tb < 0 || static_cast<uint32_t>(tb) >= n_t) return;  // never written this step: no such keys
+            key = (r * n_t + static_cast<uint32_t>(tb)) * cpb + (k - r * cpb);
+        }
         if (comp == nullptr) {
             slab[key] += x;
             return;
@@ -317,10 +326,33 @@ extern "C" size_t glide_footprint_workspace_bytes(int64_t n) {
     return footprint_layout(n, nullptr, nullptr, nullptr, nullptr);
 }
 
+static int accumulate(const uint32_t* keys, const float* vals, int64_t n, uint32_t n_cells,
+                      uint32_t cells_per_release, const int32_t* t_bin, uint32_t n_t, float* slab, float* comp,
+                      double* release_total, void* workspace, size_t workspace_bytes, void* stream);
+
 extern "C" int glide_footprint_accumulate(const uint32_t* keys, const float* vals, int64_t n,
                                           uint32_t n_cells, uint32_t cells_per_release, float* slab,
                                           float* comp, double* release_total, void* workspace,
                                           size_t workspace_bytes, void* stream) {
+    return accumulate(keys, vals, n, n_cells, cells_per_release, nullptr, 1u, slab, comp, release_total, workspace,
+                      workspace_bytes, stream);
+}
+
+extern "C" int glide_footprint_accumulate_binned(const uint32_t* keys, const float* vals, int64_t n, uint32_t n_r,
+                                                 uint32_t n_t, uint32_t cells_per_bin, const int32_t* t_bin,
+                                                 float* slab, float* comp, double* release_total, void* workspace,
+                                                 size_t workspace_bytes, void* stream) {
+    GLIDE_REQUIRE(n_r >= 1 && n_t >= 1 && cells_per_bin >= 1, "bad binned footprint dimensions");
+    GLIDE_REQUIRE(static_cast<uint64_t>(n_r) * n_t * cells_per_bin < 0xffffffffull,
+                  "footprint has too many cells for 32-bit indices");
+    GLIDE_REQUIRE(t_bin != nullptr, "NULL t_bin");
+    return accumulate(keys, vals, n, n_r * cells_per_bin, cells_per_bin, t_bin, n_t, slab, comp, release_total,
+                      workspace, workspace_bytes, stream);
+}
+
+static int accumulate(const uint32_t* keys, const float* vals, int64_t n, uint32_t n_cells,
+                      uint32_t cells_per_release, const int32_t* t_bin, uint32_t n_t, float* slab, float* comp,
+                      double* release_total, void* workspace, size_t workspace_bytes, void* stream) {
     GLIDE_REQUIRE(n >= 0, "n must be >= 0");
     if (n == 0) return GLIDE_OK;
     GLIDE_REQUIRE(keys && vals && slab, "NULL keys/vals/slab");
@@ -344,7 +376,7 @@ extern "C" int glide_footprint_accumulate(const uint32_t* keys, const float* val
     const uint32_t* sv = nullptr;
     radix_sort_pairs(keys, reinterpret_cast<const uint32_t*>(vals), n, key_bits, n_cells, sw, st, &sk, &sv);
     const float* svf = reinterpret_cast<const float*>(sv);
-    const CellTarget ct{n_cells, slab, comp};
+    const CellTarget ct{n_cells, slab, comp, t_bin, cells_per_release, n_t};
     const bool rel = release_total != nullptr;
     const uint32_t cpr = rel ? cells_per_release : 1u;
     const ReleaseTarget rt{n_cells, cpr, n_cells / cpr, release_total};

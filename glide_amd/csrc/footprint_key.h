@@ -10,9 +10,11 @@ struct FpDev {
     float lon_min, lat_min, span_x, span_y;  // float32 values, as torch sees them
     float fnx, fny;
     int n_r, n_t, n_z, n_y, n_x;
-    uint32_t n_cells;
+    uint32_t n_cells;  // flat cells n_r*n_t*n_z*n_y*n_x
+    uint32_t n_keys;   // key space; the dropped key (n_cells, or n_r*n_z*n_y*n_x when binned)
     const float* z_edges;
     float dom_lon_min, dom_lon_max, dom_lat_min, dom_lat_max, dom_alt_max;
+    int32_t* t_bin;  // release-binned keys: per-release time bin of this step (NULL: flat keys)
 };
 
 inline FpDev make_fp(const GlideFootprintGrid& g) {
@@ -31,6 +33,8 @@ inline FpDev make_fp(const GlideFootprintGrid& g) {
     d.n_y = g.n_y;
     d.n_x = g.n_x;
     d.n_cells = static_cast<uint32_t>(static_cast<uint64_t>(g.n_r) * g.n_t * g.n_z * g.n_y * g.n_x);
+    d.t_bin = g.t_bin;
+    d.n_keys = g.t_bin ? static_cast<uint32_t>(static_cast<uint64_t>(g.n_r) * g.n_z * g.n_y * g.n_x) : d.n_cells;
     d.z_edges = g.z_edges;
     d.dom_lon_min = static_cast<float>(g.dom_lon_min);
     d.dom_lon_max = static_cast<float>(g.dom_lon_max);
@@ -48,20 +52,26 @@ __device__ __forceinline__ int bin_index(float c, float c_min, float span, float
     return static_cast<int>(f);
 }
 
-// Key of one particle: flat cell index, or n_cells when the particle is
-// inactive or outside any dimension (the reference's w_eff = 0 case).
+// Key of one particle: flat cell index, or n_keys when the particle is
+// inactive or outside any dimension (the reference's w_eff = 0 case). With
+// release-binned keys the time bin leaves the key and goes to t_bin[r]
+// (every lane of a release stores the same value).
 __device__ __forceinline__ uint32_t footprint_key(const FpDev& g, bool active, float lon, float lat,
                                                   float z, int64_t t_idx, int64_t r_idx) {
-    if (!active) return g.n_cells;
-    if (r_idx < 0 || r_idx >= g.n_r || t_idx < 0 || t_idx >= g.n_t) return g.n_cells;
+    if (!active) return g.n_keys;
+    if (r_idx < 0 || r_idx >= g.n_r || t_idx < 0 || t_idx >= g.n_t) return g.n_keys;
     const int xi = bin_index(lon, g.lon_min, g.span_x, g.fnx, g.n_x);
     const int yi = bin_index(lat, g.lat_min, g.span_y, g.fny, g.n_y);
-    if (xi < 0 || yi < 0) return g.n_cells;
+    if (xi < 0 || yi < 0) return g.n_keys;
     // bucketize(z, edges, right=False) - 1: bins (e_b, e_{b+1}].
     int cnt = 0;
     for (int k = 0; k <= g.n_z; ++k) cnt += (g.z_edges[k] < z) ? 1 : 0;
     const int zi = cnt - 1;
-    if (zi < 0 || zi >= g.n_z) return g.n_cells;
+    if (zi < 0 || zi >= g.n_z) return g.n_keys;
+    if (g.t_bin != nullptr) {
+        g.t_bin[r_idx] = static_cast<int32_t>(t_idx);
+        return static_cast<uint32_t>(((static_cast<uint64_t>(r_idx) * g.n_z + zi) * g.n_y + yi) * g.n_x + xi);
+    }
     const uint64_t key =
         ((((static_cast<uint64_t>(r_idx) * g.n_t + t_idx) * g.n_z + zi) * g.n_y + yi) * g.n_x + xi);
     return static_cast<uint32_t>(key);

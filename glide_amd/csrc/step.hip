@@ -92,7 +92,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     }
     if (!active) {
         if (emit) {
-            sto(a.e.keys, i4, a.fp.n_cells);
+            sto(a.e.keys, i4, a.fp.n_keys);
             sto(a.e.vals, i4, 0.0f);
         }
         return false;
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
         if (!stepping) {
             if (run && (p.flags & GLIDE_F_EMIT)) {
                 const uint32_t i4 = static_cast<uint32_t>(i) * 4u;
-                sto(a.e.keys, i4, a.fp.n_cells);
+                sto(a.e.keys, i4, a.fp.n_keys);
                 sto(a.e.vals, i4, 0.0f);
             }
             run = false;
@@ -417,7 +417,7 @@ __global__ void footprint_key_kernel(int64_t n, int64_t ps, const float* lon, co
     const bool act = active[i] != 0;
     const uint32_t k = footprint_key(g, act, lon[i * ps], lat[i * ps], z[i * ps], t_idx[i], rel_idx[i]);
     keys[i] = k;
-    vals[i] = (k == g.n_cells) ? 0.0f : weight[i] * dt;
+    vals[i] = (k == g.n_keys) ? 0.0f : weight[i] * dt;
 }
 
 static int fill_grid(StepArgs& a, const GlideGrid& g, const GlideWindow& w) {

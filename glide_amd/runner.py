@@ -172,8 +172,11 @@ class BatchEngine:
                  convection: ConvectionOptions | None = None, compensated: bool = False,
                  device_met: DeviceMet | None = None, reorder_every: int = 0, use_graph: bool = False,
                  compact: bool = False, release_terminus: bool = False, diagnostics: bool = False,
-                 ingest=None, step_block: int = 0, overlap: bool = False):
+                 ingest=None, step_block: int = 0, overlap: bool = False, binned_keys: bool = True):
         self.cfg = cfg
+        # binned_keys: footprint keys without the time bin (a per-release table
+        # carries it): shorter keys, fewer sort passes, the same sums bit for bit
+        self.binned_keys = bool(binned_keys)
         # overlap: each eager step's footprint accumulation runs on a second
         # stream, concurrently with the next step's kernel (keys and values
         # double-buffered; a buffer is reused only after its accumulation)
@@ -255,6 +258,15 @@ class BatchEngine:
         self.emit_bufs = [(self.keys, self.vals)]
         if self.overlap:
             self.emit_bufs.append((torch.empty(n, dtype=torch.int32, device=d), torch.empty(n, dtype=f32, device=d)))
+        # per emit buffer: the release time-bin table of binned keys and the
+        # footprint grid naming it
+        g = self.grid_cfg
+        self.t_bins = [torch.full((g["n_r"],), -1, dtype=torch.int32, device=d) for _ in self.emit_bufs]
+        self.fgs = []
+        for tb in self.t_bins:
+            fg = L.GlideFootprintGrid(**{f: getattr(self.fg, f) for f, _ in L.GlideFootprintGrid._fields_})
+            fg.t_bin = tb.data_ptr() if self.binned_keys else None
+            self.fgs.append(fg)
             self.acc_stream = torch.cuda.Stream(device=d)  # default priority
             self.ev_step = [torch.cuda.Event(), torch.cuda.Event()]
             self.ev_acc = [torch.cuda.Event(), torch.cuda.Event()]
@@ -426,7 +438,7 @@ class BatchEngine:
         try:
             L.check(lib.glide_step_table(C.byref(self.step_tmpl), self.step_table.data_ptr(), self.row.data_ptr(),
                                          C.byref(self.parts), C.byref(self.met.grid), C.byref(self.met.window),
-                                         C.byref(self.fg), C.byref(em), stream))
+                                         C.byref(self.fgs[0]), C.byref(em), stream))
             self._accumulate(stream)
             if self.diagnostics:
                 self._position_means(stream, row=True)
@@ -440,6 +452,14 @@ class BatchEngine:
         if self.n_live == 0:
             return
         keys, vals = self.emit_bufs[buf]
+        if self.binned_keys:
+            g = self.grid_cfg
+            L.check(L.lib().glide_footprint_accumulate_binned(
+                keys.data_ptr(), vals.data_ptr(), self.n_live, g["n_r"], g["n_t"], g["n_z"] * g["n_y"] * g["n_x"],
+                self.t_bins[buf].data_ptr(), self.footprint.data_ptr(),
+                self.comp.data_ptr() if self.comp is not None else None, self.release_total.data_ptr(),
+                self.workspace.data_ptr(), self.workspace.numel(), stream))
+            return
         L.check(L.lib().glide_footprint_accumulate(
             keys.data_ptr(), vals.data_ptr(), self.n_live, self.n_cells, self.cells_per_release,
             self.footprint.data_ptr(), self.comp.data_ptr() if self.comp is not None else None,
@@ -601,7 +621,7 @@ class BatchEngine:
         if ev is not None:
             ev[0].record(self._torch_stream)
         L.check(lib.glide_step(C.byref(p), C.byref(self.parts), C.byref(self.met.grid),
-                               C.byref(self.met.window), C.byref(self.fg), C.byref(em), stream))
+                               C.byref(self.met.window), C.byref(self.fgs[buf]), C.byref(em), stream))
         if ev is not None:
             ev[1].record(self._torch_stream)
         if convect:
@@ -627,7 +647,7 @@ class BatchEngine:
             L.check(lib.glide_convect(C.byref(pc), C.byref(self.parts), C.byref(self.met.grid),
                                       self.i_lcl.data_ptr(), self.i_lnb.data_ptr(), self.m_b.data_ptr(),
                                       self.layer_mass.data_ptr(), self.colsum_err.data_ptr(),
-                                      C.byref(self.fg), C.byref(em), stream))
+                                      C.byref(self.fgs[buf]), C.byref(em), stream))
         if self.overlap:
             self.ev_step[buf].record(self._stream_obj(stream))
             self.acc_stream.wait_event(self.ev_step[buf])

@@ -244,7 +244,12 @@ class Server:
             n_y=grid["n_y"], n_x=grid["n_x"], reserved=0,
             dom_lon_min=domain["lon_min"], dom_lon_max=domain["lon_max"], dom_lat_min=domain["lat_min"],
             dom_lat_max=domain["lat_max"], dom_alt_max=domain["alt_max"])
-        self.cpr = self.n_t_max * self.n_z * grid["n_y"] * grid["n_x"]
+        # release-binned keys: a slot's time bin is one per step (its particles
+        # share the request's window end), so keys are slot x cell
+        self.t_bin = torch.full((self.max_slots,), -1, dtype=torch.int32, device=d)
+        self.fg.t_bin = self.t_bin.data_ptr()
+        self.cpb = self.n_z * grid["n_y"] * grid["n_x"]
+        self.cpr = self.n_t_max * self.cpb
         self.n_cells = self.max_slots * self.cpr
         if self.n_cells >= 0xFFFFFFFF:
             raise MemoryError("the slot slabs exceed 32-bit cell keys; lower max_slots or n_t_max")
@@ -396,9 +401,10 @@ class Server:
             L.check(lib.glide_step_slots(C.byref(self.tmpl), self.slots.data_ptr(), self.wave_slot.data_ptr(),
                                          C.byref(parts), C.byref(self.ring.grid), C.byref(self.ring.desc),
                                          C.byref(self.fg), C.byref(em), st))
-            L.check(lib.glide_footprint_accumulate(
-                self.keys.data_ptr(), self.vals.data_ptr(), n, self.n_cells, self.cpr, self.footprint.data_ptr(),
-                None, self.release_total.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(), st))
+            L.check(lib.glide_footprint_accumulate_binned(
+                self.keys.data_ptr(), self.vals.data_ptr(), n, self.max_slots, self.n_t_max, self.cpb,
+                self.t_bin.data_ptr(), self.footprint.data_ptr(), None, self.release_total.data_ptr(),
+                self.workspace.data_ptr(), self.workspace.numel(), st))
         self.step_count += 1
         if self.step_count % self.poll_every == 0:
             # keep at most max_inflight * poll_every steps queued ahead of the GPU,

@@ -140,6 +140,14 @@ typedef struct GlideFootprintGrid {
     const float* z_edges; /* device float[n_z + 1], strictly ascending */
     int32_t n_r, n_t, n_z, n_y, n_x, reserved;
     double dom_lon_min, dom_lon_max, dom_lat_min, dom_lat_max, dom_alt_max;
+    /* Release-binned keys (device int32[n_r], may be NULL): when set, the
+     * emitting kernels write keys r*(n_z*n_y*n_x) + (z*n_y + y)*n_x + x
+     * (dropped: n_r*n_z*n_y*n_x) and t_bin[r] = release r's time-ago bin at
+     * this step, for glide_footprint_accumulate_binned. Valid because every
+     * particle of a release shares its window end (we_off), so within a step
+     * the time bin is a property of the release: the keys are shorter (fewer
+     * sort passes) and sort in the same order as the flat cell index. */
+    int32_t* t_bin;
 } GlideFootprintGrid;
 
 /*
@@ -239,6 +247,17 @@ int glide_footprint_accumulate(const uint32_t* keys, const float* vals, int64_t 
                                uint32_t n_cells, uint32_t cells_per_release, float* slab,
                                float* comp, double* release_total, void* workspace,
                                size_t workspace_bytes, void* stream);
+
+/* glide_footprint_accumulate for release-binned keys (GlideFootprintGrid.
+ * t_bin): key k < n_r * cells_per_bin belongs to release r = k / cells_per_bin
+ * and adds to slab cell (r * n_t + t_bin[r]) * cells_per_bin + k % cells_per_bin
+ * (cells_per_bin = n_z*n_y*n_x); keys >= n_r * cells_per_bin are dropped. Same
+ * order of additions as the flat keys, so the same result bit for bit.
+ * release_total as in glide_footprint_accumulate. */
+int glide_footprint_accumulate_binned(const uint32_t* keys, const float* vals, int64_t n, uint32_t n_r,
+                                      uint32_t n_t, uint32_t cells_per_bin, const int32_t* t_bin, float* slab,
+                                      float* comp, double* release_total, void* workspace,
+                                      size_t workspace_bytes, void* stream);
 
 /* Workspace bytes glide_sort_pairs needs for n pairs. */
 size_t glide_sort_workspace_bytes(int64_t n);

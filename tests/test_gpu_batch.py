@@ -164,3 +164,19 @@ def test_overlapped_accumulation_is_bit_identical(batch, use_graph):
     assert np.array_equal(eng.escaped.cpu().numpy(), ov.escaped.cpu().numpy())
     assert np.array_equal(eng.active_steps.cpu().numpy(), ov.active_steps.cpu().numpy())
     ov.close()
+
+
+def test_binned_keys_equal_flat_keys(batch):
+    """Release-binned footprint keys (time bin in a per-release table) sort in
+    the same order as the flat cell keys, so the footprint, compensation and
+    totals are bit-identical to the flat-key run."""
+    cfg, met, eng = batch
+    flat = BatchEngine(cfg, device="cuda", met=met, compact=True, binned_keys=False)
+    flat.run()
+    torch.cuda.synchronize()
+    assert eng.binned_keys and not flat.binned_keys
+    assert np.array_equal(eng.footprint.cpu().numpy(), flat.footprint.cpu().numpy())
+    assert np.array_equal(eng.release_total.cpu().numpy(), flat.release_total.cpu().numpy())
+    a, b = eng.state_numpy(), flat.state_numpy()
+    for name in STATE:
+        assert np.array_equal(a[name], b[name]), name
