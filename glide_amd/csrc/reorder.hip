@@ -32,12 +32,8 @@ struct ReorderKeyArgs {
 
 constexpr int kMaxLev = 128;
 
-__global__ __launch_bounds__(256) void reorder_key_kernel(const ReorderKeyArgs a) {
-    __shared__ float lev[kMaxLev];
-    for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
-    __syncthreads();
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= a.s.n) return;
+// Key of particle i; returns whether it counts as live (active before the horizon).
+__device__ __forceinline__ bool key_one(const ReorderKeyArgs& a, const float* lev, int64_t i) {
     const GlideParticles& s = a.s;
     // can never be active again: the cursor only decreases
     bool over;
@@ -60,11 +56,27 @@ __global__ __launch_bounds__(256) void reorder_key_kernel(const ReorderKeyArgs a
     }
     a.keys[i] = key;
     a.idx[i] = static_cast<uint32_t>(i);
+    return !done && !later;
+}
+
+__global__ __launch_bounds__(256) void reorder_key_kernel(const ReorderKeyArgs a) {
+    __shared__ float lev[kMaxLev];
+    __shared__ uint32_t wlive[4];
+    for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
+    if (threadIdx.x < 4) wlive[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool counted = false;
+    if (i < a.s.n) counted = key_one(a, lev, i);
     if (a.live != nullptr) {
-        // one integer atomic per wavefront
-        const unsigned long long m = __ballot(!done && !later);
-        if (m != 0ull && (threadIdx.x & 63) == static_cast<unsigned>(__ffsll(static_cast<long long>(m)) - 1))
-            atomicAdd(a.live, static_cast<uint32_t>(__popcll(m)));
+        // wave ballots summed in LDS, one integer atomic per workgroup
+        const unsigned long long m = __ballot(counted);
+        if ((threadIdx.x & 63) == 0) wlive[threadIdx.x >> 6] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t c = wlive[0] + wlive[1] + wlive[2] + wlive[3];
+            if (c) atomicAdd(a.live, c);
+        }
     }
 }
 
