@@ -7,7 +7,9 @@ Two parameters change speed and nothing else:
   often; larger ones hold more waves per group.
 * ``reorder_every``: every how many steps the particles are re-binned by met
   cell (sorted by cell, state permuted), so that the 64 lanes of a wavefront
-  gather from few cache lines; 0 = only the hourly compaction.
+  gather from few cache lines and each XCD's workgroups (which take a
+  contiguous share of the particles) gather from one region of the grid;
+  0 = only the hourly compaction.
 
 Results do not depend on either (particles are independent, noise is keyed by
 particle id, counts are integers), so the tuner only times. It runs each
@@ -27,7 +29,7 @@ from pathlib import Path
 import torch
 
 DEFAULT_CANDIDATES = [
-    dict(step_block=sb, reorder_every=ro) for sb in (128, 256, 512) for ro in (0, 32)
+    dict(step_block=sb, reorder_every=ro) for sb in (256, 512) for ro in (0, 8, 16, 32)
 ]
 
 

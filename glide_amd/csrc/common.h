@@ -263,12 +263,13 @@ __device__ __forceinline__ Corner3 corners3(float ix, float iy, float iz, int nx
     const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1), z1 = min(z0 + 1, nz - 1);
     const float fx = ix - fx0, fy = iy - fy0, fz = iz - fz0;
     const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
-    const uint32_t row = static_cast<uint32_t>(nx) * rb;
-    const uint32_t lay = static_cast<uint32_t>(ny) * row;
-    const uint32_t dx = static_cast<uint32_t>(x1 - x0) * rb;
+    // column-major records: (row, column, level), a column's levels contiguous
+    const uint32_t colb = static_cast<uint32_t>(nz) * rb;
+    const uint32_t row = static_cast<uint32_t>(nx) * colb;
+    const uint32_t dx = static_cast<uint32_t>(x1 - x0) * colb;
     const uint32_t dy = static_cast<uint32_t>(y1 - y0) * row;
-    const uint32_t dz = static_cast<uint32_t>(z1 - z0) * lay;
-    const uint32_t o = static_cast<uint32_t>(z0) * lay + static_cast<uint32_t>(y0) * row + static_cast<uint32_t>(x0) * rb;
+    const uint32_t dz = static_cast<uint32_t>(z1 - z0) * rb;
+    const uint32_t o = static_cast<uint32_t>(y0) * row + static_cast<uint32_t>(x0) * colb + static_cast<uint32_t>(z0) * rb;
     Corner3 c;
     c.o[0] = o;
     c.o[1] = o + dx;

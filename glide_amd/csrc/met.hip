@@ -267,8 +267,10 @@ __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
         float wr[8] = {at(a.s3, 0, z), at(a.s3, 1, z), at(a.s3, 2, z), at(a.e3, 0, z),
                        at(a.e3, 1, z), at(a.e3, 2, z), 0.0f, 0.0f};
         float xr[8] = {sw, tl, rho(z), drho, smu, smv, 0.0f, 0.0f};
-        Rec8<BF16>::store(a.wind, zoff + col, wr);
-        Rec8<BF16>::store(a.aux, zoff + col, xr);
+        // column-major records: a column's levels are contiguous
+        const int64_t rec = col * static_cast<int64_t>(nz) + z;
+        Rec8<BF16>::store(a.wind, rec, wr);
+        Rec8<BF16>::store(a.aux, rec, xr);
     }
     if (z != 0) return;
     // Surface record: blh sp ustar shf t(level 0) at start then end.

@@ -321,6 +321,17 @@ constexpr int kModePlain = 0;  // scalars from the launch
 constexpr int kModeTable = 1;  // scalars from the device table row (captured step body)
 constexpr int kModeSlots = 2;  // served slots: scalars and window per wavefront
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8), and each XCD has its own L2. This bijection gives each XCD a
+// contiguous eighth of the particles, so particles binned by met cell keep
+// their met records in one L2 instead of all eight.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t xcd = b % kXcd, idx = b / kXcd;
+    const uint32_t q = nb / kXcd, r = nb % kXcd;
+    return xcd * q + (xcd < r ? xcd : r) + idx;
+}
+
 template <bool BF16, int MODE>
 __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
     __shared__ float lev[kMaxLevels];
@@ -329,7 +340,9 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
     GlideStepParams p = a.p;
     StepCtx x{a.wind, a.aux, a.surf, a.e.escaped, -1};
     unsigned long long* n_active = a.n_active;
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t i =
+        static_cast<int64_t>(MODE == kModeSlots ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * blockDim.x +
+        threadIdx.x;
     bool run = i < a.s.n;
     if constexpr (MODE == kModeTable) {
         int r = 0;

@@ -80,8 +80,10 @@ typedef struct GlideGrid {
 
 /*
  * One packed met window on the device (built by glide_build_window).
- *   wind: [nz][ny][nx] records of 8: u_s v_s w_s u_e v_e w_e 0 0
- *   aux : [nz][ny][nx] records of 8: sigma_w_ft T_Lw_ft rho drho_dz sigma_mu sigma_mv 0 0
+ *   wind: [ny][nx][nz] records of 8: u_s v_s w_s u_e v_e w_e 0 0
+ *   aux : [ny][nx][nz] records of 8: sigma_w_ft T_Lw_ft rho drho_dz sigma_mu sigma_mv 0 0
+ *   (column-major: a column's levels are contiguous, so particles binned by
+ *   column gather from one short run of memory whatever their levels)
  *   surf: [ny][nx] records of 12: blh sp ustar shf t0 (hour start), same (hour end), 0 0
  * wind and aux records hold float (GLIDE_MET_F32) or bfloat16
  * (GLIDE_MET_BF16) values; surf records are float in both.

@@ -76,8 +76,9 @@ def test_window_fields_match_oracle(cfg0):
     dm.build_window(0, _stream())
     torch.cuda.synchronize()
     nz, ny, nx = dm.nz, dm.ny, dm.nx
-    wind = dm.wind.view(torch.float32).cpu().numpy().reshape(nz, ny, nx, 8)
-    aux = dm.aux.view(torch.float32).cpu().numpy().reshape(nz, ny, nx, 8)
+    # column-major records [ny][nx][nz]
+    wind = dm.wind.view(torch.float32).cpu().numpy().reshape(ny, nx, nz, 8).transpose(2, 0, 1, 3)
+    aux = dm.aux.view(torch.float32).cpu().numpy().reshape(ny, nx, nz, 8).transpose(2, 0, 1, 3)
     surf = dm.surf.view(torch.float32).cpu().numpy().reshape(ny, nx, 12)
     h0, h1 = _oracle_hour(met, 0), _oracle_hour(met, 1)
     win = ORUN.build_window(h0, h1, met.spec.agl_levels, met.lon, met.lat, np.float64)
