@@ -200,3 +200,49 @@ def stream_ptr(stream=None) -> int:
 
     s = stream if stream is not None else torch.cuda.current_stream()
     return int(s.cuda_stream)
+
+
+_QUEUE_STREAMS: dict[int, "QueueStream"] = {}
+
+
+class QueueStream:
+    """A HIP stream on a hardware queue of its own, for work that must overlap
+    the stream driving the step.
+
+    A process has few hardware queues (``GPU_MAX_HW_QUEUES``, 4 by default)
+    and the runtime shares them among its streams; torch's pool streams are
+    spread over them in creation order, so a pool stream taken for the
+    accumulation can sit on the same queue as the step's stream, where the two
+    run one after the other instead of together. The runtime creates a
+    CU-masked stream on a queue of its own, and a mask of every CU leaves the
+    work where it would run anyway. Use ``queue_stream(device)``: one such
+    stream per device serves the whole process and lives until it ends (like
+    torch's pool streams), so no engine holds a queue of its own. ``torch`` is
+    a torch handle for events."""
+
+    def __init__(self, device: int = 0):
+        import torch
+
+        hip = C.CDLL("libamdhip64.so.7")  # the runtime torch and the library already use
+        hip.hipExtStreamCreateWithCUMask.argtypes = [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_uint32)]
+        ncu = torch.cuda.get_device_properties(device).multi_processor_count
+        words = (ncu + 31) // 32
+        mask = (C.c_uint32 * words)(*([0xFFFFFFFF] * words))
+        if ncu % 32:
+            mask[words - 1] = (1 << (ncu % 32)) - 1
+        raw = C.c_void_p()
+        with torch.cuda.device(device):
+            rc = hip.hipExtStreamCreateWithCUMask(C.byref(raw), words, mask)
+        if rc != 0 or not raw.value:
+            raise RuntimeError(f"hipExtStreamCreateWithCUMask failed (hipError {rc})")
+        self.ptr = int(raw.value)
+        self.torch = torch.cuda.ExternalStream(self.ptr, device=device)
+
+
+def queue_stream(device: int = 0) -> QueueStream:
+    """The process's stream on a hardware queue of its own for ``device``."""
+
+    s = _QUEUE_STREAMS.get(device)
+    if s is None:
+        s = _QUEUE_STREAMS[device] = QueueStream(device)
+    return s

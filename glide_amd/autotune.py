@@ -68,7 +68,10 @@ def tune(cfg: dict, met, *, candidates=None, steps: int = 48, warm_fraction: flo
             return {"best": hit["best"], "trials": hit.get("trials", []), "cached": True}
     s = stream if stream is not None else torch.cuda.current_stream()
     sptr = int(s.cuda_stream)
-    shared_met: DeviceMet | None = None
+    # a caller's device met (the one its timed run will use) is shared by
+    # every candidate; otherwise the first candidate uploads one
+    own_met = engine_kw.get("device_met") is None
+    shared_met: DeviceMet | None = engine_kw.pop("device_met", None)
     trials = []
     for cand in candidates:
         eng = BatchEngine(cfg, device="cuda", met=met, device_met=shared_met, **engine_kw, **cand)
@@ -88,6 +91,10 @@ def tune(cfg: dict, met, *, candidates=None, steps: int = 48, warm_fraction: flo
         trials.append(dict(cand, ms_per_step=dt * 1e3 / max(k, 1), particle_steps_per_s=act / dt if dt > 0 else 0.0))
         eng.close()
         del eng
+        torch.cuda.empty_cache()
+    if own_met:
+        shared_met = None
+        torch.cuda.empty_cache()  # the run being tuned allocates afresh
     best = max(trials, key=lambda t: t["particle_steps_per_s"])
     best = {k: best[k] for k in candidates[0]}
     if use_cache:

@@ -255,6 +255,7 @@ class BatchEngine:
         self._parts = [self._make_parts(b) for b in self.bufs]
         self.keys = torch.empty(n, dtype=torch.int32, device=d)
         self.vals = torch.empty(n, dtype=f32, device=d)
+        self._ext: dict[int, torch.cuda.Stream] = {}  # torch handles of the raw streams the engine is driven on
         self.emit_bufs = [(self.keys, self.vals)]
         if self.overlap:
             self.emit_bufs.append((torch.empty(n, dtype=torch.int32, device=d), torch.empty(n, dtype=f32, device=d)))
@@ -267,11 +268,13 @@ class BatchEngine:
             fg = L.GlideFootprintGrid(**{f: getattr(self.fg, f) for f, _ in L.GlideFootprintGrid._fields_})
             fg.t_bin = tb.data_ptr() if self.binned_keys else None
             self.fgs.append(fg)
-            self.acc_stream = torch.cuda.Stream(device=d)  # default priority
+        self.acc_stream = None
+        if self.overlap:
+            # default priority, on a hardware queue apart from the step's
+            self.acc_stream = L.queue_stream(torch.device(d).index or 0)
             self.ev_step = [torch.cuda.Event(), torch.cuda.Event()]
             self.ev_acc = [torch.cuda.Event(), torch.cuda.Event()]
-            self.acc_pending = [False, False]
-            self._ext: dict[int, torch.cuda.Stream] = {}
+        self.acc_pending = [False, False]
         self.escaped = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)
         self.active_steps = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)  # per-step actives
         self.footprint = torch.zeros(self.n_cells, dtype=f32, device=d)
@@ -381,9 +384,16 @@ class BatchEngine:
         L.check(lib.glide_reorder_keys(C.byref(p), C.byref(self._parts[self.cur]), C.byref(self.met.grid),
                                        self.horizon, self.ro_keys.data_ptr(), self.ro_idx.data_ptr(),
                                        self.live.data_ptr() if count_live else None, stream))
-        L.check(lib.glide_sort_pairs(self.ro_keys.data_ptr(), self.ro_idx.data_ptr(), self.n, self.ro_bits,
+        # the hourly compaction sorts everything; a re-binning within the hour
+        # sorts only the live prefix (what the steps launch over) and carries
+        # the rest over in place (the key kernel wrote idx[i] = i)
+        n_sort = self.n if count_live else self.n_live
+        L.check(lib.glide_sort_pairs(self.ro_keys.data_ptr(), self.ro_idx.data_ptr(), n_sort, self.ro_bits,
                                      self.ro_keys_s.data_ptr(), self.ro_idx_s.data_ptr(), self.sort_ws.data_ptr(),
                                      self.sort_ws.numel(), stream))
+        if n_sort < self.n:
+            with torch.cuda.stream(self._stream_obj(stream)):
+                self.ro_idx_s[n_sort:].copy_(self.ro_idx[n_sort:])
         nxt = 1 - self.cur
         L.check(lib.glide_permute_particles(C.byref(self._parts[self.cur]), C.byref(self._parts[nxt]),
                                             self.ro_idx_s.data_ptr(), stream))
@@ -650,9 +660,9 @@ class BatchEngine:
                                       C.byref(self.fgs[buf]), C.byref(em), stream))
         if self.overlap:
             self.ev_step[buf].record(self._stream_obj(stream))
-            self.acc_stream.wait_event(self.ev_step[buf])
-            self._accumulate(int(self.acc_stream.cuda_stream), buf)
-            self.ev_acc[buf].record(self.acc_stream)
+            self.acc_stream.torch.wait_event(self.ev_step[buf])
+            self._accumulate(self.acc_stream.ptr, buf)
+            self.ev_acc[buf].record(self.acc_stream.torch)
             self.acc_pending[buf] = True
         else:
             self._accumulate(stream)
