@@ -7,8 +7,11 @@ met hour), synthetic met on a 240 x 160 grid at 0.25 deg with 60 AGL levels and
 25 hourly snapshots, footprint 24 time-ago bins x 1 layer x 160 x 240 in float32.
 
 A "step" is one model time step of that run over all 2^20 particles: the fused
-step kernel, the deterministic footprint accumulation and, on a bracket change,
-the hourly resample, window build and convection. The met source hours are
+step kernel (which, by default, also adds every particle's residence to the
+footprint's exact fixed-point sums), the re-binning of the particles by met
+cell every few steps and, on a bracket change, the hourly resample, window
+build and convection. The float32 footprint is produced from the sums at the
+end of the timed region, inside it. The met source hours are
 resident in HBM before the timed region starts. ``value`` counts active
 particle-steps per second of wall time over the timed steps.
 
@@ -34,9 +37,12 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic HBM bytes per active particle-step of the fused step kernel
 # (float32 structure-of-arrays state): read lon lat z u' v' w' u_m v_m, weight,
 # r_off, we_off, rel_idx, alive (49 B); write lon lat z u' v' w' u_m v_m, alive
-# (33 B); emit a 4 B cell key and a 4 B value (8 B). Noise is generated in the
-# kernel (0 B). Total 90 B.
-BYTES_PER_PARTICLE_STEP = 90
+# (33 B). Noise is generated in the kernel (0 B). With the sort accumulation
+# the kernel also writes a 4 B cell key and a 4 B value (8 B; total 90 B); with
+# the fixed-point accumulation it adds to the footprint sums instead, one 8 B
+# integer atomic per run of equal cells in a wavefront, not per particle
+# (total 82 B, the atomics not counted).
+BYTES_PER_PARTICLE_STEP = {"fixed": 82, "sort": 90}
 
 
 def parse():
@@ -68,8 +74,12 @@ def parse():
     ap.add_argument("--batched-reorder", type=int, default=-1, help="batched leg: re-binning interval (no tuning)")
     ap.add_argument("--autotune", action="store_true",
                     help="tune the headline run's launch parameters too (its trials then show in a kernel trace)")
+    ap.add_argument("--accumulate", choices=("fixed", "sort"), default="fixed",
+                    help="footprint accumulation: exact fixed-point sums inside the step kernel, or keys and "
+                         "values sorted by cell and reduced in a fixed order")
     ap.add_argument("--no-overlap", action="store_true",
-                    help="accumulate each step's footprint on the step's own stream (no second stream)")
+                    help="sort accumulation: accumulate each step's footprint on the step's own stream "
+                         "(no second stream)")
     ap.add_argument("--serve-out", default="", help="keep the serving leg's Zarr stores in this directory")
     ap.add_argument("--no-plain-torch", action="store_true", help="skip the PyTorch-ops plain method leg")
     ap.add_argument("--torch-steps", type=int, default=60, help="timed steps of the PyTorch-ops plain method")
@@ -200,9 +210,10 @@ def serving_leg(args, plain_value: float) -> dict:
 def batched_leg(args, sptr: int, plain_value: float) -> dict:
     """One GPU's share of the network batch: 336 releases (7 sites x 48 hourly
     releases, 2k-20k particles, 24-120 h back) in one engine, with the
-    precision layout of the batched method (bfloat16 met records, float32
-    footprint with Neumaier compensation, float64 release totals) and the live
-    set compacted every met hour. Times the whole schedule."""
+    precision layout of the batched method (bfloat16 met records; float32
+    footprint from exact fixed-point sums, or with Neumaier compensation under
+    the sort accumulation; float64 release totals) and the live set compacted
+    every met hour. Times the whole schedule."""
 
     import numpy as np
     import torch
@@ -215,7 +226,9 @@ def batched_leg(args, sptr: int, plain_value: float) -> dict:
 
     cfg = workload_config(BATCHED)
     met = SyntheticMet(cfg["spec"])
-    kw = dict(met_dtype=L.GLIDE_MET_BF16, compensated=True, compact=True, overlap=not args.no_overlap)
+    kw = dict(met_dtype=L.GLIDE_MET_BF16, compensated=True, compact=True, overlap=not args.no_overlap,
+              accumulate=args.accumulate)
+    bpp = BYTES_PER_PARTICLE_STEP[args.accumulate]
     if args.batched_block or args.batched_reorder >= 0:
         best = {}
         if args.batched_block:
@@ -238,6 +251,7 @@ def batched_leg(args, sptr: int, plain_value: float) -> dict:
     t0 = time.perf_counter()
     for _ in range(K):
         eng.run_step(sptr)
+    eng.join(sptr)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     act = eng.active_steps.cpu().numpy()[:K].astype(np.float64)
@@ -246,7 +260,7 @@ def batched_leg(args, sptr: int, plain_value: float) -> dict:
     value = psteps / elapsed
     timed = [(i, a.elapsed_time(b)) for i, (a, b) in sorted(ev.items()) if act[i] > 0]
     t_ms = np.array([t for _, t in timed])
-    moved = float(sum(act[i] for i, _ in timed)) * BYTES_PER_PARTICLE_STEP
+    moved = float(sum(act[i] for i, _ in timed)) * bpp
     achieved = moved / (float(t_ms.sum()) * 1e-3) / 1e9 if len(t_ms) else float("nan")
     out = {
         "workload": BATCHED,
@@ -259,7 +273,9 @@ def batched_leg(args, sptr: int, plain_value: float) -> dict:
         "particles": eng.n,
         "releases": len(cfg["releases"]),
         "met_storage": "bf16",
-        "footprint": f"{g['n_r']}x{g['n_t']}x1x{g['n_y']}x{g['n_x']} f32 + Neumaier compensation",
+        "footprint": (f"{g['n_r']}x{g['n_t']}x1x{g['n_y']}x{g['n_x']} f32 "
+                      + ("from exact int64 fixed-point sums" if args.accumulate == "fixed"
+                         else "+ Neumaier compensation")),
         "release_totals": "f64",
         "vs_plain": value / plain_value,
         "autotune": {"chosen": tuned["best"], "trials": tuned["trials"]},
@@ -267,8 +283,8 @@ def batched_leg(args, sptr: int, plain_value: float) -> dict:
                      "frac": round(achieved / PEAK_HBM_GBS, 5), "kernel": "glide::step_kernel<true, 0>",
                      "kernel_launches_timed": len(t_ms),
                      "kernel_avg_us": round(float(t_ms.mean()) * 1e3, 3) if len(t_ms) else None,
-                     "bytes_per_particle_step": BYTES_PER_PARTICLE_STEP,
-                     "whole_step_achieved": round(psteps * BYTES_PER_PARTICLE_STEP / elapsed / 1e9, 2)},
+                     "bytes_per_particle_step": bpp,
+                     "whole_step_achieved": round(psteps * bpp / elapsed / 1e9, 2)},
         "error_bar": ("footprint within 1 % relative L2 per release of the float64 oracle on the same noise "
                       "(tests/test_gpu_batch.py, 6-release network batch on the 64x48 grid: measured <= 1.4e-3 "
                       "with bfloat16 met, <= 3e-4 with float32 met; residence totals within 8.5e-4 / 1.9e-4)"),
@@ -383,7 +399,7 @@ def main() -> int:
     torch.cuda.set_stream(stream)
     from glide_amd.autotune import tune
 
-    kw = dict(met_dtype=met_dtype, overlap=not args.no_overlap)
+    kw = dict(met_dtype=met_dtype, overlap=not args.no_overlap, accumulate=args.accumulate)
     if args.autotune:
         tuned = tune(cfg, met, use_cache=False, **kw)
     else:
@@ -404,6 +420,7 @@ def main() -> int:
         t0 = time.perf_counter()
         for _ in range(K):
             eng.run_step(sptr)
+        eng.join(sptr)  # the last steps' accumulations are part of the timed work
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
@@ -421,6 +438,7 @@ def main() -> int:
     t0 = time.perf_counter()
     for _ in range(K):
         eng.run_step(sptr)
+    eng.join(sptr)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
@@ -430,8 +448,9 @@ def main() -> int:
     t_ms = np.array([t for _, t in timed])
     t_act = np.array([act[i] for i, _ in timed])
     avg_ms = float(t_ms.mean()) if len(t_ms) else float("nan")
-    bytes_launch = BYTES_PER_PARTICLE_STEP * float(t_act.mean()) if len(t_act) else 0.0
-    achieved = (BYTES_PER_PARTICLE_STEP * float(t_act.sum()) / (float(t_ms.sum()) * 1e-3) / 1e9
+    bpp = BYTES_PER_PARTICLE_STEP[args.accumulate]
+    bytes_launch = bpp * float(t_act.mean()) if len(t_act) else 0.0
+    achieved = (bpp * float(t_act.sum()) / (float(t_ms.sum()) * 1e-3) / 1e9
                 if len(t_ms) else float("nan"))
     roofline = {
         "bound": "hbm",
@@ -444,9 +463,9 @@ def main() -> int:
         "kernel_avg_us": round(avg_ms * 1e3, 3),
         "kernel_launches_timed": len(t_ms),
         "bytes_per_launch": round(bytes_launch),
-        "bytes_per_particle_step": BYTES_PER_PARTICLE_STEP,
+        "bytes_per_particle_step": bpp,
     }
-    traffic_file = ROOT / "profiles" / f"traffic_{WORKLOAD}_{args.met}.json"
+    traffic_file = ROOT / "profiles" / f"traffic_{WORKLOAD}_{args.met}_{args.accumulate}.json"
     if traffic_file.exists():
         try:
             roofline["traffic"] = json.loads(traffic_file.read_text()).get("bytes_per_launch")
@@ -480,7 +499,8 @@ def main() -> int:
             "step_block": eng.step_block or 256,
             "autotune_trials": tuned["trials"],
             "step_graph": bool(args.graph),
-            "accumulate_on_second_stream": not args.no_overlap,
+            "footprint_accumulation": args.accumulate,
+            "accumulate_on_second_stream": args.accumulate == "sort" and not args.no_overlap,
             "footprint": f"1x{cfg['grid']['n_t']}x1x{cfg['grid']['n_y']}x{cfg['grid']['n_x']} f32",
         },
         "active_particle_steps": psteps,

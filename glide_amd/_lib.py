@@ -13,7 +13,7 @@ from pathlib import Path
 _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libglide_hip.so"
 
-ABI_VERSION = 3  # GLIDE_ABI_VERSION of include/glide.h
+ABI_VERSION = 4  # GLIDE_ABI_VERSION of include/glide.h
 
 GLIDE_OK = 0
 GLIDE_ERR_INVALID = -1
@@ -30,6 +30,7 @@ F_EMIT = 0x8
 F_MASK_INPUT = 0x10
 F_NO_TURB = 0x20
 F_RELEASE_TERMINUS = 0x40
+F_FIXED = 0x80
 
 GLIDE_SLOT_FREE = 0
 GLIDE_SLOT_RUNNING = 1
@@ -101,7 +102,8 @@ class GlideMetRing(C.Structure):
 
 
 class GlideEmit(C.Structure):
-    _fields_ = [("keys", vp), ("vals", vp), ("escaped", vp), ("active", vp)]
+    _fields_ = [("keys", vp), ("vals", vp), ("escaped", vp), ("active", vp), ("fixed", vp), ("fixed_exp", vp),
+                ("fixed_copies", C.c_int32), ("reserved", C.c_int32)]
 
 
 P = C.POINTER
@@ -120,6 +122,8 @@ SIGNATURES = {
     "glide_footprint_accumulate_binned": (C.c_int, [vp, vp, C.c_int64, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp,
                                                     vp, vp, vp, C.c_size_t, vp]),
     "glide_means_workspace_bytes": (C.c_size_t, []),
+    "glide_fixed_workspace_bytes": (C.c_size_t, [C.c_int32]),
+    "glide_fixed_finalize": (C.c_int, [vp, C.c_int32, C.c_int64, C.c_uint32, vp, vp, vp, vp, C.c_size_t, vp]),
     "glide_particle_means": (C.c_int, [P(GlideParticles), vp, C.c_int32, vp, vp, C.c_size_t, vp]),
     "glide_field_means": (C.c_int, [vp, C.c_int64, C.c_int32, vp, vp, C.c_size_t, vp]),
     "glide_slots_advance": (C.c_int, [vp, C.c_int32, C.c_double, vp]),
@@ -216,8 +220,8 @@ class QueueStream:
     run one after the other instead of together. The runtime creates a
     CU-masked stream on a queue of its own, and a mask of every CU leaves the
     work where it would run anyway. Use ``queue_stream(device)``: one such
-    stream per device serves the whole process and lives until it ends (like
-    torch's pool streams), so no engine holds a queue of its own. ``torch`` is
+    stream per device serves the whole process until the interpreter exits,
+    so no engine holds a queue of its own. ``torch`` is
     a torch handle for events."""
 
     def __init__(self, device: int = 0):
@@ -225,6 +229,8 @@ class QueueStream:
 
         hip = C.CDLL("libamdhip64.so.7")  # the runtime torch and the library already use
         hip.hipExtStreamCreateWithCUMask.argtypes = [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_uint32)]
+        hip.hipStreamSynchronize.argtypes = [C.c_void_p]
+        hip.hipStreamDestroy.argtypes = [C.c_void_p]
         ncu = torch.cuda.get_device_properties(device).multi_processor_count
         words = (ncu + 31) // 32
         mask = (C.c_uint32 * words)(*([0xFFFFFFFF] * words))
@@ -237,12 +243,30 @@ class QueueStream:
             raise RuntimeError(f"hipExtStreamCreateWithCUMask failed (hipError {rc})")
         self.ptr = int(raw.value)
         self.torch = torch.cuda.ExternalStream(self.ptr, device=device)
+        self._hip = hip
+
+    def destroy(self) -> None:
+        if self.ptr:
+            ptr, self.ptr = C.c_void_p(self.ptr), 0
+            self._hip.hipStreamSynchronize(ptr)
+            self._hip.hipStreamDestroy(ptr)
 
 
 def queue_stream(device: int = 0) -> QueueStream:
-    """The process's stream on a hardware queue of its own for ``device``."""
+    """The process's stream on a hardware queue of its own for ``device``
+    (destroyed, after its work ends, when the interpreter exits)."""
 
     s = _QUEUE_STREAMS.get(device)
     if s is None:
+        if not _QUEUE_STREAMS:
+            import atexit
+
+            atexit.register(_destroy_queue_streams)
         s = _QUEUE_STREAMS[device] = QueueStream(device)
     return s
+
+
+def _destroy_queue_streams() -> None:
+    for s in _QUEUE_STREAMS.values():
+        s.destroy()
+    _QUEUE_STREAMS.clear()

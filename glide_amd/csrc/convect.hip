@@ -271,19 +271,15 @@ struct ConvectArgs {
     GlideEmit e;
 };
 
-__global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
-    __shared__ float lev[kConvLevels];
-    __shared__ float lm[kConvLevels];
-    for (int k = threadIdx.x; k < a.nz; k += blockDim.x) {
-        lev[k] = a.levels[k];
-        lm[k] = a.lm[k];
-    }
-    __syncthreads();
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= a.s.n) return;
+// Convection of particle i. With GLIDE_F_FIXED its footprint key and
+// fixed-point value go to fkey/fval (the caller's dropped key and 0 stay for an
+// inactive particle) instead of memory.
+__device__ __forceinline__ void convect_particle(const ConvectArgs& a, const float* lev, const float* lm,
+                                                 int64_t i, uint32_t& fkey, long long& fval) {
     const GlideParticles& s = a.s;
     const GlideStepParams& p = a.p;
     const bool emit = (p.flags & GLIDE_F_EMIT) != 0;
+    const bool fixed = (p.flags & GLIDE_F_FIXED) != 0;
     const float co = static_cast<float>(p.cursor_off);
     bool active;
     if (p.flags & GLIDE_F_MASK_INPUT) {
@@ -292,7 +288,7 @@ __global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
         active = in_window(s, static_cast<uint32_t>(i), co, static_cast<float>(p.length_s), p.flags);
     }
     if (!active) {
-        if (emit) {
+        if (emit && !fixed) {
             a.e.keys[i] = a.fp.n_keys;
             a.e.vals[i] = 0.0f;
         }
@@ -366,12 +362,35 @@ __global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
     if (emit) {
         const int64_t tb = time_bin(s.we_off[i], co, a.fp.n_t);
         const int64_t rb = s.rel_idx ? s.rel_idx[i] : 0;
-        a.e.keys[i] = footprint_key(a.fp, true, lon, lat, z, tb, rb);
-        a.e.vals[i] = s.weight[i] * static_cast<float>(p.dt);
+        const uint32_t key = footprint_key(a.fp, true, lon, lat, z, tb, rb);
+        const float val = s.weight[i] * static_cast<float>(p.dt);
+        if (fixed) {
+            fkey = key;
+            fval = key < a.fp.n_keys ? to_fixed(val, a.e.fixed_exp[rb]) : 0;
+        } else {
+            a.e.keys[i] = key;
+            a.e.vals[i] = val;
+        }
         const bool esc = !within_domain(a.fp, lon, lat, z);
         if (esc) s.alive[i] = 0;
         count_flags(esc, a.e.escaped);
     }
+}
+
+__global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
+    __shared__ float lev[kConvLevels];
+    __shared__ float lm[kConvLevels];
+    for (int k = threadIdx.x; k < a.nz; k += blockDim.x) {
+        lev[k] = a.levels[k];
+        lm[k] = a.lm[k];
+    }
+    __syncthreads();
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint32_t fkey = a.fp.n_keys;
+    long long fval = 0;
+    if (i < a.s.n) convect_particle(a, lev, lm, i, fkey, fval);
+    if ((a.p.flags & (GLIDE_F_EMIT | GLIDE_F_FIXED)) == (GLIDE_F_EMIT | GLIDE_F_FIXED))
+        wave_fixed_add(fkey, fval, a.fp.n_keys, fixed_copy(a.e, a.fp.n_cells));  // every lane is here
 }
 
 // Mass conservation per column in float64: the float entries
@@ -550,7 +569,13 @@ extern "C" int glide_convect(const GlideStepParams* p, const GlideParticles* s, 
     a.m_b = m_b;
     a.lm = layer_mass;
     if (p->flags & GLIDE_F_EMIT) {
-        GLIDE_REQUIRE(fg && e && e->keys && e->vals && e->escaped, "emit outputs are NULL");
+        if (p->flags & GLIDE_F_FIXED) {
+            GLIDE_REQUIRE(fg && e && e->fixed && e->fixed_exp && e->escaped, "fixed-point emit outputs are NULL");
+            GLIDE_REQUIRE(fg->t_bin == nullptr, "GLIDE_F_FIXED takes flat keys (t_bin NULL)");
+            GLIDE_REQUIRE(e->fixed_copies >= 0 && e->fixed_copies <= 1024, "fixed_copies must be in [0, 1024]");
+        } else {
+            GLIDE_REQUIRE(fg && e && e->keys && e->vals && e->escaped, "emit outputs are NULL");
+        }
         GLIDE_REQUIRE(s->we_off && s->weight && s->alive, "we_off/weight/alive needed to emit");
         const uint64_t cells = static_cast<uint64_t>(fg->n_r) * fg->n_t * fg->n_z * fg->n_y * fg->n_x;
         GLIDE_REQUIRE(cells > 0 && cells < 0xffffffffull, "bad footprint size");

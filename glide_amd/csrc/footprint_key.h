@@ -91,6 +91,39 @@ __device__ __forceinline__ bool within_domain(const FpDev& g, float lon, float l
            lat <= g.dom_lat_max && z <= g.dom_alt_max;
 }
 
+// Fixed-point residence of one particle: rint(v * 2^e) (v * 2^e is exact in
+// double; the rounding only drops bits below 2^0).
+__device__ __forceinline__ long long to_fixed(float v, int e) {
+    return static_cast<long long>(llrint(ldexp(static_cast<double>(v), e)));
+}
+
+// Adds each lane's fixed-point value to acc[key] (keys >= n_keys dropped):
+// an inclusive segmented scan over runs of equal keys in lane order, then one
+// 64-bit integer atomic per run from its last lane. Integer addition is
+// associative, so the sums do not depend on the order the atomics land in.
+// Every lane of the wavefront must call it (no lane may have returned).
+__device__ __forceinline__ void wave_fixed_add(uint32_t key, long long v, uint32_t n_keys, long long* acc) {
+    if (__ballot(key < n_keys) == 0ull) return;
+    const int lane = static_cast<int>(__lane_id());
+    const uint32_t prev = __shfl_up(key, 1u);
+    const unsigned long long heads = __ballot(lane == 0 || prev != key);
+    const int start = 63 - __clzll(static_cast<long long>(heads & ((2ull << lane) - 1ull)));
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const long long o = __shfl_up(v, static_cast<unsigned>(d));
+        if (lane - d >= start) v += o;
+    }
+    const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull) != 0ull;
+    if (tail && key < n_keys && v != 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(acc + key), static_cast<unsigned long long>(v));
+}
+
+// This workgroup's copy of the fixed-point accumulator (GlideEmit.fixed_copies).
+__device__ __forceinline__ long long* fixed_copy(const GlideEmit& e, uint32_t n_cells) {
+    const uint32_t copies = e.fixed_copies > 1 ? static_cast<uint32_t>(e.fixed_copies) : 1u;
+    return e.fixed + static_cast<uint64_t>(blockIdx.x % copies) * n_cells;
+}
+
 // Adds the number of lanes with flag set to *ctr: wave ballot, one integer atomic per wave.
 __device__ __forceinline__ void count_flags(bool esc, unsigned long long* ctr) {
     const unsigned long long m = __ballot(esc);

@@ -72,16 +72,20 @@ struct StepCtx {
     int32_t slot;  // >= 0: footprint release index of a served slot; -1: rel_idx
 };
 
-// Advances particle i; returns whether it was active.
+// Advances particle i; returns whether it was active. With GLIDE_F_FIXED the
+// footprint key and fixed-point value go to fkey/fval (left as the caller set
+// them, the dropped key and 0, for an inactive particle) instead of memory.
 template <bool BF16>
 __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* lev, int64_t i,
-                                              const GlideStepParams& p, const StepCtx& x) {
+                                              const GlideStepParams& p, const StepCtx& x, uint32_t& fkey,
+                                              long long& fval) {
     // 32-bit byte offsets of this particle (the host checks the arrays fit)
     const uint32_t i1 = static_cast<uint32_t>(i);
     const uint32_t i4 = i1 * 4u;
     const uint32_t ip = static_cast<uint32_t>(i * a.s.pos_stride) * 4u;
     const GlideParticles& s = a.s;
     const bool emit = (p.flags & GLIDE_F_EMIT) != 0;
+    const bool fixed = (p.flags & GLIDE_F_FIXED) != 0;
     const float co = static_cast<float>(p.cursor_off);
 
     bool active;
@@ -91,7 +95,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         active = in_window(s, i1, co, static_cast<float>(p.length_s), p.flags);
     }
     if (!active) {
-        if (emit) {
+        if (emit && !fixed) {
             sto(a.e.keys, i4, a.fp.n_keys);
             sto(a.e.vals, i4, 0.0f);
         }
@@ -306,8 +310,15 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     if (emit) {
         const int64_t tb = time_bin(ldo(s.we_off, i4), co, a.fp.n_t);
         const int64_t rb = x.slot >= 0 ? x.slot : (s.rel_idx ? ldo(s.rel_idx, i4) : 0);
-        sto(a.e.keys, i4, footprint_key(a.fp, true, lon, lat, z, tb, rb));
-        sto(a.e.vals, i4, ldo(s.weight, i4) * dt);
+        const uint32_t key = footprint_key(a.fp, true, lon, lat, z, tb, rb);
+        const float val = ldo(s.weight, i4) * dt;
+        if (fixed) {
+            fkey = key;
+            fval = key < a.fp.n_keys ? to_fixed(val, a.e.fixed_exp[rb]) : 0;
+        } else {
+            sto(a.e.keys, i4, key);
+            sto(a.e.vals, i4, val);
+        }
         const bool esc = !within_domain(a.fp, lon, lat, z);
         if (esc) sto(s.alive, i1, static_cast<uint8_t>(0));
         count_flags(esc, x.escaped);
@@ -383,7 +394,11 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
         }
     }
     bool act = false;
-    if (run) act = step_particle<BF16>(a, lev, i, p, x);
+    uint32_t fkey = a.fp.n_keys;
+    long long fval = 0;
+    if (run) act = step_particle<BF16>(a, lev, i, p, x, fkey, fval);
+    if ((p.flags & (GLIDE_F_EMIT | GLIDE_F_FIXED)) == (GLIDE_F_EMIT | GLIDE_F_FIXED))
+        wave_fixed_add(fkey, fval, a.fp.n_keys, fixed_copy(a.e, a.fp.n_cells));  // every lane is here
     if constexpr (MODE == kModeSlots) {
         if (n_active != nullptr) count_flags(act, n_active);  // per wavefront: its slot's counter
     } else if (n_active != nullptr) {
@@ -500,7 +515,14 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
     int rc = fill_grid(a, *g, *w);
     if (rc) return rc;
     if (p->flags & GLIDE_F_EMIT) {
-        GLIDE_REQUIRE(fg && e && e->keys && e->vals && e->escaped, "emit outputs are NULL");
+        if (p->flags & GLIDE_F_FIXED) {
+            GLIDE_REQUIRE(fg && e && e->fixed && e->fixed_exp && e->escaped, "fixed-point emit outputs are NULL");
+            GLIDE_REQUIRE(fg->t_bin == nullptr, "GLIDE_F_FIXED takes flat keys (t_bin NULL)");
+            GLIDE_REQUIRE(e->fixed_copies >= 0 && e->fixed_copies <= 1024, "fixed_copies must be in [0, 1024]");
+            GLIDE_REQUIRE(slots == nullptr, "served slots emit keys (no GLIDE_F_FIXED)");
+        } else {
+            GLIDE_REQUIRE(fg && e && e->keys && e->vals && e->escaped, "emit outputs are NULL");
+        }
         GLIDE_REQUIRE(s->we_off && s->weight && s->alive, "we_off/weight/alive needed to emit");
         GLIDE_REQUIRE(fg->n_r >= 1 && fg->n_t >= 1 && fg->n_z >= 1 && fg->n_y >= 1 && fg->n_x >= 1,
                       "All footprint dimensions must be > 0");

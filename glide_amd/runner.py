@@ -172,15 +172,33 @@ class BatchEngine:
                  convection: ConvectionOptions | None = None, compensated: bool = False,
                  device_met: DeviceMet | None = None, reorder_every: int = 0, use_graph: bool = False,
                  compact: bool = False, release_terminus: bool = False, diagnostics: bool = False,
-                 ingest=None, step_block: int = 0, overlap: bool = False, binned_keys: bool = True):
+                 ingest=None, step_block: int = 0, overlap: bool = False, binned_keys: bool = True,
+                 accumulate: str = "fixed", fixed_copies: int = 0):
         self.cfg = cfg
-        # binned_keys: footprint keys without the time bin (a per-release table
-        # carries it): shorter keys, fewer sort passes, the same sums bit for bit
-        self.binned_keys = bool(binned_keys)
-        # overlap: each eager step's footprint accumulation runs on a second
-        # stream, concurrently with the next step's kernel (keys and values
-        # double-buffered; a buffer is reused only after its accumulation)
-        self.overlap = bool(overlap)
+        # accumulate: how each step's residence reaches the footprint.
+        #   "fixed": inside the step kernel, as exact 64-bit fixed-point integer
+        #     sums per cell (GLIDE_F_FIXED), converted to float32 once when the
+        #     footprint is read; order-free, so bit-identical under any launch
+        #     shape, particle order or re-binning;
+        #   "sort": keys and values written by the step, then a stable radix
+        #     sort by cell and a fixed-order segmented float reduction
+        #     (glide_footprint_accumulate).
+        if accumulate not in ("fixed", "sort"):
+            raise ValueError(f"accumulate must be 'fixed' or 'sort', not {accumulate!r}")
+        self.fixed_mode = accumulate == "fixed"
+        # fixed_copies: copies of the fixed-point accumulator the step's
+        # workgroups spread their atomics over (0: as many as fit in 512 MiB, at
+        # most 64; a cell crowded by a point release then takes 1/copies of the
+        # atomics per address); the cells are the sums over the copies
+        self.fixed_copies_req = int(fixed_copies)
+        # binned_keys (sort): footprint keys without the time bin (a per-release
+        # table carries it): shorter keys, fewer sort passes, the same sums bit
+        # for bit
+        self.binned_keys = bool(binned_keys) and not self.fixed_mode
+        # overlap (sort): each eager step's footprint accumulation runs on a
+        # second stream, concurrently with the next step's kernel (keys and
+        # values double-buffered; a buffer is reused only after its accumulation)
+        self.overlap = bool(overlap) and not self.fixed_mode
         # threads per workgroup of the step kernel (0 = the library's default);
         # a tuning parameter (glide_amd.autotune), results do not depend on it
         self.step_block = int(step_block)
@@ -196,7 +214,8 @@ class BatchEngine:
         # above its own terminus, as when it runs alone (its result does not
         # depend on the batch); default: the reference's batch terminus
         self.release_terminus = bool(release_terminus)
-        self.extra_flags = L.F_RELEASE_TERMINUS if self.release_terminus else 0
+        self.extra_flags = (L.F_RELEASE_TERMINUS if self.release_terminus else 0) | (
+            L.F_FIXED if self.fixed_mode else 0)
         self.use_graph = bool(use_graph)
         self._graphs: dict[tuple[int, int], int] = {}
         self.graph_launches = 0
@@ -277,9 +296,22 @@ class BatchEngine:
         self.acc_pending = [False, False]
         self.escaped = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)
         self.active_steps = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)  # per-step actives
-        self.footprint = torch.zeros(self.n_cells, dtype=f32, device=d)
-        self.comp = torch.zeros(self.n_cells, dtype=f32, device=d) if self.compensated else None
-        self.release_total = torch.zeros(self.grid_cfg["n_r"], dtype=torch.float64, device=d)
+        self._footprint = torch.zeros(self.n_cells, dtype=f32, device=d)
+        # Neumaier terms of the sort accumulation; the fixed-point sums are exact
+        self.comp = torch.zeros(self.n_cells, dtype=f32, device=d) if self.compensated and not self.fixed_mode else None
+        self._release_total = torch.zeros(self.grid_cfg["n_r"], dtype=torch.float64, device=d)
+        self._fixed_stale = False
+        if self.fixed_mode:
+            copies = self.fixed_copies_req
+            if copies <= 0:
+                copies = 1
+                while copies < 64 and 2 * copies * 8 * self.n_cells <= (512 << 20):
+                    copies *= 2
+            self.fixed_copies = copies
+            self.fixed = torch.zeros(copies * self.n_cells, dtype=torch.int64, device=d)
+            self.fixed_exp = _dev(self.fixed_exponents(), None, d)
+            self.fixed_ws = torch.empty(max(8, L.lib().glide_fixed_workspace_bytes(self.grid_cfg["n_r"])),
+                                        dtype=torch.uint8, device=d)
         ws = L.lib().glide_footprint_workspace_bytes(n)
         self.workspace = torch.empty(ws, dtype=torch.uint8, device=d)
         self.sort_ws = torch.empty(L.lib().glide_sort_workspace_bytes(n), dtype=torch.uint8, device=d)
@@ -336,6 +368,56 @@ class BatchEngine:
             return self.__dict__["bufs"][self.__dict__["cur"]][name]
         raise AttributeError(name)
 
+    def fixed_exponents(self) -> np.ndarray:
+        """Per-release power-of-two scale of the fixed-point footprint: release
+        r's residence is at most T_r = (sum of its weights) * (longest backward
+        length + 2 dt) (every particle is active over at most that span), and
+        e_r = 61 - ceil(log2 T_r) keeps T_r * 2^e_r <= 2^61, inside int64 with
+        room to spare; the rounding of one contribution, 2^-e_r, is then below
+        T_r * 2^-61."""
+
+        n_r = self.grid_cfg["n_r"]
+        w = np.bincount(np.clip(self.ra.rel_idx.astype(np.int64), 0, n_r - 1),
+                        weights=np.abs(self.ra.weight.astype(np.float64)), minlength=n_r)[:n_r]
+        bound = w * (float(self.length_s) + 2.0 * float(self.cfg["dt"]))
+        e = np.zeros(n_r, np.int32)
+        pos = bound > 0
+        e[pos] = 61 - np.ceil(np.log2(bound[pos])).astype(np.int32)
+        return e
+
+    def _finalize_fixed(self, stream: int) -> None:
+        """Fixed-point sums to the float32 footprint and float64 totals."""
+
+        L.check(L.lib().glide_fixed_finalize(
+            self.fixed.data_ptr(), self.fixed_copies, self.n_cells, self.cells_per_release, self.fixed_exp.data_ptr(),
+            self._footprint.data_ptr(), self._release_total.data_ptr(), self.fixed_ws.data_ptr(),
+            self.fixed_ws.numel(), stream))
+        self._fixed_stale = False
+
+    def fixed_cells(self) -> torch.Tensor:
+        """The fixed-point sums per cell (int64 [n_cells], summed over the
+        accumulator's copies): cell c holds its residence times 2^fixed_exp[r]."""
+
+        return self.fixed.view(self.fixed_copies, self.n_cells).sum(dim=0)
+
+    @property
+    def footprint(self) -> torch.Tensor:
+        """The footprint, float32 [n_r * n_t * n_z * n_y * n_x] on the device,
+        holding every step run so far (ordered before later work on the
+        current stream)."""
+
+        if self._fixed_stale:
+            self._finalize_fixed(L.stream_ptr())
+        return self._footprint
+
+    @property
+    def release_total(self) -> torch.Tensor:
+        """Per-release residence (float64 [n_r]) of the steps run so far."""
+
+        if self._fixed_stale:
+            self._finalize_fixed(L.stream_ptr())
+        return self._release_total
+
     def reset(self):
         """Back to the release state (positions at the sources, state zero,
         release order)."""
@@ -354,10 +436,13 @@ class BatchEngine:
         b["alive"].fill_(1)
         self.escaped.zero_()
         self.active_steps.zero_()
-        self.footprint.zero_()
+        self._footprint.zero_()
         if self.comp is not None:
             self.comp.zero_()
-        self.release_total.zero_()
+        self._release_total.zero_()
+        if self.fixed_mode:
+            self.fixed.zero_()
+            self._fixed_stale = False
         if self.diagnostics:
             self.pos_means.zero_()
         self.step_i = 0
@@ -413,7 +498,12 @@ class BatchEngine:
         keys, vals = self.emit_bufs[buf]
         return L.GlideEmit(keys=keys.data_ptr(), vals=vals.data_ptr(),
                            escaped=self.escaped.data_ptr() + 8 * st.index,
-                           active=self.active_steps.data_ptr() + 8 * st.index)
+                           active=self.active_steps.data_ptr() + 8 * st.index, **self._fixed_emit())
+
+    def _fixed_emit(self) -> dict:
+        if not self.fixed_mode:
+            return {}
+        return dict(fixed=self.fixed.data_ptr(), fixed_exp=self.fixed_exp.data_ptr(), fixed_copies=self.fixed_copies)
 
     def _stream_obj(self, ptr: int):
         s = self._ext.get(ptr)
@@ -424,11 +514,22 @@ class BatchEngine:
     def join(self, stream: int | None = None) -> None:
         """Order the work that follows on ``stream`` (default: the current
         stream) after every footprint accumulation still running on the
-        accumulation stream (overlap mode; a no-op otherwise)."""
+        accumulation stream (overlap mode), or convert the fixed-point sums to
+        the float footprint on ``stream`` (fixed mode)."""
+
+        ptr = L.stream_ptr() if stream is None else stream
+        if self.fixed_mode:
+            if self._fixed_stale:
+                self._finalize_fixed(ptr)
+            return
+        self._wait_acc(ptr)
+
+    def _wait_acc(self, ptr: int) -> None:
+        """Order ``ptr`` after the accumulations running on the accumulation stream."""
 
         if not self.overlap:
             return
-        main = self._stream_obj(L.stream_ptr() if stream is None else stream)
+        main = self._stream_obj(ptr)
         for b in (0, 1):
             if self.acc_pending[b]:
                 main.wait_event(self.ev_acc[b])
@@ -437,13 +538,16 @@ class BatchEngine:
     def _capture(self, stream: int) -> int:
         """Capture the plain step body (table-driven step, which also advances
         the device cursor, and the footprint accumulation) for the current
-        particle buffer."""
+        particle buffer. (A body whose accumulation of the previous step runs
+        on a parallel branch beside the step kernel replayed more than twice
+        as slowly as this serial one on MI355X, 0.35 against 0.15 ms per step:
+        the step kernel fills every CU, so the branches hardly overlap.)"""
 
         lib = L.lib()
         if stream == 0:
             raise ValueError("graph capture needs a non-default stream")
         em = L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(), escaped=self.escaped.data_ptr(),
-                         active=self.active_steps.data_ptr())
+                         active=self.active_steps.data_ptr(), **self._fixed_emit())
         L.check(lib.glide_graph_begin(stream))
         try:
             L.check(lib.glide_step_table(C.byref(self.step_tmpl), self.step_table.data_ptr(), self.row.data_ptr(),
@@ -459,6 +563,9 @@ class BatchEngine:
         return h.value
 
     def _accumulate(self, stream: int, buf: int = 0) -> None:
+        if self.fixed_mode:
+            self._fixed_stale = True  # the step kernels added to the fixed-point sums
+            return
         if self.n_live == 0:
             return
         keys, vals = self.emit_bufs[buf]
@@ -466,14 +573,14 @@ class BatchEngine:
             g = self.grid_cfg
             L.check(L.lib().glide_footprint_accumulate_binned(
                 keys.data_ptr(), vals.data_ptr(), self.n_live, g["n_r"], g["n_t"], g["n_z"] * g["n_y"] * g["n_x"],
-                self.t_bins[buf].data_ptr(), self.footprint.data_ptr(),
-                self.comp.data_ptr() if self.comp is not None else None, self.release_total.data_ptr(),
+                self.t_bins[buf].data_ptr(), self._footprint.data_ptr(),
+                self.comp.data_ptr() if self.comp is not None else None, self._release_total.data_ptr(),
                 self.workspace.data_ptr(), self.workspace.numel(), stream))
             return
         L.check(L.lib().glide_footprint_accumulate(
             keys.data_ptr(), vals.data_ptr(), self.n_live, self.n_cells, self.cells_per_release,
-            self.footprint.data_ptr(), self.comp.data_ptr() if self.comp is not None else None,
-            self.release_total.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(), stream))
+            self._footprint.data_ptr(), self.comp.data_ptr() if self.comp is not None else None,
+            self._release_total.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(), stream))
 
     def _position_means(self, stream: int, row: bool = False) -> None:
         """Mean lon/lat/z over all particles of the batch into pos_means[step]
@@ -541,6 +648,7 @@ class BatchEngine:
         """The footprint as (R, T, Z, Y, X) float32 on the host (the
         compensation terms added when the footprint is compensated)."""
 
+        self.join()  # after every step's accumulation (fixed mode: the float footprint made)
         torch.cuda.synchronize()
         g = self.grid_cfg
         f = self.footprint if self.comp is None else self.footprint + self.comp
@@ -596,6 +704,8 @@ class BatchEngine:
         if stream is None:
             stream = L.stream_ptr()
         st = self.sched[self.step_i]
+        if self.fixed_mode:
+            self._fixed_stale = True  # this step adds to the fixed-point sums
         new_bracket = st.bracket_hour != self.last_bracket
         if self.compact and new_bracket:
             self.reorder(st, stream, count_live=True)
@@ -608,7 +718,7 @@ class BatchEngine:
         convect = new_bracket and self.conv_opts is not None
         ev = self.step_events.get(self.step_i) if self.step_events is not None else None
         if self.use_graph and not convect and ev is None:
-            self.join(stream)  # the captured body emits into buffer 0 and accumulates in line
+            self._wait_acc(stream)  # the captured body emits into buffer 0 and accumulates in line
             key = (self.cur, self.n_live)
             g = self._graphs.get(key)
             if g is None:

@@ -19,6 +19,8 @@
  *                              (footprint_gridder.py:156-196)
  *   glide_footprint_accumulate FootprintGridder.accumulate scatter_add_
  *                              (footprint_gridder.py:197)
+ *   GLIDE_F_FIXED +            the same scatter_add_, fused into the step as
+ *   glide_fixed_finalize       an exact fixed-point sum (footprint_gridder.py:197)
  *   glide_resample_hour        ArcoEra5ZarrReader._resample_hour_to_agl
  *                              (met_reader.py:641-693; vertical_grid.py:166-316)
  *   glide_build_window         HannaScheme window stacks + raw met staging
@@ -43,7 +45,7 @@
 extern "C" {
 #endif
 
-#define GLIDE_ABI_VERSION 3
+#define GLIDE_ABI_VERSION 4
 
 #define GLIDE_OK 0
 #define GLIDE_ERR_INVALID (-1)
@@ -66,6 +68,10 @@ extern "C" {
                                          is stepped exactly as when it runs alone (no counterpart in the
                                          reference, whose batch terminus gives non-earliest releases one
                                          more step, main.py:1383) */
+#define GLIDE_F_FIXED 0x80         /* with GLIDE_F_EMIT: add each particle's residence to a
+                                      fixed-point accumulator in the step (GlideEmit.fixed) instead of
+                                      writing keys/values; replaces the reference's scatter_add_
+                                      (footprint_gridder.py:197) with an order-free exact sum */
 
 /*
  * Met grid and the per-level AGL ladder (metadata.lon/lat/level,
@@ -172,12 +178,31 @@ typedef struct GlideStepParams {
  * value (weight * dt), and the step's escape count (added to *escaped).
  * active (may be NULL): the step's count of active particles (its
  * particle-steps) is added to *active. In table mode (glide_step_table) both
- * counters point at per-row arrays indexed by the cursor. */
+ * counters point at per-row arrays indexed by the cursor.
+ *
+ * GLIDE_F_FIXED (flat keys only: GlideFootprintGrid.t_bin NULL): keys/vals
+ * are not written (may be NULL); instead each particle's value v (the float
+ * weight * dt) becomes the integer rint(v * 2^fixed_exp[r]) of its release r
+ * and is added to fixed[key] (int64[n_cells], two's complement in 64-bit
+ * words) by integer atomics after a per-wavefront sum over runs of equal
+ * keys. Integer sums do not depend on order, so the result is bit-identical
+ * whatever the launch shape or particle order; with fixed_exp chosen so every
+ * release's whole residence stays below 2^62 (glide_fixed_finalize), the
+ * integers hold each float contribution exactly in practice. fixed_copies
+ * (>= 1; 0 = 1) copies of the n_cells accumulator lie back to back in fixed,
+ * and workgroup b adds to copy b % fixed_copies: particles gathered in a few
+ * cells (a point release) then spread their atomics over that many addresses
+ * per cell instead of queueing on one. The cells' values are the sums over
+ * the copies. */
 typedef struct GlideEmit {
     uint32_t* keys;
     float* vals;
     unsigned long long* escaped;
     unsigned long long* active;
+    long long* fixed;
+    const int32_t* fixed_exp;
+    int32_t fixed_copies;
+    int32_t reserved;
 } GlideEmit;
 
 int glide_abi_version(void);
@@ -260,6 +285,19 @@ int glide_footprint_accumulate_binned(const uint32_t* keys, const float* vals, i
                                       uint32_t n_t, uint32_t cells_per_bin, const int32_t* t_bin, float* slab,
                                       float* comp, double* release_total, void* workspace,
                                       size_t workspace_bytes, void* stream);
+
+/* Fixed-point footprint (GLIDE_F_FIXED) to float: slab[c] = (the sum over
+ * the copies of fixed[k * n_cells + c]) * 2^-e rounded once to float32 (the correctly rounded exact sum of the float
+ * contributions when each was held exactly), e = fixed_exp[c /
+ * cells_per_release]; release_total[r] (may be NULL) = the exact integer sum
+ * of release r's cells * 2^-e in float64 (overwritten, not added). The
+ * exponent of a release whose residence is bounded by T (sum of its weights
+ * times its backward length plus two steps) is 61 - ceil(log2 T). Workspace:
+ * glide_fixed_workspace_bytes(n_r). */
+size_t glide_fixed_workspace_bytes(int32_t n_r);
+int glide_fixed_finalize(const long long* fixed, int32_t copies, int64_t n_cells, uint32_t cells_per_release,
+                         const int32_t* fixed_exp, float* slab, double* release_total, void* workspace,
+                         size_t workspace_bytes, void* stream);
 
 /* Workspace bytes glide_sort_pairs needs for n pairs. */
 size_t glide_sort_workspace_bytes(int64_t n);

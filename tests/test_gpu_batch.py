@@ -12,7 +12,12 @@ m (some of them straddle the 100 m top of the output layer), the totals then
 differ by up to 5e-4 (measured; mixed signs), while the single 10k-particle
 release of the plain run sits at 1e-5 against its 1e-4 bar. Bars here: 1e-3
 per release, and 1e-4 on the mean signed error over the releases (no bias;
-measured 1.2e-5)."""
+measured 1.2e-5).
+
+Every engine test runs with both footprint accumulations: "fixed" (exact
+fixed-point sums inside the step kernel) and "sort" (radix sort plus
+fixed-order float reduction); the overlap and binned-key options belong to the
+sort accumulation."""
 
 from __future__ import annotations
 
@@ -35,18 +40,27 @@ BIAS_BAR = 1e-4
 
 
 @pytest.fixture(scope="module")
-def batch():
+def cfg_met():
     cfg = workload_config("batched-cpu-6r")
-    met = SyntheticMet(cfg["spec"])
-    eng = BatchEngine(cfg, device="cuda", met=met, compact=True)
+    return cfg, SyntheticMet(cfg["spec"])
+
+
+@pytest.fixture(scope="module", params=["fixed", "sort"])
+def batch(request, cfg_met):
+    cfg, met = cfg_met
+    eng = BatchEngine(cfg, device="cuda", met=met, compact=True, accumulate=request.param)
     eng.run()
     torch.cuda.synchronize()
     return cfg, met, eng
 
 
+def _mode(eng) -> str:
+    return "fixed" if eng.fixed_mode else "sort"
+
+
 @pytest.fixture(scope="module")
-def oracle_batch(batch):
-    cfg, met, _ = batch
+def oracle_batch(cfg_met):
+    cfg, met = cfg_met
     return ORUN.run(cfg, dty=np.float64, met=met)
 
 
@@ -83,8 +97,8 @@ def test_release_alone_equals_release_in_batch(batch):
     differ only in float32 summation order inside a cell. (Under the
     reference's batch terminus a release other than the earliest-ending one is
     stepped once more, main.py:1383.)"""
-    cfg, met, _ = batch
-    eng = BatchEngine(cfg, device="cuda", met=met, compact=True, release_terminus=True)
+    cfg, met, b = batch
+    eng = BatchEngine(cfg, device="cuda", met=met, compact=True, release_terminus=True, accumulate=_mode(b))
     eng.run()
     S = eng.state_numpy()
     fp = _slabs(eng, cfg)
@@ -94,7 +108,7 @@ def test_release_alone_equals_release_in_batch(batch):
         one = dataclasses.replace(rel, release_idx=0, noise_id=rel.release_idx)
         g1 = dict(cfg["grid"], n_r=1)
         c1 = dict(cfg, releases=[one], grid=g1)
-        e1 = BatchEngine(c1, device="cuda", met=met)
+        e1 = BatchEngine(c1, device="cuda", met=met, accumulate=_mode(b))
         e1.run()
         s1 = e1.state_numpy()
         sel = np.isin(S["uid"], s1["uid"])
@@ -110,7 +124,7 @@ def test_release_alone_equals_release_in_batch(batch):
 
 def test_compaction_leaves_results_unchanged(batch):
     cfg, met, eng = batch
-    plain = BatchEngine(cfg, device="cuda", met=met, compact=False)
+    plain = BatchEngine(cfg, device="cuda", met=met, compact=False, accumulate=_mode(eng))
     plain.run()
     a, b = plain.state_numpy(), eng.state_numpy()
     for name in STATE:
@@ -127,13 +141,16 @@ def test_network_batch_bf16_met_within_bar(batch, oracle_batch):
     oracle on float32 met: footprint within 1 % relative L2 per release (the
     batched method's bar); totals reported (bfloat16 rounding of the winds is a
     small systematic change of the transport, not summation noise)."""
-    cfg, met, _ = batch
+    cfg, met, b = batch
     from glide_amd import _lib as L
 
-    eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=L.GLIDE_MET_BF16, compensated=True, compact=True)
+    eng = BatchEngine(cfg, device="cuda", met=met, met_dtype=L.GLIDE_MET_BF16, compensated=True, compact=True,
+                      accumulate=_mode(b))
     eng.run()
     ref = oracle_batch
-    fp = _slabs(eng, cfg) + eng.comp.cpu().numpy().astype(np.float64).reshape(cfg["grid"]["n_r"], -1)
+    fp = _slabs(eng, cfg)
+    if eng.comp is not None:  # Neumaier terms of the sort accumulation
+        fp = fp + eng.comp.cpu().numpy().astype(np.float64).reshape(cfg["grid"]["n_r"], -1)
     fr = ref.footprint.reshape(fp.shape)
     tot = eng.release_total.cpu().numpy()
     l2 = [np.linalg.norm(fp[r] - fr[r]) / np.linalg.norm(fr[r]) for r in range(fp.shape[0])]
@@ -151,9 +168,12 @@ def test_overlapped_accumulation_is_bit_identical(batch, use_graph):
     counts, bit for bit (each accumulation still sums its own step's keys in
     the same fixed order; only the streams differ)."""
     cfg, met, eng = batch
+    if eng.fixed_mode:
+        pytest.skip("overlap is an option of the sort accumulation")
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
-        ov = BatchEngine(cfg, device="cuda", met=met, compact=True, overlap=True, use_graph=use_graph)
+        ov = BatchEngine(cfg, device="cuda", met=met, compact=True, overlap=True, use_graph=use_graph,
+                         accumulate="sort")
         ov.run()
     torch.cuda.synchronize()
     a, b = eng.state_numpy(), ov.state_numpy()
@@ -171,7 +191,9 @@ def test_binned_keys_equal_flat_keys(batch):
     the same order as the flat cell keys, so the footprint, compensation and
     totals are bit-identical to the flat-key run."""
     cfg, met, eng = batch
-    flat = BatchEngine(cfg, device="cuda", met=met, compact=True, binned_keys=False)
+    if eng.fixed_mode:
+        pytest.skip("binned keys are an option of the sort accumulation")
+    flat = BatchEngine(cfg, device="cuda", met=met, compact=True, binned_keys=False, accumulate="sort")
     flat.run()
     torch.cuda.synchronize()
     assert eng.binned_keys and not flat.binned_keys

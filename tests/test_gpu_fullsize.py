@@ -7,9 +7,12 @@ not need the oracle (which cannot run 2^20 particles in test time):
   bit-identical; every step's active count equals the schedule's window count;
   the float32 footprint sums (in float64) to the float64 release total within
   float32 rounding (mass conservation of the accumulation);
-* batched bfloat16 workload (336 releases, 2.4 M particles, Neumaier
-  compensation): per-release slab sums plus compensation equal the float64
-  release totals, and compaction on/off gives the same particles."""
+* batched bfloat16 workload (336 releases, 2.4 M particles): per-release slab
+  sums equal the float64 release totals, and compaction on/off gives the same
+  particles.
+
+The engines use the default fixed-point accumulation; test_gpu_fixed.py pins it
+bit for bit against exact integer sums at small size."""
 
 from __future__ import annotations
 
@@ -94,7 +97,10 @@ def test_batched_full_size_totals_and_compaction():
     for k in STATE:
         assert np.array_equal(sa[k], sb[k]), k
     R = cfg["grid"]["n_r"]
-    slabs = (a.footprint.cpu().numpy().astype(np.float64) + a.comp.cpu().numpy().astype(np.float64)).reshape(R, -1)
+    slabs = a.footprint.cpu().numpy().astype(np.float64)
+    if a.comp is not None:  # Neumaier terms of the sort accumulation
+        slabs = slabs + a.comp.cpu().numpy().astype(np.float64)
+    slabs = slabs.reshape(R, -1)
     tot = a.release_total.cpu().numpy()
     live = tot > 0
     assert live.sum() > 10
