@@ -29,7 +29,7 @@ from pathlib import Path
 import torch
 
 DEFAULT_CANDIDATES = [
-    dict(step_block=sb, reorder_every=ro) for sb in (256, 512) for ro in (0, 8, 16, 32)
+    dict(step_block=sb, reorder_every=ro) for sb in (256, 512) for ro in (0, 8, 16, 32, 64)
 ]
 
 

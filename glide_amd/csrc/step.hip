@@ -75,7 +75,7 @@ struct StepCtx {
 // Advances particle i; returns whether it was active. With GLIDE_F_FIXED the
 // footprint key and fixed-point value go to fkey/fval (left as the caller set
 // them, the dropped key and 0, for an inactive particle) instead of memory.
-template <bool BF16>
+template <bool BF16, bool FIXED>
 __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* lev, int64_t i,
                                               const GlideStepParams& p, const StepCtx& x, uint32_t& fkey,
                                               long long& fval) {
@@ -85,7 +85,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     const uint32_t ip = static_cast<uint32_t>(i * a.s.pos_stride) * 4u;
     const GlideParticles& s = a.s;
     const bool emit = (p.flags & GLIDE_F_EMIT) != 0;
-    const bool fixed = (p.flags & GLIDE_F_FIXED) != 0;
+    constexpr bool fixed = FIXED;
     const float co = static_cast<float>(p.cursor_off);
 
     bool active;
@@ -343,7 +343,10 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
     return xcd * q + (xcd < r ? xcd : r) + idx;
 }
 
-template <bool BF16, int MODE>
+// FIXED: the fixed-point footprint (GLIDE_F_FIXED) as its own instantiation,
+// so neither variant keeps the other's pointers live (the kernel is short of
+// scalar registers; the spare ones spill to vector lanes).
+template <bool BF16, int MODE, bool FIXED>
 __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
     __shared__ float lev[kMaxLevels];
     for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
@@ -396,8 +399,8 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
     bool act = false;
     uint32_t fkey = a.fp.n_keys;
     long long fval = 0;
-    if (run) act = step_particle<BF16>(a, lev, i, p, x, fkey, fval);
-    if ((p.flags & (GLIDE_F_EMIT | GLIDE_F_FIXED)) == (GLIDE_F_EMIT | GLIDE_F_FIXED))
+    if (run) act = step_particle<BF16, FIXED>(a, lev, i, p, x, fkey, fval);
+    if (FIXED && (p.flags & GLIDE_F_EMIT))
         wave_fixed_add(fkey, fval, a.fp.n_keys, fixed_copy(a.e, a.fp.n_cells));  // every lane is here
     if constexpr (MODE == kModeSlots) {
         if (n_active != nullptr) count_flags(act, n_active);  // per wavefront: its slot's counter
@@ -538,15 +541,26 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
     hipStream_t st = static_cast<hipStream_t>(stream);
     const dim3 grid(static_cast<unsigned>(blocks)), block(static_cast<unsigned>(bs));
     const bool bf16 = w->dtype == GLIDE_MET_BF16;
+    const bool fixed = (p->flags & (GLIDE_F_EMIT | GLIDE_F_FIXED)) == (GLIDE_F_EMIT | GLIDE_F_FIXED);
     if (table != nullptr) {
-        if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeTable>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((step_kernel<false, kModeTable>), grid, block, 0, st, a);
+        if (fixed) {
+            if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeTable, true>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((step_kernel<false, kModeTable, true>), grid, block, 0, st, a);
+        } else {
+            if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeTable, false>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((step_kernel<false, kModeTable, false>), grid, block, 0, st, a);
+        }
     } else if (slots != nullptr) {
-        if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeSlots>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((step_kernel<false, kModeSlots>), grid, block, 0, st, a);
+        if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeSlots, false>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((step_kernel<false, kModeSlots, false>), grid, block, 0, st, a);
     } else {
-        if (bf16) hipLaunchKernelGGL((step_kernel<true, kModePlain>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((step_kernel<false, kModePlain>), grid, block, 0, st, a);
+        if (fixed) {
+            if (bf16) hipLaunchKernelGGL((step_kernel<true, kModePlain, true>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((step_kernel<false, kModePlain, true>), grid, block, 0, st, a);
+        } else {
+            if (bf16) hipLaunchKernelGGL((step_kernel<true, kModePlain, false>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((step_kernel<false, kModePlain, false>), grid, block, 0, st, a);
+        }
     }
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
