@@ -8,8 +8,9 @@ Two parameters change speed and nothing else:
 * ``reorder_every``: every how many steps the particles are re-binned by met
   cell (sorted by cell, state permuted), so that the 64 lanes of a wavefront
   gather from few cache lines and each XCD's workgroups (which take a
-  contiguous share of the particles) gather from one region of the grid;
-  0 = only the hourly compaction.
+  contiguous share of the particles) gather from one region of the grid, and
+  (fixed-point accumulation) a wavefront's particles share few footprint cells,
+  so it issues few atomics; 0 = only the hourly compaction.
 
 Results do not depend on either (particles are independent, noise is keyed by
 particle id, counts are integers), so the tuner only times. It runs each
@@ -41,7 +42,8 @@ def _signature(cfg: dict, engine_kw: dict) -> str:
     spec, g = cfg["spec"], cfg["grid"]
     n = sum(int(r.n_particles) for r in cfg["releases"])
     dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
-    fixed = {k: engine_kw[k] for k in sorted(engine_kw) if k in ("met_dtype", "compensated", "compact", "overlap")}
+    fixed = {k: engine_kw[k] for k in sorted(engine_kw) if k in ("met_dtype", "compensated", "compact", "overlap",
+                                                                   "accumulate")}
     return json.dumps([dev, spec.nx, spec.ny, spec.za, n, len(cfg["releases"]), g["n_t"], fixed], sort_keys=True)
 
 
