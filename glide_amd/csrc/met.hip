@@ -208,17 +208,30 @@ __device__ float local_std(const float* __restrict__ s, const float* __restrict_
     return sqrtf(ss / static_cast<float>(cnt));
 }
 
-// One thread per (column, level): blockIdx.y is the level. Level 0's threads
-// also write the surface record.
+// Levels per workgroup of the window build: one 128 B run of records per
+// column, so the column-major record writes go out in whole cache lines.
+template <bool BF16>
+constexpr int window_levels() { return 128 / static_cast<int>(Rec8<BF16>::kBytes); }
+
+// One thread per (column, level); a workgroup takes kColBlock / kTz
+// consecutive columns (the fastest index, so the field reads coalesce) times
+// kTz consecutive levels (blockIdx.y). The records are staged in LDS and
+// written column by column. Level 0's threads also write the surface record.
 template <bool BF16>
 __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
+    constexpr int kTz = window_levels<BF16>();
+    constexpr int kTc = kColBlock / kTz;
+    constexpr int kUnits = static_cast<int>(Rec8<BF16>::kBytes) / 16;  // 16 B units per record
+    __shared__ uint4 stage[2][kColBlock * kUnits];  // wind, aux records [column][level]
     const int64_t ncol = static_cast<int64_t>(a.ny) * a.nx;
-    const int64_t col = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (col >= ncol) return;
-    const int y = static_cast<int>(col / a.nx);
-    const int x = static_cast<int>(col % a.nx);
+    const int tc = static_cast<int>(threadIdx.x) % kTc;
+    const int tz = static_cast<int>(threadIdx.x) / kTc;
+    const int64_t col = static_cast<int64_t>(blockIdx.x) * kTc + tc;
     const int nz = a.nz;
-    const int z = blockIdx.y;
+    const int z = static_cast<int>(blockIdx.y) * kTz + tz;
+    const bool ok = col < ncol && z < nz;
+    const int y = static_cast<int>((ok ? col : 0) / a.nx);
+    const int x = static_cast<int>((ok ? col : 0) % a.nx);
     auto at = [&](const float* f, int ch, int zz) {
         return f[(static_cast<int64_t>(ch) * nz + zz) * ncol + col];
     };
@@ -233,7 +246,7 @@ __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
         const float tm = 0.5f * (at(a.s3, 3, zz) + at(a.e3, 3, zz));
         return pres(zz) / (287.05f * fmaxf(tm, 1.0f));
     };
-    {
+    if (ok) {
         const int zl = z == 0 ? 0 : z - 1;
         const int zh = z == nz - 1 ? nz - 1 : z + 1;
         const float dh = safe_den(a.lev[zh] - a.lev[zl]);
@@ -267,12 +280,23 @@ __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
         float wr[8] = {at(a.s3, 0, z), at(a.s3, 1, z), at(a.s3, 2, z), at(a.e3, 0, z),
                        at(a.e3, 1, z), at(a.e3, 2, z), 0.0f, 0.0f};
         float xr[8] = {sw, tl, rho(z), drho, smu, smv, 0.0f, 0.0f};
-        // column-major records: a column's levels are contiguous
-        const int64_t rec = col * static_cast<int64_t>(nz) + z;
-        Rec8<BF16>::store(a.wind, rec, wr);
-        Rec8<BF16>::store(a.aux, rec, xr);
+        Rec8<BF16>::store(stage[0], tc * kTz + tz, wr);
+        Rec8<BF16>::store(stage[1], tc * kTz + tz, xr);
     }
-    if (z != 0) return;
+    __syncthreads();
+    // column-major records: a column's levels are contiguous, so each column's
+    // kTz staged records are one run of 128 B in the window
+    for (int q = static_cast<int>(threadIdx.x); q < kColBlock * kUnits; q += kColBlock) {
+        const int rl = q / kUnits;
+        const int64_t c = static_cast<int64_t>(blockIdx.x) * kTc + rl / kTz;
+        const int zz = static_cast<int>(blockIdx.y) * kTz + rl % kTz;
+        if (c < ncol && zz < nz) {
+            const int64_t g = (c * nz + zz) * kUnits + q % kUnits;
+            static_cast<uint4*>(a.wind)[g] = stage[0][q];
+            static_cast<uint4*>(a.aux)[g] = stage[1][q];
+        }
+    }
+    if (!ok || z != 0) return;
     // Surface record: blh sp ustar shf t(level 0) at start then end.
     float sr[12];
     for (int c = 0; c < 4; ++c) {
@@ -346,10 +370,12 @@ extern "C" int glide_build_window(const float* h3_s, const float* h3_e, const fl
     WindowArgs a{h3_s, h3_e, h2_s, h2_e, p_level_hpa, levels, nz, ny, nx, meander_radius,
                  static_cast<float>(meander_coeff), wind, aux, surf};
     const int64_t ncol = static_cast<int64_t>(ny) * nx;
-    const unsigned nblk = static_cast<unsigned>((ncol + kColBlock - 1) / kColBlock);
+    const bool bf16 = dtype == GLIDE_MET_BF16;
+    const int tz = bf16 ? window_levels<true>() : window_levels<false>();
+    const int tc = kColBlock / tz;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const dim3 grid(nblk, static_cast<unsigned>(nz));
-    if (dtype == GLIDE_MET_BF16) {
+    const dim3 grid(static_cast<unsigned>((ncol + tc - 1) / tc), static_cast<unsigned>((nz + tz - 1) / tz));
+    if (bf16) {
         hipLaunchKernelGGL(window_kernel<true>, grid, dim3(kColBlock), 0, st, a);
     } else {
         hipLaunchKernelGGL(window_kernel<false>, grid, dim3(kColBlock), 0, st, a);
