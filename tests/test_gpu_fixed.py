@@ -119,3 +119,43 @@ def test_fixed_exponents_bound_the_release_residence(setup):
         assert 0 <= tot < 2**61, (r, tot)
         assert tot > 2**40, (r, tot)  # the scale leaves the sums far above the rounding unit
     assert np.all(e > 0)
+
+
+def test_fixed_finalize_rounds_correctly_through_the_abi():
+    """glide_fixed_finalize on chosen int64 cells: the copies are summed
+    exactly, each cell is rounded once to the nearest float32 (ties to even:
+    2^25 + 1 -> 2^25, 2^25 + 3 -> 2^25 + 4) and scaled by 2^-e of its
+    release, and the totals are the exact integer sums in float64. Checked
+    bit for bit against numpy's correctly rounded int64 conversions."""
+    from glide_amd import _lib as L
+
+    rng = np.random.default_rng(5)
+    n_r, cpr, copies = 3, 4099, 3
+    n_cells = n_r * cpr
+    cells = rng.integers(-(2**60), 2**60, size=(copies, n_cells), dtype=np.int64)
+    ties = np.array([2**25 + 1, 2**25 + 3, 2**40 + 2**16, -(2**25 + 1), 0, 1, 2**61 - 1], np.int64)
+    cells[:, : ties.size] = 0
+    cells[0, : ties.size] = ties
+    e = np.array([44, -3, 61], np.int32)
+    fixed = torch.from_numpy(cells.reshape(-1)).cuda()
+    exp = torch.from_numpy(e).cuda()
+    slab = torch.empty(n_cells, dtype=torch.float32, device="cuda")
+    tot = torch.empty(n_r, dtype=torch.float64, device="cuda")
+    ws = torch.empty(max(8, L.lib().glide_fixed_workspace_bytes(n_r)), dtype=torch.uint8, device="cuda")
+    L.check(L.lib().glide_fixed_finalize(fixed.data_ptr(), copies, n_cells, cpr, exp.data_ptr(), slab.data_ptr(),
+                                         tot.data_ptr(), ws.data_ptr(), ws.numel(), L.stream_ptr()))
+    torch.cuda.synchronize()
+    s = cells.sum(axis=0)  # |values| < 2^62: no wrap
+    want = np.ldexp(s.astype(np.float32), -np.repeat(e, cpr)).astype(np.float32)
+    assert np.float32(2**25 + 1) == np.float32(2**25) and np.float32(2**25 + 3) == np.float32(2**25 + 4)
+    assert np.array_equal(slab.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    want_t = [np.ldexp(float(int(x)), -int(k)) for x, k in zip([sum(int(v) for v in s[r * cpr:(r + 1) * cpr])
+                                                                for r in range(n_r)], e)]
+    assert np.array_equal(tot.cpu().numpy(), np.array(want_t))
+    # argument checks: copies out of range, cells_per_release not dividing
+    with pytest.raises(ValueError):
+        L.check(L.lib().glide_fixed_finalize(fixed.data_ptr(), 2000, n_cells, cpr, exp.data_ptr(), slab.data_ptr(),
+                                             None, None, 0, L.stream_ptr()))
+    with pytest.raises(ValueError):
+        L.check(L.lib().glide_fixed_finalize(fixed.data_ptr(), copies, n_cells, cpr + 1, exp.data_ptr(),
+                                             slab.data_ptr(), None, None, 0, L.stream_ptr()))
