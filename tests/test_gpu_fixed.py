@@ -132,7 +132,9 @@ def test_fixed_finalize_rounds_correctly_through_the_abi():
     rng = np.random.default_rng(5)
     n_r, cpr, copies = 3, 4099, 3
     n_cells = n_r * cpr
-    cells = rng.integers(-(2**60), 2**60, size=(copies, n_cells), dtype=np.int64)
+    # a release's cells sum below 2^62 (the exponent rule's promise), so the
+    # integer totals cannot wrap; the chosen values reach 2^61
+    cells = rng.integers(-(2**48), 2**48, size=(copies, n_cells), dtype=np.int64)
     ties = np.array([2**25 + 1, 2**25 + 3, 2**40 + 2**16, -(2**25 + 1), 0, 1, 2**61 - 1], np.int64)
     cells[:, : ties.size] = 0
     cells[0, : ties.size] = ties
@@ -146,6 +148,7 @@ def test_fixed_finalize_rounds_correctly_through_the_abi():
                                          tot.data_ptr(), ws.data_ptr(), ws.numel(), L.stream_ptr()))
     torch.cuda.synchronize()
     s = cells.sum(axis=0)  # |values| < 2^62: no wrap
+    assert all(abs(sum(int(v) for v in s[r * cpr:(r + 1) * cpr])) < 2**62 for r in range(n_r))
     want = np.ldexp(s.astype(np.float32), -np.repeat(e, cpr)).astype(np.float32)
     assert np.float32(2**25 + 1) == np.float32(2**25) and np.float32(2**25 + 3) == np.float32(2**25 + 4)
     assert np.array_equal(slab.cpu().numpy().view(np.uint32), want.view(np.uint32))
