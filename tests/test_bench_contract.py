@@ -1,0 +1,34 @@
+"""CPU: the parts of bench.py's contract that need no GPU: more than one GPU
+exits with a message (this engine runs on one GPU), and the flags the
+measurement documents exist with their defaults (fixed-point accumulation,
+re-binning every 32 steps, event samples every 7th step)."""
+
+from __future__ import annotations
+
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_documented_defaults():
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    old = sys.argv
+    try:
+        sys.argv = ["bench.py"]
+        a = bench.parse()
+    finally:
+        sys.argv = old
+    assert a.gpus == 1 and a.steps == 1440 and a.accumulate == "fixed"
+    assert a.reorder_every == 32 and a.time_every == 7
+    assert bench.BYTES_PER_PARTICLE_STEP == {"fixed": 82, "sort": 90}
