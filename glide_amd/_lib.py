@@ -136,7 +136,8 @@ SIGNATURES = {
                                              C.c_size_t, vp]),
     "glide_sort_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "glide_sort_pairs": (C.c_int, [vp, vp, C.c_int64, C.c_int32, vp, vp, vp, C.c_size_t, vp]),
-    "glide_reorder_keys": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), C.c_double, vp, vp, vp, vp]),
+    "glide_reorder_keys": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), C.c_double, C.c_int32,
+                                     vp, vp, vp, vp]),
     "glide_permute_particles": (C.c_int, [P(GlideParticles), P(GlideParticles), vp, vp]),
     "glide_resample_hour": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp, C.c_int32,
                                       vp, vp, C.c_double, C.c_double, vp, C.c_int32, C.c_int32, C.c_int32,

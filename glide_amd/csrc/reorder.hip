@@ -10,6 +10,12 @@
 // moves most particles only a short way. Particles that can never be active
 // again (escaped, or past their backward window) sort to the end.
 //
+// With several releases (n_releases > 1) the key is release-major: a
+// release's particles stay together, sorted by cell, so a wavefront's
+// particles share a release and a few cells and the fused fixed-point
+// footprint sums long runs of one key per atomic (releases interleaved in a
+// column would change the key from lane to lane).
+//
 // Results do not depend on the order: noise is keyed by the particle id, which
 // moves with the particle.
 #include "sort.h"
@@ -25,6 +31,7 @@ struct ReorderKeyArgs {
     float horizon;  // particles with r_off < horizon become active only after it: second bucket
     bool terminus;  // GLIDE_F_RELEASE_TERMINUS: lo_off holds the release terminus
     uint32_t ncell, sentinel;
+    uint32_t groups;  // release groups of the key (1: by cell only)
     uint32_t* keys;
     uint32_t* idx;
     uint32_t* live;  // optional count of particles that can still become active
@@ -52,7 +59,13 @@ __device__ __forceinline__ bool key_one(const ReorderKeyArgs& a, const float* le
         const int ix = static_cast<int>(frac_index(s.lon[i * ps], a.lon0, a.sx, a.nx));
         const int iy = static_cast<int>(frac_index(s.lat[i * ps], a.lat0, a.sy, a.ny));
         const int iz = static_cast<int>(level_index(lev, a.nz, s.z[i * ps]));
-        key = static_cast<uint32_t>((static_cast<int64_t>(iy) * a.nx + ix) * a.nz + iz) + (later ? a.ncell : 0u);
+        uint32_t grp = 0;
+        if (a.groups > 1) {
+            const int32_t r = s.rel_idx[i];
+            grp = static_cast<uint32_t>(r < 0 ? 0 : (r >= static_cast<int32_t>(a.groups) ? a.groups - 1 : r));
+        }
+        const uint32_t cell = static_cast<uint32_t>((static_cast<int64_t>(iy) * a.nx + ix) * a.nz + iz);
+        key = ((later ? a.groups : 0u) + grp) * a.ncell + cell;
     }
     a.keys[i] = key;
     a.idx[i] = static_cast<uint32_t>(i);
@@ -118,8 +131,8 @@ __global__ __launch_bounds__(256) void permute_kernel(const PermuteArgs a) {
 using namespace glide;
 
 extern "C" int glide_reorder_keys(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
-                                  double horizon_off, uint32_t* keys, uint32_t* idx, uint32_t* live,
-                                  void* stream) {
+                                  double horizon_off, int32_t n_releases, uint32_t* keys, uint32_t* idx,
+                                  uint32_t* live, void* stream) {
     GLIDE_REQUIRE(p && s && g && keys && idx, "glide_reorder_keys: NULL argument");
     GLIDE_REQUIRE(s->lon && s->lat && s->z && s->alive && s->r_off, "particle arrays are NULL");
     GLIDE_REQUIRE(s->pos_stride >= 1 && s->n >= 0, "bad particles");
@@ -141,9 +154,12 @@ extern "C" int glide_reorder_keys(const GlideStepParams* p, const GlideParticles
     a.terminus = (p->flags & GLIDE_F_RELEASE_TERMINUS) != 0;
     GLIDE_REQUIRE(!a.terminus || s->lo_off, "GLIDE_F_RELEASE_TERMINUS needs lo_off");
     const int64_t ncell = static_cast<int64_t>(g->nz) * g->ny * g->nx;
-    GLIDE_REQUIRE(2 * ncell < 0xffffffffll, "met grid too large for the reorder keys");
+    const int64_t groups = n_releases > 1 ? n_releases : 1;
+    GLIDE_REQUIRE(groups == 1 || s->rel_idx, "release-major keys need rel_idx");
+    GLIDE_REQUIRE(2 * groups * ncell < 0xffffffffll, "met grid x releases too large for the reorder keys");
     a.ncell = static_cast<uint32_t>(ncell);
-    a.sentinel = static_cast<uint32_t>(2 * ncell);
+    a.groups = static_cast<uint32_t>(groups);
+    a.sentinel = static_cast<uint32_t>(2 * groups * ncell);
     a.horizon = static_cast<float>(horizon_off);
     a.keys = keys;
     a.idx = idx;

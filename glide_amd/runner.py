@@ -173,7 +173,7 @@ class BatchEngine:
                  device_met: DeviceMet | None = None, reorder_every: int = 0, use_graph: bool = False,
                  compact: bool = False, release_terminus: bool = False, diagnostics: bool = False,
                  ingest=None, step_block: int = 0, overlap: bool = False, binned_keys: bool = True,
-                 accumulate: str = "fixed", fixed_copies: int = 0):
+                 accumulate: str = "fixed", fixed_copies: int = 0, bin_by_release: bool | None = None):
         self.cfg = cfg
         # accumulate: how each step's residence reaches the footprint.
         #   "fixed": inside the step kernel, as exact 64-bit fixed-point integer
@@ -191,6 +191,11 @@ class BatchEngine:
         # most 64; a cell crowded by a point release then takes 1/copies of the
         # atomics per address); the cells are the sums over the copies
         self.fixed_copies_req = int(fixed_copies)
+        # bin_by_release: the re-binning keeps each release's particles
+        # together (release-major keys), so a wavefront's particles share a
+        # release and its fixed-point atomics sum long runs; None: with the
+        # fixed-point accumulation and more than one release
+        self.bin_by_release = bin_by_release
         # binned_keys (sort): footprint keys without the time bin (a per-release
         # table carries it): shorter keys, fewer sort passes, the same sums bit
         # for bit
@@ -320,7 +325,11 @@ class BatchEngine:
         self.ro_keys_s = torch.empty(n, dtype=torch.int32, device=d)
         self.ro_idx_s = torch.empty(n, dtype=torch.int32, device=d)
         ncell = self.met.nz * self.met.ny * self.met.nx
-        self.ro_bits = max(1, int(2 * ncell).bit_length())  # cells, cells of the later bucket, the sentinel
+        by_rel = self.bin_by_release if self.bin_by_release is not None else (
+            self.fixed_mode and self.grid_cfg["n_r"] > 1)
+        self.bin_groups = int(self.grid_cfg["n_r"]) if by_rel else 1
+        # cells (per release group), the later bucket's, the sentinel
+        self.ro_bits = max(1, int(2 * self.bin_groups * ncell).bit_length())
         nc = self.met.ny * self.met.nx
         self.i_lcl = torch.empty(nc, dtype=torch.int32, device=d)
         self.i_lnb = torch.empty(nc, dtype=torch.int32, device=d)
@@ -467,7 +476,7 @@ class BatchEngine:
             self.live.zero_()
             self.horizon = float(np.float32(self.sched[self._bracket_end[st.index]].cursor_off))
         L.check(lib.glide_reorder_keys(C.byref(p), C.byref(self._parts[self.cur]), C.byref(self.met.grid),
-                                       self.horizon, self.ro_keys.data_ptr(), self.ro_idx.data_ptr(),
+                                       self.horizon, self.bin_groups, self.ro_keys.data_ptr(), self.ro_idx.data_ptr(),
                                        self.live.data_ptr() if count_live else None, stream))
         # the hourly compaction sorts everything; a re-binning within the hour
         # sorts only the live prefix (what the steps launch over) and carries

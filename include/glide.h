@@ -314,17 +314,20 @@ int glide_sort_pairs(const uint32_t* keys, const uint32_t* vals, int64_t n, int3
  * Particle reordering for gather locality and compaction of finished
  * particles (no counterpart in the reference, which keeps particles in release
  * order): keys[i] = met cell of particle i, column-major ((row * nx + column)
- * * nz + level); plus ncell = nz*ny*nx for a particle released further back
- * than horizon_off (r_off < horizon_off: not active before the cursor passes
- * it); 2*ncell for a particle that can never be active again (escaped, or its
- * earliest active offset lo_off (r_off - length_s) is after cursor_off);
- * idx[i] = i. When live is not NULL, the number of particles that can be
+ * * nz + level); with n_releases > 1, plus rel_idx[i] * ncell (release-major:
+ * a release's particles together, by cell; rel_idx clamped to the range), G =
+ * n_releases groups, else G = 1; plus G*ncell (ncell = nz*ny*nx) for a particle
+ * released further back than horizon_off (r_off < horizon_off: not active
+ * before the cursor passes it); 2*G*ncell for a particle that can never be
+ * active again (escaped, or its earliest active offset lo_off (r_off -
+ * length_s) is after cursor_off); idx[i] = i. When live is not NULL, the number of particles that can be
  * active before the horizon is added to *live (zero it first). Sort with
  * glide_sort_pairs, then permute: those particles come first, then the ones
  * released further back. horizon_off below every r_off keeps one bucket.
  */
 int glide_reorder_keys(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
-                       double horizon_off, uint32_t* keys, uint32_t* idx, uint32_t* live, void* stream);
+                       double horizon_off, int32_t n_releases, uint32_t* keys, uint32_t* idx, uint32_t* live,
+                       void* stream);
 
 /* dst.array[j] = src.array[idx[j]] for every non-NULL array of the particle
  * state (structure of arrays, pos_stride 1, src and dst distinct). */

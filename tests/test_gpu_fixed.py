@@ -85,6 +85,33 @@ def test_fixed_cells_equal_exact_integer_sums(setup):
     np.testing.assert_allclose(fs, got, rtol=2e-5, atol=0)
 
 
+@pytest.mark.parametrize("n", [1, 63, 65, 1000])
+def test_fixed_cells_ragged_particle_counts(n):
+    """Particle counts that leave a wavefront partly empty (1, 63, 65) or
+    several whole: the lanes past n and the inactive particles add nothing,
+    and the cells equal the exact integer sums of the sort path's keys and
+    values, step by step (first 40 steps of the 3 h plain workload)."""
+    import dataclasses
+
+    base = workload_config("plain-cpu-10k-3h")
+    rel = dataclasses.replace(base["releases"][0], n_particles=n)
+    cfg = dict(base, releases=[rel])
+    met = SyntheticMet(cfg["spec"])
+    srt = BatchEngine(cfg, device="cuda", met=met, accumulate="sort", binned_keys=False)
+    fix = BatchEngine(cfg, device="cuda", met=met, device_met=srt.met)
+    e = fix.fixed_exp.cpu().numpy()
+    acc = np.zeros(fix.n_cells, np.int64)
+    for _ in range(min(40, len(fix.sched))):
+        srt.run_step()
+        fix.run_step()
+        torch.cuda.synchronize()
+        k, q = _exact_contributions(srt.keys[:n].cpu().numpy().view(np.uint32), srt.vals[:n].cpu().numpy(), e,
+                                    fix.cells_per_release, fix.n_cells)
+        np.add.at(acc, k, q)
+        assert np.array_equal(fix.fixed_cells().cpu().numpy(), acc)
+    assert acc.sum() > 0
+
+
 def test_fixed_footprint_invariant_to_launch_and_order(setup):
     cfg, met = setup
     kw = dict(device="cuda", met=met, compact=True, convection=ConvectionOptions())
@@ -95,7 +122,7 @@ def test_fixed_footprint_invariant_to_launch_and_order(setup):
     tot = ref.release_total.cpu().numpy()
     s = torch.cuda.Stream()
     for extra in (dict(step_block=64, fixed_copies=1), dict(step_block=512, reorder_every=4, fixed_copies=3),
-                  dict(use_graph=True, reorder_every=16)):
+                  dict(use_graph=True, reorder_every=16), dict(reorder_every=8, bin_by_release=False)):
         with torch.cuda.stream(s):
             eng = BatchEngine(cfg, device_met=ref.met, **kw, **extra)
             eng.run()
