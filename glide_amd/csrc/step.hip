@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
             }
         }
         if (!stepping) {
-            if (run && (p.flags & GLIDE_F_EMIT)) {
+            if (!FIXED && run && (p.flags & GLIDE_F_EMIT)) {
                 const uint32_t i4 = static_cast<uint32_t>(i) * 4u;
                 sto(a.e.keys, i4, a.fp.n_keys);
                 sto(a.e.vals, i4, 0.0f);
@@ -522,7 +522,6 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
             GLIDE_REQUIRE(fg && e && e->fixed && e->fixed_exp && e->escaped, "fixed-point emit outputs are NULL");
             GLIDE_REQUIRE(fg->t_bin == nullptr, "GLIDE_F_FIXED takes flat keys (t_bin NULL)");
             GLIDE_REQUIRE(e->fixed_copies >= 0 && e->fixed_copies <= 1024, "fixed_copies must be in [0, 1024]");
-            GLIDE_REQUIRE(slots == nullptr, "served slots emit keys (no GLIDE_F_FIXED)");
         } else {
             GLIDE_REQUIRE(fg && e && e->keys && e->vals && e->escaped, "emit outputs are NULL");
         }
@@ -551,8 +550,13 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
             else hipLaunchKernelGGL((step_kernel<false, kModeTable, false>), grid, block, 0, st, a);
         }
     } else if (slots != nullptr) {
-        if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeSlots, false>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((step_kernel<false, kModeSlots, false>), grid, block, 0, st, a);
+        if (fixed) {
+            if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeSlots, true>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((step_kernel<false, kModeSlots, true>), grid, block, 0, st, a);
+        } else {
+            if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeSlots, false>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((step_kernel<false, kModeSlots, false>), grid, block, 0, st, a);
+        }
     } else {
         if (fixed) {
             if (bf16) hipLaunchKernelGGL((step_kernel<true, kModePlain, true>), grid, block, 0, st, a);

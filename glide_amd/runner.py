@@ -62,6 +62,20 @@ class ConvectionOptions:
     parcel_lift: bool = False
 
 
+def fixed_exponent(weight_sum: np.ndarray | float, length_s: float, dt: float) -> np.ndarray:
+    """Power-of-two scale of a release's fixed-point footprint: its residence
+    is at most T = weight_sum * (length_s + 2 dt) (every particle is active over
+    at most that span), and e = 61 - ceil(log2 T) keeps T * 2^e <= 2^61, inside
+    int64 with room to spare; one contribution then rounds by 2^-e, below
+    T * 2^-61. T = 0 gives 0."""
+
+    bound = np.abs(np.asarray(weight_sum, np.float64)) * (float(length_s) + 2.0 * float(dt))
+    e = np.zeros(bound.shape, np.int32)
+    pos = bound > 0
+    e[pos] = 61 - np.ceil(np.log2(bound[pos])).astype(np.int32)
+    return e
+
+
 def _dev(a, dtype=None, device="cuda"):
     t = torch.from_numpy(np.ascontiguousarray(a))
     if dtype is not None:
@@ -388,11 +402,7 @@ class BatchEngine:
         n_r = self.grid_cfg["n_r"]
         w = np.bincount(np.clip(self.ra.rel_idx.astype(np.int64), 0, n_r - 1),
                         weights=np.abs(self.ra.weight.astype(np.float64)), minlength=n_r)[:n_r]
-        bound = w * (float(self.length_s) + 2.0 * float(self.cfg["dt"]))
-        e = np.zeros(n_r, np.int32)
-        pos = bound > 0
-        e[pos] = 61 - np.ceil(np.log2(bound[pos])).astype(np.int32)
-        return e
+        return fixed_exponent(w, self.length_s, self.cfg["dt"])
 
     def _finalize_fixed(self, stream: int) -> None:
         """Fixed-point sums to the float32 footprint and float64 totals."""

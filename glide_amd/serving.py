@@ -33,7 +33,7 @@ import numpy as np
 import torch
 
 from glide_amd import _lib as L
-from glide_amd.runner import DeviceMet, HannaOptions
+from glide_amd.runner import DeviceMet, HannaOptions, fixed_exponent
 from glide_amd.schedule import release_offsets
 from glide_amd.synthetic import SyntheticMet
 
@@ -219,8 +219,14 @@ class Server:
     def __init__(self, ring: MetRing, grid: dict, domain: dict, *, max_slots: int, pool_particles: int,
                  n_t_max: int, seed: int, dt: int = 60, hanna: HannaOptions | None = None, device="cuda",
                  stream: int | None = None, poll_every: int = 8, pinned_slabs: int = 16,
-                 keep_state: bool = False, writer: ResultWriter | None = None):
+                 keep_state: bool = False, writer: ResultWriter | None = None, accumulate: str = "fixed"):
         self.ring = ring
+        # accumulate: "fixed" adds each step's residence to exact int64 sums in
+        # the step kernel (a finished slot's slab is converted to float when it
+        # is collected); "sort" writes keys and values and sorts them by cell
+        if accumulate not in ("fixed", "sort"):
+            raise ValueError(f"accumulate must be 'fixed' or 'sort', not {accumulate!r}")
+        self.fixed_mode = accumulate == "fixed"
         self.grid = dict(grid)
         self.writer = writer  # finished footprints go to Zarr stores before they count as done
         self.dt = int(dt)
@@ -244,10 +250,10 @@ class Server:
             n_y=grid["n_y"], n_x=grid["n_x"], reserved=0,
             dom_lon_min=domain["lon_min"], dom_lon_max=domain["lon_max"], dom_lat_min=domain["lat_min"],
             dom_lat_max=domain["lat_max"], dom_alt_max=domain["alt_max"])
-        # release-binned keys: a slot's time bin is one per step (its particles
-        # share the request's window end), so keys are slot x cell
+        # release-binned keys (sort): a slot's time bin is one per step (its
+        # particles share the request's window end), so keys are slot x cell
         self.t_bin = torch.full((self.max_slots,), -1, dtype=torch.int32, device=d)
-        self.fg.t_bin = self.t_bin.data_ptr()
+        self.fg.t_bin = None if self.fixed_mode else self.t_bin.data_ptr()
         self.cpb = self.n_z * grid["n_y"] * grid["n_x"]
         self.cpr = self.n_t_max * self.cpb
         self.n_cells = self.max_slots * self.cpr
@@ -255,6 +261,12 @@ class Server:
             raise MemoryError("the slot slabs exceed 32-bit cell keys; lower max_slots or n_t_max")
         self.footprint = torch.zeros(self.n_cells, dtype=torch.float32, device=d)
         self.release_total = torch.zeros(self.max_slots, dtype=torch.float64, device=d)
+        if self.fixed_mode:
+            # one int64 accumulator per slot slab and the slot's scale exponent
+            # (set at admission); copies = 1: served releases are small
+            self.fixed = torch.zeros(self.n_cells, dtype=torch.int64, device=d)
+            self.fixed_exp = torch.zeros(self.max_slots, dtype=torch.int32, device=d)
+            self.fixed_ws = torch.empty(max(8, L.lib().glide_fixed_workspace_bytes(1)), dtype=torch.uint8, device=d)
         # particle pool (structure of arrays)
         f32 = torch.float32
         types = dict(rel_idx=torch.int32, uid=torch.int32, alive=torch.uint8)
@@ -284,7 +296,7 @@ class Server:
             dt=float(self.dt), alpha=0.0, cursor_off=0.0, length_s=0.0, substep_c=self.h.substep_c,
             z_ubl_m=self.h.z_ubl_m, meander_timescale_s=self.h.meander_timescale_s, seed=self.seed,
             step_index=0, max_substeps=self.h.max_substeps,
-            flags=self.h.flags() | L.F_EMIT | L.F_RELEASE_TERMINUS, reserved=0)
+            flags=self.h.flags() | L.F_EMIT | L.F_RELEASE_TERMINUS | (L.F_FIXED if self.fixed_mode else 0))
         self.step_count = 0
         self.particle_steps = 0
         self.completed: list[Result] = []  # finished requests not yet handed out
@@ -376,6 +388,10 @@ class Server:
         self.slots[slot * self.slot_bytes:(slot + 1) * self.slot_bytes].copy_(put(raw), non_blocking=True)
         self.wave_slot_h[b0:b0 + nb] = slot
         self.wave_slot[b0:b0 + nb].view(torch.uint8).copy_(put(self.wave_slot_h[b0:b0 + nb]), non_blocking=True)
+        if self.fixed_mode:
+            # the slot's scale: n particles of weight float32(1/n), whose sum n * w is exact in float64
+            e = fixed_exponent(n * float(np.float32(1.0 / n)), req.length_s, self.dt).reshape(1)
+            self.fixed_exp[slot:slot + 1].view(torch.uint8).copy_(put(e), non_blocking=True)
         self.staging.done(j)
         self.running[slot] = _Running(req, slot, b0, nb, now_s if now_s is not None else self.now())
         return True
@@ -396,11 +412,14 @@ class Server:
                 n=n, pos_stride=1, **{k: self.P[k].data_ptr() for k in (
                     "lon", "lat", "z", "up", "vp", "wp", "um", "vm", "weight", "r_off", "we_off", "rel_idx",
                     "uid", "alive", "lo_off")}, active=None)
+            fx = dict(fixed=self.fixed.data_ptr(), fixed_exp=self.fixed_exp.data_ptr(), fixed_copies=1) \
+                if self.fixed_mode else {}
             em = L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(),
-                             escaped=self.dummy_escaped.data_ptr(), active=None)
+                             escaped=self.dummy_escaped.data_ptr(), active=None, **fx)
             L.check(lib.glide_step_slots(C.byref(self.tmpl), self.slots.data_ptr(), self.wave_slot.data_ptr(),
                                          C.byref(parts), C.byref(self.ring.grid), C.byref(self.ring.desc),
                                          C.byref(self.fg), C.byref(em), st))
+        if n > 0 and not self.fixed_mode:
             L.check(lib.glide_footprint_accumulate_binned(
                 self.keys.data_ptr(), self.vals.data_ptr(), n, self.max_slots, self.n_t_max, self.cpb,
                 self.t_bin.data_ptr(), self.footprint.data_ptr(), None, self.release_total.data_ptr(),
@@ -465,6 +484,12 @@ class Server:
         n_t = int(math.ceil(req.length_s / 3600.0))
         host = self.pinned.pop() if self.pinned else torch.empty(self.cpr, dtype=torch.float32).pin_memory()
         lo = slot * self.cpr
+        if self.fixed_mode:
+            # the slot's exact sums to its float slab and total, then clear them
+            L.check(L.lib().glide_fixed_finalize(
+                self.fixed.data_ptr() + 8 * lo, 1, self.cpr, self.cpr, self.fixed_exp.data_ptr() + 4 * slot,
+                self.footprint.data_ptr() + 4 * lo, self.release_total.data_ptr() + 8 * slot,
+                self.fixed_ws.data_ptr(), self.fixed_ws.numel(), self.stream))
         host.copy_(self.footprint[lo:lo + self.cpr], non_blocking=True)
         total = torch.empty(1, dtype=torch.float64).pin_memory()
         total.copy_(self.release_total[slot:slot + 1], non_blocking=True)
@@ -473,6 +498,8 @@ class Server:
         # reuse: the copies above precede these writes on the stream
         self.footprint[lo:lo + self.cpr].zero_()
         self.release_total[slot].zero_()
+        if self.fixed_mode:
+            self.fixed[lo:lo + self.cpr].zero_()
         self.slots[slot * self.slot_bytes:(slot + 1) * self.slot_bytes].zero_()
         self.wave_slot_h[run.block0:run.block0 + run.n_blocks] = -1
         self.wave_slot[run.block0:run.block0 + run.n_blocks].fill_(-1)

@@ -473,7 +473,9 @@ int glide_slots_advance(GlideSlot* slots, int32_t n_slots, double dt, void* stre
  * slot and its met window from the ring (the slot's bracket); flags, seed and
  * the Hanna options from tmpl. Footprint keys use the slot as the release
  * index (fg->n_r >= number of slots); escape and active counts go to the
- * slot. Waves of non-stepping slots only write dropped keys. */
+ * slot. Waves of non-stepping slots only write dropped keys. With
+ * GLIDE_F_FIXED the residence goes to e->fixed as in glide_step, with
+ * e->fixed_exp indexed by slot. */
 int glide_step_slots(const GlideStepParams* tmpl, GlideSlot* slots, const int32_t* wave_slot,
                      const GlideParticles* s, const GlideGrid* g, const GlideMetRing* ring,
                      const GlideFootprintGrid* fg, const GlideEmit* e, void* stream);

@@ -2,8 +2,11 @@
 stepped together, each on its own cursor, give the same result as when each
 runs alone, and the same as the single-release engine (pinned to the oracle
 by tests/test_gpu_parity.py) on the same release: every particle
-bit-identical, footprints within float32 summation-order rounding (2e-5),
-residence totals within 1e-9."""
+bit-identical; with the default fixed-point footprint (exact integer sums)
+the footprints and totals of a request alone and in a mix are bit-identical
+too, and match the engine's within float32 rounding (2e-5); the sort
+accumulation's float32 sums match within summation-order rounding (2e-5),
+totals within 1e-9."""
 
 from __future__ import annotations
 
@@ -84,6 +87,8 @@ def test_request_alone_equals_request_in_mix(setup, mixed):
         np.testing.assert_allclose(mix.footprint, alone.footprint, rtol=2e-5, atol=0)
         assert abs(mix.total - alone.total) <= 1e-9 * alone.total
         assert alone.total > 0
+        if srv.fixed_mode:  # exact sums: the order of the atomics cannot show
+            assert np.array_equal(mix.footprint, alone.footprint) and mix.total == alone.total
 
 
 def test_served_request_equals_single_release_engine(setup, mixed):
@@ -141,3 +146,23 @@ def test_finished_footprints_stream_to_zarr(setup, mixed, tmp_path):
         np.testing.assert_allclose(res.footprint, mixed[rid].footprint, rtol=2e-5, atol=0)
         assert st.read_coord("release_lon")[0] == pytest.approx(res.req.lon)
         assert np.isfinite(res.latency_s)
+
+
+def test_sort_accumulation_server_matches_fixed(setup, mixed):
+    """A server with the sort accumulation (keys and values, radix sort,
+    fixed-order float32 sums) gives the same particles and the fixed-point
+    server's footprint within float32 summation rounding."""
+    cfg, met, ring, reqs = setup
+    srv = Server(ring, cfg["grid"], cfg["domain"], max_slots=8, pool_particles=8192, n_t_max=3, seed=SEED,
+                 keep_state=True, accumulate="sort")
+    assert not srv.fixed_mode
+    req = reqs[0]
+    assert srv.admit(req)
+    (res,) = srv.drain()
+    ref = mixed[req.req_id]
+    a, b = _by_uid(res.state), _by_uid(ref.state)
+    for k in STATE:
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(res.footprint != 0, ref.footprint != 0)
+    np.testing.assert_allclose(res.footprint, ref.footprint, rtol=2e-5, atol=0)
+    assert abs(res.total - ref.total) <= 1e-9 * ref.total
