@@ -3,10 +3,12 @@
 Mirrors the reference's static per-step body (main.py:1383-1477, S2 order)
 with everything on the GPU: the met source hours stay in HBM, the hourly
 resample and window build run as column kernels on a bracket change, and each
-step is one fused step kernel (which also emits the footprint keys and applies
-the escape kill) followed by the deterministic footprint accumulation.
+step is one fused step kernel, which also adds every particle's residence to
+the footprint (exact fixed-point sums; or, with the sort accumulation, writes
+keys and values that a deterministic sort-and-reduce then accumulates) and
+applies the escape kill.
 Convection runs once per bracket, between the step and the gridding, as in the
-reference; on that step the convection kernel emits the keys instead.
+reference; on that step the convection kernel emits the residence instead.
 """
 
 from __future__ import annotations

@@ -6,8 +6,10 @@ with its own cursor; every engine step advances all running slots by one step
 of their own schedules (``glide_slots_advance``), runs the fused step kernel
 over the shared particle pool with each wavefront reading its slot's scalars
 and its bracket's window from a device-resident met ring
-(``glide_step_slots``), and accumulates all footprints deterministically into
-per-slot slabs (``glide_footprint_accumulate``). A request is stepped exactly
+(``glide_step_slots``), which adds every footprint to its slot's exact
+fixed-point sums (converted to float per slot when the request finishes,
+``glide_fixed_finalize``; or, with ``accumulate="sort"``, keys and values
+sorted into per-slot slabs by ``glide_footprint_accumulate_binned``). A request is stepped exactly
 as it would be alone (its own terminus, offsets relative to its own release
 time, noise keyed by its id and absolute time), so its result does not depend
 on what else is running.
