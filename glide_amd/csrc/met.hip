@@ -92,8 +92,17 @@ __global__ __launch_bounds__(kColBlock) void resample_kernel(const ResampleArgs 
 
     double pval = 0.0;
     if (live) {
-        int k0 = 0;
-        for (int k = 0; k < zp; ++k) k0 += (hasc(k) < 0.0) ? 1 : 0;
+        const double tg = a.agl[j];
+        // k0 = number of sub-surface levels; idx = searchsorted(h, tg,
+        // side="left"), which on the ascending heights is the number of levels
+        // below tg: one pass of independent loads (a scan that stops at the
+        // first level >= tg makes every load wait for the one before)
+        int k0 = 0, idx = 0;
+        for (int k = 0; k < zp; ++k) {
+            const double hk = hasc(k);
+            k0 += (hk < 0.0) ? 1 : 0;
+            idx += (hk < tg) ? 1 : 0;
+        }
         const int y = static_cast<int>(col / a.nx);
         const int x = static_cast<int>(col % a.nx);
         const float* tc = a.terrain + col;
@@ -103,9 +112,6 @@ __global__ __launch_bounds__(kColBlock) void resample_kernel(const ResampleArgs 
         const float dhdy = dlat_g / static_cast<float>(kMPerDegLatD);  // float32, as numpy keeps it
         const double ztop = a.agl[a.za - 1];
         const double ztop_c = ztop > 1.0 ? ztop : 1.0;
-        const double tg = a.agl[j];
-        int idx = 0;  // searchsorted(h, tg, side="left")
-        while (idx < zp && hasc(idx) < tg) ++idx;
         int lower = idx - 1;
         if (lower < 0) lower = 0;
         if (lower < k0) lower = k0;
