@@ -197,7 +197,7 @@ def serving_leg(args, plain_value: float) -> dict:
         "zarr_stores": len(writer.paths),
         "zarr_bytes": sum(f.stat().st_size for p in writer.paths for f in Path(p).rglob("*") if f.is_file()),
         "slots": srv.max_slots,
-        "vs_plain": value / plain_value,
+        "vs_plain": value / plain_value if plain_value > 0 else None,
         "mean_particles": float(np.mean([r.n_particles for r in reqs])) if reqs else 0.0,
     }
     writer.close()
@@ -278,7 +278,7 @@ def batched_leg(args, sptr: int, plain_value: float) -> dict:
                       + ("from exact int64 fixed-point sums" if args.accumulate == "fixed"
                          else "+ Neumaier compensation")),
         "release_totals": "f64",
-        "vs_plain": value / plain_value,
+        "vs_plain": value / plain_value if plain_value > 0 else None,
         "autotune": {"chosen": tuned["best"], "trials": tuned["trials"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 5),
@@ -364,7 +364,7 @@ def ingest_leg(args, sptr: int, plain_value: float) -> dict:
         "value": psteps / elapsed,
         "unit": "particle-steps/s",
         "seconds": elapsed,
-        "vs_resident": psteps / elapsed / plain_value,
+        "vs_resident": psteps / elapsed / plain_value if plain_value > 0 else None,
         "workers": args.ingest_workers,
         "hours_decoded": st["decoded"],
         "decode_cpu_s": st["decode_s"],
@@ -518,11 +518,12 @@ def main() -> int:
     torch.cuda.empty_cache()
     if not args.no_plain_torch:
         out["plain_torch"] = plain_torch_leg(args, sptr)
-        out["vs_plain_torch"] = out["value"] / out["plain_torch"]["value"]
+        pt = out["plain_torch"]["value"]
+        out["vs_plain_torch"] = out["value"] / pt if pt > 0 else None
     if not args.no_batched:
         out["batched"] = batched_leg(args, sptr, out["value"])
         if "plain_torch" in out:
-            out["batched"]["vs_plain_torch"] = out["batched"]["value"] / out["plain_torch"]["value"]
+            out["batched"]["vs_plain_torch"] = out["batched"]["value"] / pt if pt > 0 else None
     if not args.no_serving:
         out["serving"] = serving_leg(args, out["value"])
     if not args.no_ingest:
