@@ -103,7 +103,7 @@ class GlideMetRing(C.Structure):
 
 class GlideEmit(C.Structure):
     _fields_ = [("keys", vp), ("vals", vp), ("escaped", vp), ("active", vp), ("fixed", vp), ("fixed_exp", vp),
-                ("fixed_copies", C.c_int32), ("reserved", C.c_int32)]
+                ("fixed_copies", C.c_int32), ("active_copies", C.c_int32)]
 
 
 P = C.POINTER

@@ -326,6 +326,11 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     return true;
 }
 
+// Counters the step's active count is spread over (GlideEmit.active_copies).
+__device__ __forceinline__ uint32_t active_copies(const GlideEmit& e) {
+    return e.active_copies > 1 ? static_cast<uint32_t>(e.active_copies) : 1u;
+}
+
 // Launch modes, fixed at compile time so each instantiation keeps only its
 // own scalars live (the kernel is otherwise short of scalar registers).
 constexpr int kModePlain = 0;  // scalars from the launch
@@ -362,7 +367,7 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
         int r = 0;
         p = load_row(a.table, a.row, r);
         x.escaped += r;
-        if (n_active != nullptr) n_active += r;
+        if (n_active != nullptr) n_active += static_cast<int64_t>(r) * active_copies(a.e);
     } else if constexpr (MODE == kModeSlots) {
         // Served slots: one slot per wavefront (64-particle blocks).
         const int32_t slot = __builtin_amdgcn_readfirstlane(run ? a.wave_slot[i >> 6] : -1);
@@ -413,7 +418,7 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
         if (threadIdx.x == 0) {
             uint32_t c = 0;
             for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) c += wact[w];
-            if (c) atomicAdd(n_active, static_cast<unsigned long long>(c));
+            if (c) atomicAdd(n_active + blockIdx.x % active_copies(a.e), static_cast<unsigned long long>(c));
         }
     }
     if constexpr (MODE == kModeTable) {
@@ -512,6 +517,10 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
     a.table = table;
     a.row = row;
     a.n_active = e ? e->active : nullptr;
+    if (e != nullptr) {
+        GLIDE_REQUIRE(e->active_copies >= 0 && e->active_copies <= 1024, "active_copies must be in [0, 1024]");
+        a.e.active_copies = e->active_copies;  // read by the step kernel with or without GLIDE_F_EMIT
+    }
     a.slots = slots;
     a.wave_slot = wave_slot;
     if (ring != nullptr) a.ring = *ring;

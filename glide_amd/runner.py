@@ -64,6 +64,10 @@ class ConvectionOptions:
     parcel_lift: bool = False
 
 
+# counters each step's active count is spread over (GlideEmit.active_copies)
+ACTIVE_COPIES = 64
+
+
 def fixed_exponent(weight_sum: np.ndarray | float, length_s: float, dt: float) -> np.ndarray:
     """Power-of-two scale of a release's fixed-point footprint: its residence
     is at most T = weight_sum * (length_s + 2 dt) (every particle is active over
@@ -316,7 +320,9 @@ class BatchEngine:
             self.ev_acc = [torch.cuda.Event(), torch.cuda.Event()]
         self.acc_pending = [False, False]
         self.escaped = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)
-        self.active_steps = torch.zeros(max(len(self.sched), 1), dtype=torch.int64, device=d)  # per-step actives
+        # per-step active counts, each spread over ACTIVE_COPIES counters (one
+        # atomic per workgroup on a single counter would queue); read as sums
+        self._active = torch.zeros((max(len(self.sched), 1), ACTIVE_COPIES), dtype=torch.int64, device=d)
         self._footprint = torch.zeros(self.n_cells, dtype=f32, device=d)
         # Neumaier terms of the sort accumulation; the fixed-point sums are exact
         self.comp = torch.zeros(self.n_cells, dtype=f32, device=d) if self.compensated and not self.fixed_mode else None
@@ -415,6 +421,13 @@ class BatchEngine:
             self.fixed_ws.numel(), stream))
         self._fixed_stale = False
 
+    @property
+    def active_steps(self) -> torch.Tensor:
+        """Active particles per schedule step (int64 [steps]): the step's
+        particle-steps, counted by the step kernel."""
+
+        return self._active.sum(dim=1)
+
     def fixed_cells(self) -> torch.Tensor:
         """The fixed-point sums per cell (int64 [n_cells], summed over the
         accumulator's copies): cell c holds its residence times 2^fixed_exp[r]."""
@@ -456,7 +469,7 @@ class BatchEngine:
             b[k].zero_()
         b["alive"].fill_(1)
         self.escaped.zero_()
-        self.active_steps.zero_()
+        self._active.zero_()
         self._footprint.zero_()
         if self.comp is not None:
             self.comp.zero_()
@@ -519,7 +532,8 @@ class BatchEngine:
         keys, vals = self.emit_bufs[buf]
         return L.GlideEmit(keys=keys.data_ptr(), vals=vals.data_ptr(),
                            escaped=self.escaped.data_ptr() + 8 * st.index,
-                           active=self.active_steps.data_ptr() + 8 * st.index, **self._fixed_emit())
+                           active=self._active.data_ptr() + 8 * ACTIVE_COPIES * st.index,
+                           active_copies=ACTIVE_COPIES, **self._fixed_emit())
 
     def _fixed_emit(self) -> dict:
         if not self.fixed_mode:
@@ -568,7 +582,7 @@ class BatchEngine:
         if stream == 0:
             raise ValueError("graph capture needs a non-default stream")
         em = L.GlideEmit(keys=self.keys.data_ptr(), vals=self.vals.data_ptr(), escaped=self.escaped.data_ptr(),
-                         active=self.active_steps.data_ptr(), **self._fixed_emit())
+                         active=self._active.data_ptr(), active_copies=ACTIVE_COPIES, **self._fixed_emit())
         L.check(lib.glide_graph_begin(stream))
         try:
             L.check(lib.glide_step_table(C.byref(self.step_tmpl), self.step_table.data_ptr(), self.row.data_ptr(),

@@ -193,7 +193,13 @@ typedef struct GlideStepParams {
  * and workgroup b adds to copy b % fixed_copies: particles gathered in a few
  * cells (a point release) then spread their atomics over that many addresses
  * per cell instead of queueing on one. The cells' values are the sums over
- * the copies. */
+ * the copies.
+ *
+ * active_copies (>= 1; 0 = 1): the step's active count is spread the same
+ * way, workgroup b adding to active[b % active_copies] (table mode: the row's
+ * active_copies counters start at active + row * active_copies); the count is
+ * the sum of those counters. One counter per step takes one atomic from every
+ * workgroup, which queue on its address (≈10 ns each). */
 typedef struct GlideEmit {
     uint32_t* keys;
     float* vals;
@@ -202,7 +208,7 @@ typedef struct GlideEmit {
     long long* fixed;
     const int32_t* fixed_exp;
     int32_t fixed_copies;
-    int32_t reserved;
+    int32_t active_copies;
 } GlideEmit;
 
 int glide_abi_version(void);

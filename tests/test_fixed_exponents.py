@@ -43,3 +43,4 @@ def test_fixed_exponent_of_a_unit_release():
     e = BatchEngine.fixed_exponents(fake)
     assert e.tolist() == [44]
     assert np.float64(np.float32(60 * 2.0**-20)) * 2.0**44 == 60 * 2**24
+
