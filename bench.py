@@ -56,6 +56,9 @@ def parse():
                          "choice on this workload with the fixed-point accumulation: binned particles gather "
                          "from fewer cache lines and sum longer runs per atomic, and the gain decays as they "
                          "move)")
+    ap.add_argument("--step-block", type=int, default=128,
+                    help="threads per workgroup of the step launch (64..512; 128 is the autotuner's choice on "
+                         "this workload)")
     ap.add_argument("--graph", action="store_true",
                     help="time the captured step body (graph replay) as the headline run instead of "
                          "eager launches (both are always measured)")
@@ -405,7 +408,7 @@ def main() -> int:
     if args.autotune:
         tuned = tune(cfg, met, use_cache=False, **kw)
     else:
-        tuned = {"best": {"reorder_every": args.reorder_every}, "trials": []}
+        tuned = {"best": {"reorder_every": args.reorder_every, "step_block": args.step_block}, "trials": []}
     eng = BatchEngine(cfg, device="cuda", met=met, use_graph=True, **kw, **tuned["best"])
     n_sched = len(eng.sched)
     K = max(1, min(args.steps, n_sched))

@@ -30,5 +30,5 @@ def test_documented_defaults():
     finally:
         sys.argv = old
     assert a.gpus == 1 and a.steps == 1440 and a.accumulate == "fixed"
-    assert a.reorder_every == 32 and a.time_every == 7
+    assert a.reorder_every == 32 and a.time_every == 7 and a.step_block == 128
     assert bench.BYTES_PER_PARTICLE_STEP == {"fixed": 82, "sort": 90}
