@@ -72,24 +72,26 @@ __device__ __forceinline__ bool key_one(const ReorderKeyArgs& a, const float* le
     return !done && !later;
 }
 
-__global__ __launch_bounds__(256) void reorder_key_kernel(const ReorderKeyArgs a) {
+constexpr int kKeyThreads = 1024;
+constexpr int kKeyMaxBlocks = 512;  // bounded grid: few live-count atomics on the one counter
+
+// Grid-stride over the particles. The live count goes through LDS and one
+// integer atomic per workgroup: atomics from every workgroup of a launch over
+// all particles queue on the single counter's address (≈10 ns each).
+__global__ __launch_bounds__(kKeyThreads) void reorder_key_kernel(const ReorderKeyArgs a) {
     __shared__ float lev[kMaxLev];
-    __shared__ uint32_t wlive[4];
+    __shared__ uint32_t blive;
     for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
-    if (threadIdx.x < 4) wlive[threadIdx.x] = 0;
+    if (threadIdx.x == 0) blive = 0;
     __syncthreads();
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    bool counted = false;
-    if (i < a.s.n) counted = key_one(a, lev, i);
+    uint32_t c = 0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.s.n; i += stride)
+        c += key_one(a, lev, i) ? 1u : 0u;
     if (a.live != nullptr) {
-        // wave ballots summed in LDS, one integer atomic per workgroup
-        const unsigned long long m = __ballot(counted);
-        if ((threadIdx.x & 63) == 0) wlive[threadIdx.x >> 6] = static_cast<uint32_t>(__popcll(m));
+        if (c) atomicAdd(&blive, c);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint32_t c = wlive[0] + wlive[1] + wlive[2] + wlive[3];
-            if (c) atomicAdd(a.live, c);
-        }
+        if (threadIdx.x == 0 && blive) atomicAdd(a.live, blive);
     }
 }
 
@@ -164,7 +166,9 @@ extern "C" int glide_reorder_keys(const GlideStepParams* p, const GlideParticles
     a.keys = keys;
     a.idx = idx;
     a.live = live;
-    hipLaunchKernelGGL(reorder_key_kernel, dim3(static_cast<unsigned>((s->n + 255) / 256)), dim3(256), 0,
+    int64_t blocks = (s->n + kKeyThreads - 1) / kKeyThreads;
+    if (blocks > kKeyMaxBlocks) blocks = kKeyMaxBlocks;
+    hipLaunchKernelGGL(reorder_key_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kKeyThreads), 0,
                        static_cast<hipStream_t>(stream), a);
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
