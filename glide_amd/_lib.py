@@ -146,7 +146,7 @@ SIGNATURES = {
                                      C.c_int32, C.c_int32, vp, vp, vp, vp]),
     "glide_convect_columns": (C.c_int, [vp, vp, vp, C.c_double, vp, vp, C.c_int32, C.c_int32, C.c_int32,
                                         C.c_double, C.c_double, C.c_double, C.c_double, vp, vp, vp, vp, vp]),
-    "glide_convect": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), vp, vp, vp, vp, vp,
+    "glide_convect": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), vp, vp, vp, vp, vp, vp,
                                 P(GlideFootprintGrid), P(GlideEmit), vp]),
     "glide_graph_begin": (C.c_int, [vp]),
     "glide_graph_end": (C.c_int, [vp, P(vp)]),

@@ -267,9 +267,28 @@ struct ConvectArgs {
     const int32_t* i_lnb;
     const float* m_b;
     const float* lm;
+    const float4* cloud;  // per column: bl_total, w_total, scale, sum of d (NULL: derive per particle)
     FpDev fp;
     GlideEmit e;
 };
+
+// Per column: the cloud totals every particle of the column needs (the same
+// make_cloud and detrainment sum the particles would each repeat).
+__global__ void cloud_prepare_kernel(const int32_t* i_lcl, const int32_t* i_lnb, const float* m_b, const float* lm,
+                                     int nz, int64_t ncol, float4* cloud) {
+    const int64_t col = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    const int lcl = i_lcl[col], lnb = i_lnb[col];
+    const float mb = m_b[col];
+    float4 v = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+    if (lcl >= 0 && lnb > lcl && mb > 0.0f) {
+        const Cloud c = make_cloud(lcl, lnb, mb, lm, nz);
+        float sum_d = 0.0f;
+        for (int k = 0; k < nz; ++k) sum_d += cloud_d(c, k);
+        v = make_float4(c.bl_total, c.w_total, c.scale, sum_d);
+    }
+    cloud[col] = v;
+}
 
 // Convection of particle i. With GLIDE_F_FIXED its footprint key and
 // fixed-point value go to fkey/fval (the caller's dropped key and 0 stay for an
@@ -313,10 +332,17 @@ __device__ __forceinline__ void convect_particle(const ConvectArgs& a, const flo
         const int up = min(max(cnt, 1), nz - 1);
         const int lo = up - 1;
         const int host = ((z - lev[lo]) < (lev[up] - z)) ? lo : up;
-        const Cloud c = make_cloud(lcl, lnb, mb, lm, nz);
         // Column `host` of fmassfrac, over m[host] (emanuel.py:756-779).
+        Cloud c;
         float sum_d = 0.0f;
-        for (int k = 0; k < nz; ++k) sum_d += cloud_d(c, k);
+        if (a.cloud != nullptr) {
+            const float4 cv = a.cloud[col];
+            c = Cloud{lcl, lnb, mb, cv.x, cv.y, cv.z};
+            sum_d = cv.w;
+        } else {
+            c = make_cloud(lcl, lnb, mb, lm, nz);
+            for (int k = 0; k < nz; ++k) sum_d += cloud_d(c, k);
+        }
         const float d_h = cloud_d(c, host);
         float ce = 0.0f, cd = 0.0f;
         for (int k = 0; k <= host && k < nz; ++k) {
@@ -541,8 +567,8 @@ extern "C" int glide_column_cape(const float* t_s, const float* t_e, const float
 
 extern "C" int glide_convect(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
                              const int32_t* i_lcl, const int32_t* i_lnb, const float* m_b,
-                             const float* layer_mass, double* colsum_err, const GlideFootprintGrid* fg,
-                             const GlideEmit* e, void* stream) {
+                             const float* layer_mass, double* colsum_err, float* cloud,
+                             const GlideFootprintGrid* fg, const GlideEmit* e, void* stream) {
     GLIDE_REQUIRE(p && s && g && i_lcl && i_lnb && m_b && layer_mass, "glide_convect: NULL argument");
     GLIDE_REQUIRE(g->nz >= 2 && g->nz <= kConvLevels && g->nx >= 2 && g->ny >= 2, "bad grid");
     GLIDE_REQUIRE(g->lon1 != g->lon0 && g->lat1 != g->lat0, "Grid dimension has 0 span (first == last)");
@@ -583,6 +609,12 @@ extern "C" int glide_convect(const GlideStepParams* p, const GlideParticles* s, 
         a.e = *e;
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (cloud != nullptr && s->n > 0) {
+        const int64_t ncol = static_cast<int64_t>(g->ny) * g->nx;
+        hipLaunchKernelGGL(cloud_prepare_kernel, dim3(static_cast<unsigned>((ncol + 255) / 256)), dim3(256), 0, st,
+                           i_lcl, i_lnb, m_b, layer_mass, g->nz, ncol, reinterpret_cast<float4*>(cloud));
+        a.cloud = reinterpret_cast<const float4*>(cloud);
+    }
     if (s->n > 0) {
         hipLaunchKernelGGL(convect_kernel, dim3(static_cast<unsigned>((s->n + 255) / 256)), dim3(256), 0, st, a);
     }

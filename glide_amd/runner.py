@@ -359,6 +359,7 @@ class BatchEngine:
         self.layer_mass = torch.empty(self.met.nz, dtype=f32, device=d)
         self.colsum_err = torch.zeros(nc, dtype=torch.float64, device=d)
         self.conv_lift = torch.empty((4, self.met.ny, self.met.nx), dtype=f32, device=d)
+        self.conv_cloud = torch.empty(4 * nc, dtype=f32, device=d)  # per-column cloud totals of glide_convect
         self.live = torch.zeros(1, dtype=torch.int32, device=d)
         # Step table for the captured step body: one GlideStepParams row per
         # schedule step (the flags of a plain emitting step), and the device
@@ -802,7 +803,7 @@ class BatchEngine:
             L.check(lib.glide_convect(C.byref(pc), C.byref(self.parts), C.byref(self.met.grid),
                                       self.i_lcl.data_ptr(), self.i_lnb.data_ptr(), self.m_b.data_ptr(),
                                       self.layer_mass.data_ptr(), self.colsum_err.data_ptr(),
-                                      C.byref(self.fgs[buf]), C.byref(em), stream))
+                                      self.conv_cloud.data_ptr(), C.byref(self.fgs[buf]), C.byref(em), stream))
         if self.overlap:
             self.ev_step[buf].record(self._stream_obj(stream))
             self.acc_stream.torch.wait_event(self.ev_step[buf])

@@ -414,13 +414,17 @@ int glide_convect_columns(const float* conv, const float* t_s, const float* t_e,
  * Active particles sample a destination level from the transposed, renormalised
  * column of the non-divergent mass-flux matrix of their nearest column; movers
  * get z uniform in [h[dest-1], h[dest]]. colsum_err (may be NULL): per-column
- * float64 max |row sum - column sum| / m (mass-conservation check).
+ * float64 max |row sum - column sum| / m (mass-conservation check). cloud (may
+ * be NULL): device float[4*ny*nx] scratch (16 B aligned); when given, each
+ * column's cloud totals (boundary-layer and detrainment-shape totals, CFL
+ * scale, detrainment sum) are computed once into it before the particles
+ * sample, instead of in every particle: the same arithmetic, the same result.
  * With GLIDE_F_EMIT in p->flags the footprint keys and the escape kill are
  * produced after the move, as glide_step does.
  */
 int glide_convect(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
                   const int32_t* i_lcl, const int32_t* i_lnb, const float* m_b,
-                  const float* layer_mass, double* colsum_err, const GlideFootprintGrid* fg,
+                  const float* layer_mass, double* colsum_err, float* cloud, const GlideFootprintGrid* fg,
                   const GlideEmit* e, void* stream);
 
 /*

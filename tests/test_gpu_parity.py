@@ -331,11 +331,20 @@ def test_convection_parity():
     parts.active = act_d.data_ptr()
     p = eng.step_params(eng.sched[5], L.F_MASK_INPUT)
     err = torch.zeros(ny * nx, dtype=torch.float64, device="cuda")
+    # with the per-column cloud totals prepared once (the engine's call), then
+    # derived in every particle (cloud NULL) from the same start: the same bits
+    cloud = torch.empty(4 * ny * nx, dtype=torch.float32, device="cuda")
     L.check(lib.glide_convect(C.byref(p), C.byref(parts), C.byref(eng.met.grid), eng.i_lcl.data_ptr(),
                               eng.i_lnb.data_ptr(), eng.m_b.data_ptr(), eng.layer_mass.data_ptr(), err.data_ptr(),
-                              None, None, _stream()))
+                              cloud.data_ptr(), None, None, _stream()))
     torch.cuda.synchronize()
     z_dev = eng.z.cpu().numpy()
+    eng.z.copy_(torch.from_numpy(z))
+    L.check(lib.glide_convect(C.byref(p), C.byref(parts), C.byref(eng.met.grid), eng.i_lcl.data_ptr(),
+                              eng.i_lnb.data_ptr(), eng.m_b.data_ptr(), eng.layer_mass.data_ptr(), None,
+                              None, None, None, _stream()))
+    torch.cuda.synchronize()
+    assert np.array_equal(eng.z.cpu().numpy(), z_dev)
     col = OC.nearest_column(lon, lat, met.lon[0], met.lon[-1], met.lat[0], met.lat[-1], nx, ny)
     z_ref = OC.convect_columns(z.astype(np.float64), active, col, olcl, olnb, omb.astype(np.float64),
                                ph.cpu().numpy(), met.spec.agl_levels, eng.seed, eng.ra.uid,
