@@ -185,6 +185,36 @@ __device__ __forceinline__ float safe_den(float d) { return fabsf(d) < 1e-6f ? 1
 // Two-pass population std of 0.5 * (s + e) over the edge-clipped stencil.
 __device__ float local_std(const float* __restrict__ s, const float* __restrict__ e, int y, int x, int ny,
                            int nx, int r) {
+    if (r == 1) {
+        // the shipped 3 x 3 stencil: the midpoints stay in registers between
+        // the passes (the same values, added in the same order)
+        float v[9];
+        bool in[9];
+        int cnt = 0;
+        float sum = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int yy = y + q / 3 - 1, xx = x + q % 3 - 1;
+            in[q] = yy >= 0 && yy < ny && xx >= 0 && xx < nx;
+            v[q] = 0.0f;
+            if (in[q]) {
+                const int64_t c = static_cast<int64_t>(yy) * nx + xx;
+                v[q] = 0.5f * (s[c] + e[c]);
+                sum += v[q];
+                ++cnt;
+            }
+        }
+        const float mean = sum / static_cast<float>(cnt);
+        float ss = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            if (in[q]) {
+                const float d = v[q] - mean;
+                ss += d * d;
+            }
+        }
+        return sqrtf(ss / static_cast<float>(cnt));
+    }
     float sum = 0.0f;
     int cnt = 0;
     for (int dy = -r; dy <= r; ++dy) {
