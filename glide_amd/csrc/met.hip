@@ -1,5 +1,5 @@
-// Hourly met kernels: terrain-following resample (one thread per column,
-// coalesced along x) and the per-window packed-record build.
+// Hourly met kernels: terrain-following resample (one thread per column and
+// target level, coalesced along x) and the per-window packed-record build.
 #include "common.h"
 
 namespace glide {
@@ -152,13 +152,25 @@ __global__ __launch_bounds__(kColBlock) void resample_kernel(const ResampleArgs 
     if (threadIdx.x == 0) a.p_part[static_cast<int64_t>(blockIdx.x) * a.za + j] = psum[0];
 }
 
-__global__ void pressure_mean_kernel(const double* __restrict__ part, int64_t nblk, int32_t za, int64_t ncol,
-                                     double* __restrict__ out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= za) return;
+constexpr int kMeanStage = 1024;  // partials staged in LDS per pass
+
+// One workgroup per level: the per-block partials are loaded in parallel into
+// LDS, then added in block order by one thread (the order the float64 sum
+// has always had), so the loads no longer wait on one another.
+__global__ __launch_bounds__(256) void pressure_mean_kernel(const double* __restrict__ part, int64_t nblk,
+                                                            int32_t za, int64_t ncol, double* __restrict__ out) {
+    __shared__ double stage[kMeanStage];
+    const int j = blockIdx.x;
     double s = 0.0;
-    for (int64_t b = 0; b < nblk; ++b) s += part[b * za + j];
-    out[j] = s / static_cast<double>(ncol);
+    for (int64_t b0 = 0; b0 < nblk; b0 += kMeanStage) {
+        const int64_t nb = nblk - b0 < kMeanStage ? nblk - b0 : kMeanStage;
+        for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) stage[b] = part[(b0 + b) * za + j];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int64_t b = 0; b < nb; ++b) s += stage[b];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = s / static_cast<double>(ncol);
 }
 
 // ---------------------------------------------------------------------------
@@ -388,7 +400,8 @@ extern "C" int glide_resample_hour(const float* src3, int32_t c3, int32_t zp, in
                    p_level_hpa, iu, iv, iw, dst3, part, flipped};
     hipLaunchKernelGGL(resample_kernel, dim3(static_cast<unsigned>(nblk), static_cast<unsigned>(za)), dim3(kColBlock),
                        0, st, a);
-    hipLaunchKernelGGL(pressure_mean_kernel, dim3((za + 63) / 64), dim3(64), 0, st, part, nblk, za, ncol, p_out_hpa);
+    hipLaunchKernelGGL(pressure_mean_kernel, dim3(static_cast<unsigned>(za)), dim3(256), 0, st, part, nblk, za, ncol,
+                       p_out_hpa);
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
 }
