@@ -81,6 +81,8 @@ def parse():
     ap.add_argument("--accumulate", choices=("fixed", "sort"), default="fixed",
                     help="footprint accumulation: exact fixed-point sums inside the step kernel, or keys and "
                          "values sorted by cell and reduced in a fixed order")
+    ap.add_argument("--no-other-accumulation", action="store_true",
+                    help="skip timing the headline steps once more with the other footprint accumulation")
     ap.add_argument("--no-overlap", action="store_true",
                     help="sort accumulation: accumulate each step's footprint on the step's own stream "
                          "(no second stream)")
@@ -516,6 +518,27 @@ def main() -> int:
         },
         "roofline": roofline,
     }
+    if not args.no_other_accumulation:
+        # the same K steps with the other footprint accumulation, on the same met
+        other = "sort" if args.accumulate == "fixed" else "fixed"
+        oe = BatchEngine(cfg, device="cuda", met=met, device_met=eng.met, accumulate=other, overlap=True,
+                         **{k: v for k, v in kw.items() if k not in ("accumulate", "overlap")}, **tuned["best"])
+        oe.run(W)
+        torch.cuda.synchronize()
+        oe.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            oe.run_step(sptr)
+        oe.join(sptr)
+        torch.cuda.synchronize()
+        oel = time.perf_counter() - t0
+        out[f"accumulate_{other}"] = {"ms_per_step": oel * 1e3 / K, "value": psteps / oel,
+                                      "method": ("keys and values written by the step, stable radix sort by cell, "
+                                                 "fixed-order float32 reduction on a second stream" if other == "sort"
+                                                 else "exact int64 fixed-point sums inside the step kernel")}
+        oe.close()
+        del oe
     eng.close()
     del eng
     torch.cuda.empty_cache()
