@@ -366,6 +366,7 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
     if constexpr (MODE == kModeTable) {
         int r = 0;
         p = load_row(a.table, a.row, r);
+        p.flags = a.p.flags;  // the template's, which the host checked the pointers for
         x.escaped += r;
         if (n_active != nullptr) n_active += static_cast<int64_t>(r) * active_copies(a.e);
     } else if constexpr (MODE == kModeSlots) {

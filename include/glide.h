@@ -229,7 +229,8 @@ int glide_step(const GlideStepParams* p, const GlideParticles* s, const GlideGri
  * workgroup to finish advances row[0] by one. row points to GLIDE_ROW_WORDS
  * device int32: the cursor, then finished-workgroup counters that must be 0
  * at launch (they are left 0; glide_row_set sets them). tmpl is a host row
- * with the flags every table row carries; it is used for argument checks. */
+ * whose flags every step takes (the rows' flags fields are not read); the
+ * argument checks use it. */
 #define GLIDE_ROW_WORDS 66
 int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, int32_t* row,
                      const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
