@@ -234,6 +234,26 @@ def _set_state(eng, S):
 
 def test_step_parity_first_met_hour(cfg0):
     cfg, met = cfg0
+    _step_parity(cfg, met, 61)
+
+
+@pytest.mark.parametrize("za", [2, 128])
+def test_step_parity_at_the_level_limits(za):
+    """The same step-by-step check on a met grid of 2 AGL levels (one layer:
+    every lookup clamps to it) and of 128 (the most the step kernel takes:
+    the level search's deepest case), 20 steps of the 10k workload."""
+    from glide_amd.synthetic import MetSpec
+
+    base = workload_config("plain-cpu-10k-3h")
+    cfg = dict(base, spec=MetSpec(nx=64, ny=48, za=za, n_hours=4, seed=20240101))
+    met = SyntheticMet(cfg["spec"])
+    assert len(met.spec.agl_levels) == za
+    _step_parity(cfg, met, 21)
+
+
+def _step_parity(cfg, met, n_steps):
+    """Steps the engine from the oracle's float32-rounded state, one schedule
+    step at a time, and compares each step with the oracle's."""
     eng = BatchEngine(cfg, device="cuda", met=met)
     sched = eng.sched
     ra = eng.ra
@@ -252,7 +272,7 @@ def test_step_parity_first_met_hour(cfg0):
     S["uid"] = ra.uid
     worst = []
     n_checked = 0
-    for st in sched[:61]:
+    for st in sched[:n_steps]:
         bh = st.bracket_hour
         if bh not in windows:
             windows[bh] = ORUN.build_window(hour(bh), hour(bh + 1), met.spec.agl_levels, met.lon, met.lat,
@@ -282,7 +302,7 @@ def test_step_parity_first_met_hour(cfg0):
             worst.append((st.index, frac_ok, float(e_lon.max()), float(e_lat.max()), float(e_z.max())))
             assert frac_ok >= 0.999, worst[-1]
         S = {k: (ref[k] if k != "uid" else v) for k, v in S.items()}  # oracle chain in float64
-    assert n_checked >= 59
+    assert n_checked >= n_steps - 2
     print("step parity (index, frac within 1e-5, max rel lon, lat, z):", worst[:3], worst[-3:])
 
 
