@@ -23,7 +23,7 @@ Snapshots differ by slow phase drift of every mode.
 
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from functools import cached_property
 
 import numpy as np

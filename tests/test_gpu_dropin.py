@@ -4,7 +4,7 @@ FootprintGridder (footprint_gridder.py), HannaScheme.step
 
 from __future__ import annotations
 
-from datetime import datetime, timedelta, timezone
+from datetime import datetime, timezone
 
 import numpy as np
 import pytest

@@ -22,7 +22,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 from glide_amd import _lib as L  # noqa: E402
-from glide_amd.runner import BatchEngine, ConvectionOptions, DeviceMet, HannaOptions  # noqa: E402
+from glide_amd.runner import BatchEngine, ConvectionOptions, DeviceMet  # noqa: E402
 from glide_amd.synthetic import SyntheticMet, workload_config  # noqa: E402
 from oracle import convection as OC  # noqa: E402
 from oracle import gridder as OG  # noqa: E402

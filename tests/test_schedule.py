@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from glide_amd.schedule import cursor_schedule, particle_ids, release_arrays
-from glide_amd.synthetic import MetSpec, Release, SyntheticMet, network_releases, serving_requests, workload_config
+from glide_amd.synthetic import MetSpec, Release, SyntheticMet, serving_requests, workload_config
 from oracle import run as ORUN
 
 
