@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kMeanThreads) void mean_partial_kernel(const MeanAr
         if (lane == 0) red[wave][c] = s;
     }
     __syncthreads();
-    if (threadIdx.x < a.nch) {
+    if (static_cast<int>(threadIdx.x) < a.nch) {
         double s = 0.0;
         for (int w = 0; w < kMeanThreads / 64; ++w) s += red[w][threadIdx.x];
         a.partial[static_cast<int64_t>(blockIdx.x) * kMeanCh + threadIdx.x] = s;
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kMeanThreads) void mean_final_kernel(const double* 
         if (lane == 0) red[wave][c] = s;
     }
     __syncthreads();
-    if (threadIdx.x < nch) {
+    if (static_cast<int>(threadIdx.x) < nch) {
         double s = 0.0;
         for (int w = 0; w < kMeanThreads / 64; ++w) s += red[w][threadIdx.x];
         out[static_cast<int64_t>(r) * nch + threadIdx.x] = s / static_cast<double>(n);

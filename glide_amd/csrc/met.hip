@@ -28,7 +28,7 @@ __global__ __launch_bounds__(256) void level_orientation_kernel(const float* __r
     s1[threadIdx.x] = b;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) {
+        if (static_cast<int>(threadIdx.x) < off) {
             s0[threadIdx.x] += s0[threadIdx.x + off];
             s1[threadIdx.x] += s1[threadIdx.x + off];
         }
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kColBlock) void resample_kernel(const ResampleArgs 
     psum[threadIdx.x] = pval;
     __syncthreads();
     for (int off = kColBlock / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) psum[threadIdx.x] += psum[threadIdx.x + off];
+        if (static_cast<int>(threadIdx.x) < off) psum[threadIdx.x] += psum[threadIdx.x + off];
         __syncthreads();
     }
     if (threadIdx.x == 0) a.p_part[static_cast<int64_t>(blockIdx.x) * a.za + j] = psum[0];

@@ -102,7 +102,6 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         return false;
     }
 
-    const int64_t ps = s.pos_stride;
     float lon = ldo(s.lon, ip);
     float lat = ldo(s.lat, ip);
     float z = ldo(s.z, ip);
