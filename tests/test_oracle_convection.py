@@ -143,8 +143,12 @@ def test_lcl_lnb_cape_against_an_independent_solve():
                 f = lambda T, k=k: CV.equivalent_potential_temperature(  # noqa: E731
                     T, CV.saturation_specific_humidity(T, p[k]), p[k]) - the
                 # no root in the bracket (the adiabat leaves [180, 350] K aloft):
-                # the bisection converges onto the bracket end
-                tp[k] = 180.0 if f(180.0) > 0 else (350.0 if f(350.0) <= 0 else brentq(f, 180.0, 350.0, xtol=1e-12))
+                # the bisection converges onto the bracket end. At 350 K and
+                # low pressure q_sat is so large that theta_e's exponent
+                # overflows to inf, which still orders the bracket correctly.
+                with np.errstate(over="ignore"):
+                    tp[k] = 180.0 if f(180.0) > 0 else (350.0 if f(350.0) <= 0 else
+                                                        brentq(f, 180.0, 350.0, xtol=1e-12))
         qp = np.where(p >= pl, q[0], CV.saturation_specific_humidity(tp, p))
         tvp, tve = CV.virtual_temperature(tp, qp), CV.virtual_temperature(t, q)
         b = (tvp - tve) / tve
