@@ -22,7 +22,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 from glide_amd import _lib as L  # noqa: E402
-from glide_amd.runner import BatchEngine, ConvectionOptions, DeviceMet  # noqa: E402
+from glide_amd.runner import BatchEngine, ConvectionOptions, DeviceMet, HannaOptions  # noqa: E402
 from glide_amd.synthetic import SyntheticMet, workload_config  # noqa: E402
 from oracle import convection as OC  # noqa: E402
 from oracle import gridder as OG  # noqa: E402
@@ -475,3 +475,43 @@ def test_graph_replay_matches_eager_step_by_step(cfg0):
             assert ea.footprint.sum().item() > 0
         finally:
             eb.close()
+
+
+def test_table_rows_flags_fields_are_not_read(cfg0):
+    """glide_step_table takes every step's flags from the host template: a
+    device table whose rows carry GLIDE_F_SL_OVERRIDE (the surface-layer
+    override of sigma_w near the ground: arithmetic only, no extra memory)
+    replays bit for bit like the untouched table, while an engine whose
+    template carries the flag gives other positions."""
+    cfg, met = cfg0
+    n_steps = 40
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        ea = BatchEngine(cfg, device="cuda", met=met, use_graph=True)
+        eb = BatchEngine(cfg, device="cuda", met=met, use_graph=True, device_met=ea.met)
+        ec = BatchEngine(cfg, device="cuda", met=met, use_graph=True, device_met=ea.met,
+                         hanna=HannaOptions(surface_layer_override=True))
+        try:
+            word = L.GlideStepParams.flags.offset // 4
+            per_row = C.sizeof(L.GlideStepParams) // 4
+            tab = eb.step_table.cpu().numpy().copy()
+            rows = tab.view(np.int32).reshape(-1, per_row)
+            assert not (rows[:, word] & L.F_SL_OVERRIDE).any()
+            rows[:, word] |= L.F_SL_OVERRIDE
+            eb.step_table.copy_(torch.from_numpy(tab))
+            for _ in range(n_steps):
+                ea.run_step()
+                eb.run_step()
+                ec.run_step()
+            s.synchronize()
+            assert eb.graph_launches > 0
+            back = eb.step_table.cpu().numpy().view(np.int32).reshape(-1, per_row)
+            assert (back[:, word] & L.F_SL_OVERRIDE).all()  # the rows kept the flag throughout
+            A, B, X = ea.state_numpy(), eb.state_numpy(), ec.state_numpy()
+            for k in A:
+                assert np.array_equal(A[k], B[k]), k
+            assert torch.equal(ea.footprint, eb.footprint)
+            assert not np.array_equal(A["z"], X["z"])  # the flag, when it is the template's, matters
+        finally:
+            for e in (ea, eb, ec):
+                e.close()
