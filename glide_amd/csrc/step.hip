@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
         // Served slots: one slot per wavefront (64-particle blocks).
         const int32_t slot = __builtin_amdgcn_readfirstlane(run ? a.wave_slot[i >> 6] : -1);
         bool stepping = false;
-        if (slot >= 0) {
+        if (slot >= 0 && slot < a.fp.n_r) {  // the slots array holds fp.n_r entries
             GlideSlot* sl = a.slots + slot;
             stepping = __builtin_amdgcn_readfirstlane(sl->stepping) != 0;
             const int32_t hb = __builtin_amdgcn_readfirstlane(sl->bracket) - a.ring.first_hour;

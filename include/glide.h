@@ -483,8 +483,9 @@ int glide_slots_advance(GlideSlot* slots, int32_t n_slots, double dt, void* stre
  * such blocks. Each wavefront takes dt, alpha, cursor and noise step from its
  * slot and its met window from the ring (the slot's bracket); flags, seed and
  * the Hanna options from tmpl. Footprint keys use the slot as the release
- * index (fg->n_r >= number of slots); escape and active counts go to the
- * slot. Waves of non-stepping slots only write dropped keys. With
+ * index: slots holds fg->n_r entries, and a wave_slot entry outside
+ * [0, fg->n_r) marks an unused block like -1; escape and active counts go to
+ * the slot. Waves of non-stepping slots only write dropped keys. With
  * GLIDE_F_FIXED the residence goes to e->fixed as in glide_step, with
  * e->fixed_exp indexed by slot. */
 int glide_step_slots(const GlideStepParams* tmpl, GlideSlot* slots, const int32_t* wave_slot,

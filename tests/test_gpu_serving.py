@@ -166,3 +166,31 @@ def test_sort_accumulation_server_matches_fixed(setup, mixed):
     assert np.array_equal(res.footprint != 0, ref.footprint != 0)
     np.testing.assert_allclose(res.footprint, ref.footprint, rtol=2e-5, atol=0)
     assert abs(res.total - ref.total) <= 1e-9 * ref.total
+
+
+def test_out_of_range_wave_slots_mark_unused_blocks(setup):
+    """wave_slot entries outside [0, max_slots) (the footprint's n_r, the
+    size of the slots array) mark unused blocks like -1: a server whose free
+    blocks hold such values on the device serves a request bit for bit like
+    an untouched server."""
+    cfg, met, ring, reqs = setup
+    out = []
+    for poke in (False, True):
+        srv = _server(cfg, ring)
+        assert srv.admit(reqs[0])
+        if poke:
+            free = np.flatnonzero(srv.wave_slot_h < 0)
+            assert free.size > 3
+            bad = np.resize(np.array([srv.max_slots, srv.max_slots + 1000, 2**31 - 1], np.int32), free.size)
+            dev = srv.wave_slot.device
+            srv.wave_slot[torch.from_numpy(free).to(dev)] = torch.from_numpy(bad).to(dev)
+            torch.cuda.synchronize()
+            print(f"wave_slot: {free.size} free blocks set to {bad[:3].tolist()}", flush=True)
+        (res,) = srv.drain()
+        out.append(res)
+    a, b = _by_uid(out[0].state), _by_uid(out[1].state)
+    for k in STATE:
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(out[0].footprint, out[1].footprint) and out[0].total == out[1].total
+    assert out[0].escaped == out[1].escaped and out[0].active_steps == out[1].active_steps
+    assert out[0].total > 0
