@@ -13,7 +13,7 @@ from pathlib import Path
 _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libglide_hip.so"
 
-ABI_VERSION = 4  # GLIDE_ABI_VERSION of include/glide.h
+ABI_VERSION = 5  # GLIDE_ABI_VERSION of include/glide.h
 
 GLIDE_OK = 0
 GLIDE_ERR_INVALID = -1
@@ -114,7 +114,7 @@ SIGNATURES = {
     "glide_last_error": (C.c_char_p, []),
     "glide_step": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), P(GlideWindow),
                              P(GlideFootprintGrid), P(GlideEmit), vp]),
-    "glide_step_table": (C.c_int, [P(GlideStepParams), vp, vp, P(GlideParticles), P(GlideGrid), P(GlideWindow),
+    "glide_step_table": (C.c_int, [P(GlideStepParams), vp, C.c_int32, vp, P(GlideParticles), P(GlideGrid), P(GlideWindow),
                                    P(GlideFootprintGrid), P(GlideEmit), vp]),
     "glide_row_set": (C.c_int, [vp, C.c_int32, vp]),
     "glide_column_cape": (C.c_int, [vp, vp, vp, vp, C.c_double, vp, vp, C.c_int32, C.c_int32, C.c_int32,

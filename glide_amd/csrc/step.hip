@@ -35,6 +35,7 @@ struct StepArgs {
     // escape count goes to e.escaped[row[0]], and the last workgroup to finish
     // advances row[0] (row[1..] count finished workgroups).
     const GlideStepParams* table;
+    int32_t n_rows;  // rows of the table; a cursor outside [0, n_rows) steps nothing
     int32_t* row;
     unsigned long long* n_active;  // optional per-step active count (e->active)
     // Served slots (glide_step_slots): per-wavefront slot, scalars and window.
@@ -48,12 +49,11 @@ constexpr int kStepBlock = 256;     // default threads per workgroup
 constexpr int kStepBlockMax = 512;  // GlideStepParams.block may choose 64 .. 512
 constexpr uint32_t kRowGroups = GLIDE_ROW_WORDS - 2;  // finished-workgroup counters of the step cursor
 
-// The step's row of the table, read into scalar registers (one readfirstlane
-// per 32-bit word), so everything derived from it stays scalar.
-__device__ __forceinline__ GlideStepParams load_row(const GlideStepParams* table, const int32_t* row, int& r) {
+// Row r of the table, read into scalar registers (one readfirstlane per
+// 32-bit word), so everything derived from it stays scalar.
+__device__ __forceinline__ GlideStepParams load_row(const GlideStepParams* table, int r) {
     static_assert(sizeof(GlideStepParams) % 4 == 0, "GlideStepParams is a whole number of words");
     constexpr int kWords = sizeof(GlideStepParams) / 4;
-    r = __builtin_amdgcn_readfirstlane(*row);
     const uint32_t* w = reinterpret_cast<const uint32_t*>(table + r);
     uint32_t buf[kWords];
 #pragma unroll
@@ -362,12 +362,25 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
         static_cast<int64_t>(MODE == kModeSlots ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * blockDim.x +
         threadIdx.x;
     bool run = i < a.s.n;
+    bool in_table = true;  // table mode: the cursor names a row
     if constexpr (MODE == kModeTable) {
-        int r = 0;
-        p = load_row(a.table, a.row, r);
-        p.flags = a.p.flags;  // the template's, which the host checked the pointers for
-        x.escaped += r;
-        if (n_active != nullptr) n_active += static_cast<int64_t>(r) * active_copies(a.e);
+        // every workgroup reads row[0] before any can advance it: the same r
+        const int r = __builtin_amdgcn_readfirstlane(*a.row);
+        in_table = r >= 0 && r < a.n_rows;
+        if (in_table) {
+            p = load_row(a.table, r);
+            p.flags = a.p.flags;  // the template's, which the host checked the pointers for
+            x.escaped += r;
+            if (n_active != nullptr) n_active += static_cast<int64_t>(r) * active_copies(a.e);
+        } else {
+            if (!FIXED && run && (p.flags & GLIDE_F_EMIT)) {
+                const uint32_t i4 = static_cast<uint32_t>(i) * 4u;
+                sto(a.e.keys, i4, a.fp.n_keys);
+                sto(a.e.vals, i4, 0.0f);
+            }
+            run = false;
+            n_active = nullptr;
+        }
     } else if constexpr (MODE == kModeSlots) {
         // Served slots: one slot per wavefront (64-particle blocks).
         const int32_t slot = __builtin_amdgcn_readfirstlane(run ? a.wave_slot[i >> 6] : -1);
@@ -421,7 +434,7 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
             if (c) atomicAdd(n_active + blockIdx.x % active_copies(a.e), static_cast<unsigned long long>(c));
         }
     }
-    if constexpr (MODE == kModeTable) {
+    if (MODE == kModeTable && in_table) {
         // Every workgroup has used row[0] before it counts itself finished, so
         // the last one to finish can advance the cursor (no waiting). The count
         // is split over kRowGroups counters and a top counter, so no single
@@ -486,7 +499,7 @@ static int fill_grid(StepArgs& a, const GlideGrid& g, const GlideWindow& w) {
 
 using namespace glide;
 
-static int launch_step(const GlideStepParams* p, const GlideStepParams* table, int32_t* row,
+static int launch_step(const GlideStepParams* p, const GlideStepParams* table, int32_t n_rows, int32_t* row,
                        const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
                        const GlideFootprintGrid* fg, const GlideEmit* e, void* stream,
                        GlideSlot* slots = nullptr, const int32_t* wave_slot = nullptr,
@@ -515,6 +528,7 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
     a.p = *p;
     a.s = *s;
     a.table = table;
+    a.n_rows = n_rows;
     a.row = row;
     a.n_active = e ? e->active : nullptr;
     if (e != nullptr) {
@@ -582,7 +596,7 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
 extern "C" int glide_step(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
                           const GlideWindow* w, const GlideFootprintGrid* fg, const GlideEmit* e,
                           void* stream) {
-    return launch_step(p, nullptr, nullptr, s, g, w, fg, e, stream);
+    return launch_step(p, nullptr, 0, nullptr, s, g, w, fg, e, stream);
 }
 
 __global__ void slots_advance_kernel(GlideSlot* slots, int32_t n, double dt) {
@@ -633,14 +647,15 @@ extern "C" int glide_step_slots(const GlideStepParams* tmpl, GlideSlot* slots, c
     GLIDE_REQUIRE(!(tmpl->flags & GLIDE_F_MASK_INPUT), "served slots compute their own mask");
     GLIDE_REQUIRE(ring->wind && ring->aux && ring->surf && ring->n_hours >= 1, "empty met ring");
     const GlideWindow w{ring->wind, ring->aux, ring->surf, ring->dtype, 0};
-    return launch_step(tmpl, nullptr, nullptr, s, g, &w, fg, e, stream, slots, wave_slot, ring);
+    return launch_step(tmpl, nullptr, 0, nullptr, s, g, &w, fg, e, stream, slots, wave_slot, ring);
 }
 
-extern "C" int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, int32_t* row,
-                                const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
+extern "C" int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, int32_t n_rows,
+                                int32_t* row, const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
                                 const GlideFootprintGrid* fg, const GlideEmit* e, void* stream) {
     GLIDE_REQUIRE(table != nullptr && row != nullptr, "glide_step_table: NULL table/row");
-    return launch_step(tmpl, table, row, s, g, w, fg, e, stream);
+    GLIDE_REQUIRE(n_rows >= 1, "glide_step_table: n_rows must be >= 1");
+    return launch_step(tmpl, table, n_rows, row, s, g, w, fg, e, stream);
 }
 
 __global__ void row_set_kernel(int32_t* row, int32_t value) {

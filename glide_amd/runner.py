@@ -586,7 +586,8 @@ class BatchEngine:
                          active=self._active.data_ptr(), active_copies=ACTIVE_COPIES, **self._fixed_emit())
         L.check(lib.glide_graph_begin(stream))
         try:
-            L.check(lib.glide_step_table(C.byref(self.step_tmpl), self.step_table.data_ptr(), self.row.data_ptr(),
+            L.check(lib.glide_step_table(C.byref(self.step_tmpl), self.step_table.data_ptr(), len(self.sched),
+                                         self.row.data_ptr(),
                                          C.byref(self.parts), C.byref(self.met.grid), C.byref(self.met.window),
                                          C.byref(self.fgs[0]), C.byref(em), stream))
             self._accumulate(stream)

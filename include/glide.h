@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define GLIDE_ABI_VERSION 4
+#define GLIDE_ABI_VERSION 5
 
 #define GLIDE_OK 0
 #define GLIDE_ERR_INVALID (-1)
@@ -226,13 +226,15 @@ int glide_step(const GlideStepParams* p, const GlideParticles* s, const GlideGri
  * hanna.py:1117-1160 with main.py:1396-1477): the step reads
  * table[row[0]] (device GlideStepParams rows, one per schedule step), adds its
  * escape count to e->escaped[row[0]], and the last
- * workgroup to finish advances row[0] by one. row points to GLIDE_ROW_WORDS
+ * workgroup to finish advances row[0] by one. A cursor outside [0, n_rows)
+ * steps no particle (with GLIDE_F_EMIT and no GLIDE_F_FIXED it writes
+ * dropped keys), counts nothing and is left as it is. row points to GLIDE_ROW_WORDS
  * device int32: the cursor, then finished-workgroup counters that must be 0
  * at launch (they are left 0; glide_row_set sets them). tmpl is a host row
  * whose flags every step takes (the rows' flags fields are not read); the
  * argument checks use it. */
 #define GLIDE_ROW_WORDS 66
-int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, int32_t* row,
+int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, int32_t n_rows, int32_t* row,
                      const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
                      const GlideFootprintGrid* fg, const GlideEmit* e, void* stream);
 

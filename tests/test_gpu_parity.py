@@ -515,3 +515,35 @@ def test_table_rows_flags_fields_are_not_read(cfg0):
         finally:
             for e in (ea, eb, ec):
                 e.close()
+
+
+def test_table_cursor_outside_the_table_steps_nothing(cfg0):
+    """The captured step replayed with its device cursor at or past the
+    table's n_rows, or negative, steps no particle, adds nothing to the
+    footprint and leaves the cursor and its counters as they were."""
+    cfg, met = cfg0
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        eng = BatchEngine(cfg, device="cuda", met=met, use_graph=True)
+        try:
+            for _ in range(5):
+                eng.run_step()
+            s.synchronize()
+            assert eng.graph_launches > 0 and eng._graphs
+            g = next(iter(eng._graphs.values()))
+            before = eng.state_numpy()
+            cells = eng.fixed_cells().clone()
+            sp = int(s.cuda_stream)
+            for cursor in (len(eng.sched), len(eng.sched) + 7, -1):
+                print(f"replay with cursor {cursor} (table rows {len(eng.sched)})", flush=True)
+                L.check(L.lib().glide_row_set(eng.row.data_ptr(), cursor, sp))
+                L.check(L.lib().glide_graph_launch(g, sp))
+                s.synchronize()
+                assert int(eng.row[0].item()) == cursor
+                assert not bool(eng.row[1:].any())
+            after = eng.state_numpy()
+            for k in before:
+                assert np.array_equal(before[k], after[k]), k
+            assert torch.equal(eng.fixed_cells(), cells)
+        finally:
+            eng.close()
