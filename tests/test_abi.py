@@ -106,3 +106,22 @@ def test_step_block_is_validated_before_any_launch():
         p = L.GlideStepParams(dt=60.0, substep_c=0.5, max_substeps=6, block=bad)
         with pytest.raises(ValueError, match="step block"):
             L.check(lib.glide_step(C.byref(p), C.byref(s), C.byref(g), C.byref(w), None, None, None))
+
+
+def test_step_table_row_count_is_validated_before_any_launch():
+    """glide_step_table needs the table's row count (>= 1): the kernel steps
+    nothing for a cursor outside [0, n_rows), so a table without rows is an
+    argument error, raised before anything reaches the device (the pointers
+    here are never read)."""
+
+    lib = L.lib()
+    fake = 0x1000
+    s = L.GlideParticles(n=64, pos_stride=1, lon=fake, lat=fake, z=fake, up=fake, vp=fake, wp=fake, um=fake,
+                         vm=fake, r_off=fake, alive=fake)
+    g = L.GlideGrid(nx=4, ny=4, nz=4, lon0=0.0, lon1=1.0, lat0=0.0, lat1=1.0, levels=fake)
+    w = L.GlideWindow(wind=fake, aux=fake, surf=fake, dtype=L.GLIDE_MET_F32)
+    p = L.GlideStepParams(dt=60.0, substep_c=0.5, max_substeps=6)
+    for bad in (0, -1):
+        with pytest.raises(ValueError, match="n_rows"):
+            L.check(lib.glide_step_table(C.byref(p), fake, bad, fake, C.byref(s), C.byref(g), C.byref(w), None,
+                                         None, None))
