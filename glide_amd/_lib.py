@@ -13,7 +13,7 @@ from pathlib import Path
 _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libglide_hip.so"
 
-ABI_VERSION = 5  # GLIDE_ABI_VERSION of include/glide.h
+ABI_VERSION = 6  # GLIDE_ABI_VERSION of include/glide.h
 
 GLIDE_OK = 0
 GLIDE_ERR_INVALID = -1
@@ -86,7 +86,7 @@ class GlideStepParams(C.Structure):
 class GlideSlot(C.Structure):
     _fields_ = [
         ("t", C.c_double), ("term", C.c_double), ("rel_time", C.c_double),
-        ("state", C.c_int32), ("k", C.c_int32),
+        ("state", C.c_int32), ("k", C.c_int32), ("noise_hi", C.c_uint32), ("reserved", C.c_int32),
         ("last_d", C.c_double), ("cursor_off", C.c_float), ("alpha", C.c_float), ("dt", C.c_float),
         ("noise_step", C.c_int32), ("bracket", C.c_int32), ("stepping", C.c_int32),
         ("escaped", C.c_uint64), ("active_steps", C.c_uint64),
@@ -153,6 +153,39 @@ SIGNATURES = {
     "glide_graph_launch": (C.c_int, [vp, vp]),
     "glide_graph_destroy": (C.c_int, [vp]),
 }
+
+def window_bytes(ncell: int, met_dtype: int) -> int:
+    """Bytes of one packed wind (or support) field of ``ncell`` met cells:
+    bfloat16 records of 16 B per cell, or float32 plane A (16 B per cell) then
+    plane B (8 B per cell)."""
+
+    return int(ncell) * (16 if met_dtype == GLIDE_MET_BF16 else 24)
+
+
+def unpack_window(wind, aux, nz: int, ny: int, nx: int, met_dtype: int):
+    """Packed window fields (uint8 torch tensors, column-major cells
+    [ny][nx][nz]) as float32 [nz, ny, nx, 6] tensors: wind channels u_s v_s w_s
+    u_e v_e w_e, support channels sigma_w_ft T_Lw_ft rho drho_dz sigma_mu
+    sigma_mv."""
+
+    import torch
+
+    ncell = nz * ny * nx
+    out = []
+    for buf, wind_order in ((wind, True), (aux, False)):
+        if met_dtype == GLIDE_MET_BF16:
+            u16 = buf[: ncell * 16].view(torch.int16).view(ncell, 8).to(torch.int32) & 0xFFFF
+            rec = (u16 << 16).view(torch.float32)[:, :6]
+        else:
+            a = buf[: ncell * 16].view(torch.float32).view(ncell, 4)
+            b = buf[ncell * 16: ncell * 24].view(torch.float32).view(ncell, 2)
+            if wind_order:  # plane A u_s v_s u_e v_e, plane B w_s w_e
+                rec = torch.stack([a[:, 0], a[:, 1], b[:, 0], a[:, 2], a[:, 3], b[:, 1]], dim=1)
+            else:  # plane A sigma_w T_Lw rho drho_dz, plane B sigma_mu sigma_mv
+                rec = torch.cat([a, b], dim=1)
+        out.append(rec.reshape(ny, nx, nz, 6).permute(2, 0, 1, 3))
+    return out[0], out[1]
+
 
 _LIB = None
 

@@ -130,6 +130,14 @@ struct Rec8<false> {
         v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
         v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
     }
+    // The first 6 values only (the records' last two slots are padding).
+    __device__ __forceinline__ static void load6_at(const void* base, uint32_t off, float (&v)[6]) {
+        const char* q = static_cast<const char*>(base) + off;
+        const float4 a = *reinterpret_cast<const float4*>(q);
+        const float2 b = *reinterpret_cast<const float2*>(q + 16);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y;
+    }
     __device__ __forceinline__ static void load2_at(const void* base, uint32_t off, float (&v)[2]) {
         const float2 a = *reinterpret_cast<const float2*>(static_cast<const char*>(base) + off);
         v[0] = a.x; v[1] = a.y;
@@ -165,6 +173,12 @@ struct Rec8<true> {
         v[2] = bf16_lo(a.y); v[3] = bf16_hi(a.y);
         v[4] = bf16_lo(a.z); v[5] = bf16_hi(a.z);
         v[6] = bf16_lo(a.w); v[7] = bf16_hi(a.w);
+    }
+    __device__ __forceinline__ static void load6_at(const void* base, uint32_t off, float (&v)[6]) {
+        const uint3 a = *reinterpret_cast<const uint3*>(static_cast<const char*>(base) + off);
+        v[0] = bf16_lo(a.x); v[1] = bf16_hi(a.x);
+        v[2] = bf16_lo(a.y); v[3] = bf16_hi(a.y);
+        v[4] = bf16_lo(a.z); v[5] = bf16_hi(a.z);
     }
     __device__ __forceinline__ static void load2_at(const void* base, uint32_t off, float (&v)[2]) {
         const uint32_t a = *reinterpret_cast<const uint32_t*>(static_cast<const char*>(base) + off);
@@ -229,17 +243,34 @@ __device__ __forceinline__ float frac_index(float c, float c0, float scale, int 
     return fminf(fmaxf(f, 0.0f), static_cast<float>(n - 1));
 }
 
-// Fractional ascending level index of z against levels[0..nz) (reference
+// Level ladders in LDS hold kLadder entries: levels[0..nz), then +inf, so a
+// branchless binary search may read past nz (a padded entry is never < z).
+constexpr int kLadder = 128;
+
+// First step of the level search: the smallest power of two t with
+// 2t - 1 >= nz - 1 (the steps t, t/2, .., 1 reach every count that the clamp
+// to nz - 1 distinguishes; the last entry read, 2t - 2, is below kLadder for
+// nz <= kLadder).
+__host__ __device__ inline int ladder_top(int nz) {
+    int t = 1;
+    while (2 * t - 1 < nz - 1) t *= 2;
+    return t;
+}
+
+// Copies levels[0..nz) into the LDS ladder, +inf after; every thread of the
+// block calls it, a barrier must follow.
+__device__ __forceinline__ void fill_ladder(float* lev, const float* levels, int nz) {
+    for (int k = threadIdx.x; k < kLadder; k += blockDim.x) lev[k] = k < nz ? levels[k] : __builtin_inff();
+}
+
+// Fractional ascending level index of z against the ladder (reference
 // gpu_engine.py:264-292): count = #levels < z, upper = clamp(count, 1, nz-1),
-// frac = clamp((z - L_lo) / max(L_hi - L_lo, 1e-6), 0, 1).
-__device__ __forceinline__ float level_index(const float* __restrict__ lev, int nz, float z) {
+// frac = clamp((z - L_lo) / max(L_hi - L_lo, 1e-6), 0, 1). The count is a
+// branchless binary search from step top (ladder_top(nz)); counts at or above
+// nz - 1 give the same upper level.
+__device__ __forceinline__ float level_index(const float* __restrict__ lev, int nz, int top, float z) {
     int cnt = 0;
-    int step = 64;
-    while (step > nz) step >>= 1;
-    for (; step > 0; step >>= 1) {
-        const int j = cnt + step;
-        if (j <= nz && lev[j - 1] < z) cnt = j;
-    }
+    for (int step = top; step > 0; step >>= 1) cnt = (lev[cnt + step - 1] < z) ? cnt + step : cnt;
     const int up = min(max(cnt, 1), nz - 1);
     const float lo = lev[up - 1];
     const float hi = lev[up];
@@ -286,18 +317,122 @@ __device__ __forceinline__ Corner3 corners3(float ix, float iy, float iz, int nx
     return c;
 }
 
-// Trilinear sample of the first K (<= 8) channels of a record field.
+// Trilinear sample of the first K (<= 8) channels of a record field (K <= 6
+// loads the first 6 values of each record only).
 template <bool BF16, int K>
 __device__ __forceinline__ void sample_rec(const void* base, const Corner3& c, float (&acc)[K]) {
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = 0.0f;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        float v[8];
-        Rec8<BF16>::load_at(base, c.o[q], v);
+        if constexpr (K <= 6) {
+            float v[6];
+            Rec8<BF16>::load6_at(base, c.o[q], v);
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] += c.w[q] * v[k];
+            for (int k = 0; k < K; ++k) acc[k] += c.w[q] * v[k];
+        } else {
+            float v[8];
+            Rec8<BF16>::load_at(base, c.o[q], v);
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc[k] += c.w[q] * v[k];
+        }
     }
+}
+
+// Packed window layouts (GlideWindow): bfloat16 records of 8 values, 16 B per
+// cell; float32 in two planes over the same cell index, plane A (16 B per
+// cell) followed by plane B (8 B per cell, from byte ncell * 16). Corner
+// offsets (corners3 with rb = kCellUnit) are cell * 16 B for bfloat16 (the
+// record) and cell * 8 B for float32 (plane B's entry; plane A's is at twice
+// that). A float32 gather thus reads 16 B and 8 B entries of a column's
+// levels, which share fewer cache lines than 32 B records would.
+template <bool BF16>
+struct WinRec {
+    static constexpr uint32_t kCellUnit = BF16 ? 16u : 8u;
+};
+__device__ __forceinline__ float4 plane_a(const void* base, uint32_t o) {
+    return *reinterpret_cast<const float4*>(static_cast<const char*>(base) + (o << 1));
+}
+__device__ __forceinline__ float plane_a0(const void* base, uint32_t o) {
+    return *reinterpret_cast<const float*>(static_cast<const char*>(base) + (o << 1));
+}
+__device__ __forceinline__ float2 plane_b(const void* base, uint32_t planeb, uint32_t o) {
+    return *reinterpret_cast<const float2*>(static_cast<const char*>(base) + planeb + o);
+}
+
+// Trilinear sample of the wind channels u_s v_s w_s u_e v_e w_e (float32
+// plane A = u_s v_s u_e v_e, plane B = w_s w_e).
+template <bool BF16>
+__device__ __forceinline__ void sample_wind(const void* base, uint32_t planeb, const Corner3& c, float (&acc)[6]) {
+    if constexpr (BF16) {
+        sample_rec<true, 6>(base, c, acc);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[k] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float4 A = plane_a(base, c.o[q]);
+            const float2 B = plane_b(base, planeb, c.o[q]);
+            acc[0] += c.w[q] * A.x;
+            acc[1] += c.w[q] * A.y;
+            acc[2] += c.w[q] * B.x;
+            acc[3] += c.w[q] * A.z;
+            acc[4] += c.w[q] * A.w;
+            acc[5] += c.w[q] * B.y;
+        }
+    }
+}
+
+// Trilinear sample of the support channels sigma_w_ft T_Lw_ft rho drho_dz
+// (float32 plane A) into acc[0..3]; bfloat16 records also give the meander
+// pair sigma_mu sigma_mv in acc[4..5] (float32: plane B, sample_aux_b).
+template <bool BF16>
+__device__ __forceinline__ void sample_aux_a(const void* base, const Corner3& c, float (&acc)[6]) {
+    if constexpr (BF16) {
+        sample_rec<true, 6>(base, c, acc);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[k] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float4 A = plane_a(base, c.o[q]);
+            acc[0] += c.w[q] * A.x;
+            acc[1] += c.w[q] * A.y;
+            acc[2] += c.w[q] * A.z;
+            acc[3] += c.w[q] * A.w;
+        }
+    }
+}
+
+// float32 support plane B (the meander pair sigma_mu sigma_mv).
+__device__ __forceinline__ void sample_aux_b(const void* base, uint32_t planeb, const Corner3& c, float& m0, float& m1) {
+    m0 = 0.0f;
+    m1 = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const float2 B = plane_b(base, planeb, c.o[q]);
+        m0 += c.w[q] * B.x;
+        m1 += c.w[q] * B.y;
+    }
+}
+
+// Trilinear sample of sigma_w_ft alone (the free-troposphere probes).
+template <bool BF16>
+__device__ __forceinline__ float sample_sigma_ft(const void* base, const Corner3& c) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        float v0;
+        if constexpr (BF16) {
+            float v[2];
+            Rec8<true>::load2_at(base, c.o[q], v);
+            v0 = v[0];
+        } else {
+            v0 = plane_a0(base, c.o[q]);
+        }
+        acc += c.w[q] * v0;
+    }
+    return acc;
 }
 
 // Trilinear sample of channels 0..1 only (the free-troposphere pair).

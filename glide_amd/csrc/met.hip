@@ -257,9 +257,11 @@ __device__ float local_std(const float* __restrict__ s, const float* __restrict_
 }
 
 // Levels per workgroup of the window build: one 128 B run of records per
-// column, so the column-major record writes go out in whole cache lines.
+// column (bfloat16: 8 records of 16 B; float32: 8 plane-A entries of 16 B, and
+// 8 plane-B entries of 8 B), so the column-major writes go out in whole or
+// half cache lines.
 template <bool BF16>
-constexpr int window_levels() { return 128 / static_cast<int>(Rec8<BF16>::kBytes); }
+constexpr int window_levels() { return 8; }
 
 // One thread per (column, level); a workgroup takes kColBlock / kTz
 // consecutive columns (the fastest index, so the field reads coalesce) times
@@ -269,8 +271,10 @@ template <bool BF16>
 __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
     constexpr int kTz = window_levels<BF16>();
     constexpr int kTc = kColBlock / kTz;
-    constexpr int kUnits = static_cast<int>(Rec8<BF16>::kBytes) / 16;  // 16 B units per record
-    __shared__ uint4 stage[2][kColBlock * kUnits];  // wind, aux records [column][level]
+    // staged [column][level]: bfloat16 records (wind, aux), or float32 plane-A
+    // entries (wind, aux) and plane-B entries (wind, aux)
+    __shared__ uint4 stage[2][kColBlock];
+    __shared__ uint2 stage_b[BF16 ? 1 : 2][BF16 ? 1 : kColBlock];
     const int64_t ncol = static_cast<int64_t>(a.ny) * a.nx;
     const int tc = static_cast<int>(threadIdx.x) % kTc;
     const int tz = static_cast<int>(threadIdx.x) / kTc;
@@ -325,23 +329,37 @@ __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
         const float smv = a.coeff * local_std(a.s3 + (static_cast<int64_t>(nz) + z) * ncol,
                                               a.e3 + (static_cast<int64_t>(nz) + z) * ncol, y, x, a.ny, a.nx,
                                               a.radius);
-        float wr[8] = {at(a.s3, 0, z), at(a.s3, 1, z), at(a.s3, 2, z), at(a.e3, 0, z),
-                       at(a.e3, 1, z), at(a.e3, 2, z), 0.0f, 0.0f};
-        float xr[8] = {sw, tl, rho(z), drho, smu, smv, 0.0f, 0.0f};
-        Rec8<BF16>::store(stage[0], tc * kTz + tz, wr);
-        Rec8<BF16>::store(stage[1], tc * kTz + tz, xr);
+        const int sl = tc * kTz + tz;
+        if constexpr (BF16) {
+            float wr[8] = {at(a.s3, 0, z), at(a.s3, 1, z), at(a.s3, 2, z), at(a.e3, 0, z),
+                           at(a.e3, 1, z), at(a.e3, 2, z), 0.0f, 0.0f};
+            float xr[8] = {sw, tl, rho(z), drho, smu, smv, 0.0f, 0.0f};
+            Rec8<true>::store(stage[0], sl, wr);
+            Rec8<true>::store(stage[1], sl, xr);
+        } else {
+            stage[0][sl] = make_uint4(__float_as_uint(at(a.s3, 0, z)), __float_as_uint(at(a.s3, 1, z)),
+                                      __float_as_uint(at(a.e3, 0, z)), __float_as_uint(at(a.e3, 1, z)));
+            stage_b[0][sl] = make_uint2(__float_as_uint(at(a.s3, 2, z)), __float_as_uint(at(a.e3, 2, z)));
+            stage[1][sl] = make_uint4(__float_as_uint(sw), __float_as_uint(tl), __float_as_uint(rho(z)),
+                                      __float_as_uint(drho));
+            stage_b[1][sl] = make_uint2(__float_as_uint(smu), __float_as_uint(smv));
+        }
     }
     __syncthreads();
-    // column-major records: a column's levels are contiguous, so each column's
-    // kTz staged records are one run of 128 B in the window
-    for (int q = static_cast<int>(threadIdx.x); q < kColBlock * kUnits; q += kColBlock) {
-        const int rl = q / kUnits;
-        const int64_t c = static_cast<int64_t>(blockIdx.x) * kTc + rl / kTz;
-        const int zz = static_cast<int>(blockIdx.y) * kTz + rl % kTz;
+    // column-major: a column's levels are contiguous, so each column's kTz
+    // staged entries are one run in the window (float32: in each plane)
+    const int64_t ncell = ncol * nz;
+    for (int q = static_cast<int>(threadIdx.x); q < kColBlock; q += kColBlock) {
+        const int64_t c = static_cast<int64_t>(blockIdx.x) * kTc + q / kTz;
+        const int zz = static_cast<int>(blockIdx.y) * kTz + q % kTz;
         if (c < ncol && zz < nz) {
-            const int64_t g = (c * nz + zz) * kUnits + q % kUnits;
+            const int64_t g = c * nz + zz;
             static_cast<uint4*>(a.wind)[g] = stage[0][q];
             static_cast<uint4*>(a.aux)[g] = stage[1][q];
+            if constexpr (!BF16) {
+                reinterpret_cast<uint2*>(static_cast<uint4*>(a.wind) + ncell)[g] = stage_b[0][q];
+                reinterpret_cast<uint2*>(static_cast<uint4*>(a.aux) + ncell)[g] = stage_b[1][q];
+            }
         }
     }
     if (!ok || z != 0) return;

@@ -143,4 +143,17 @@ __device__ __forceinline__ float ou(float w, float tl, float s2, float dt, float
     return a * w + drift_dt + fsqrt(var) * eta;
 }
 
+// The same update split in two: the decay a and amplitude sqrt(var), which
+// depend only on (T_L, sigma^2, dt), and their application to one draw.
+struct OuCoef {
+    float a, amp;
+};
+__device__ __forceinline__ OuCoef ou_coef(float tl, float s2, float dt) {
+    const float a = fexp(-fdiv(dt, tl));
+    return OuCoef{a, fsqrt(fmaxf(s2 * (1.0f - a * a), 0.0f))};
+}
+__device__ __forceinline__ float ou_apply(float w, const OuCoef& c, float eta, float drift_dt) {
+    return c.a * w + drift_dt + c.amp * eta;
+}
+
 }  // namespace glide

@@ -58,7 +58,7 @@ __device__ __forceinline__ bool key_one(const ReorderKeyArgs& a, const float* le
         const int64_t ps = s.pos_stride;
         const int ix = static_cast<int>(frac_index(s.lon[i * ps], a.lon0, a.sx, a.nx));
         const int iy = static_cast<int>(frac_index(s.lat[i * ps], a.lat0, a.sy, a.ny));
-        const int iz = static_cast<int>(level_index(lev, a.nz, s.z[i * ps]));
+        const int iz = static_cast<int>(level_index(lev, a.nz, ladder_top(a.nz), s.z[i * ps]));
         uint32_t grp = 0;
         if (a.groups > 1) {
             const int32_t r = s.rel_idx[i];
@@ -79,9 +79,9 @@ constexpr int kKeyMaxBlocks = 512;  // bounded grid: few live-count atomics on t
 // integer atomic per workgroup: atomics from every workgroup of a launch over
 // all particles queue on the single counter's address (≈10 ns each).
 __global__ __launch_bounds__(kKeyThreads) void reorder_key_kernel(const ReorderKeyArgs a) {
-    __shared__ float lev[kMaxLev];
+    __shared__ float lev[kLadder];
     __shared__ uint32_t blive;
-    for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
+    fill_ladder(lev, a.levels, a.nz);
     if (threadIdx.x == 0) blive = 0;
     __syncthreads();
     uint32_t c = 0;

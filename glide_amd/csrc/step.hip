@@ -25,6 +25,8 @@ struct StepArgs {
     float lon0, lat0, sx, sy;  // met grid: ix = (lon - lon0) * sx
     float blon0, blat0, bsx, bsy;  // surface bilinear mapping (same grid)
     int nx, ny, nz;
+    int ltop;  // first step of the level search (ladder_top(nz))
+    uint32_t planeb;  // float32 window: byte offset of plane B (ncell * 16)
     const float* levels;
     const void* wind;
     const void* aux;
@@ -44,7 +46,7 @@ struct StepArgs {
     GlideMetRing ring;
 };
 
-constexpr int kMaxLevels = 128;
+constexpr int kMaxLevels = kLadder;
 constexpr int kStepBlock = 256;     // default threads per workgroup
 constexpr int kStepBlockMax = 512;  // GlideStepParams.block may choose 64 .. 512
 constexpr uint32_t kRowGroups = GLIDE_ROW_WORDS - 2;  // finished-workgroup counters of the step cursor
@@ -70,6 +72,7 @@ struct StepCtx {
     const void* surf;
     unsigned long long* escaped;
     int32_t slot;  // >= 0: footprint release index of a served slot; -1: rel_idx
+    uint32_t noise_hi;  // Philox counter word 3 (a served slot's; 0 otherwise)
 };
 
 // Advances particle i; returns whether it was active. With GLIDE_F_FIXED the
@@ -88,6 +91,11 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     constexpr bool fixed = FIXED;
     const float co = static_cast<float>(p.cursor_off);
 
+    // The position loads go out with the mask's: one memory round trip before
+    // the first gather instead of two (i < n, so they are in bounds).
+    float lon = ldo(s.lon, ip);
+    float lat = ldo(s.lat, ip);
+    float z = ldo(s.z, ip);
     bool active;
     if (p.flags & GLIDE_F_MASK_INPUT) {
         active = ldo(s.active, i1) != 0;
@@ -101,10 +109,6 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         }
         return false;
     }
-
-    float lon = ldo(s.lon, ip);
-    float lat = ldo(s.lat, ip);
-    float z = ldo(s.z, ip);
     const float dt = static_cast<float>(p.dt);
     const float al = static_cast<float>(p.alpha);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -113,10 +117,10 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     auto wind_at = [&](float lo, float la, float zz, float& dlo, float& dla, float& dzz) {
         const float ix = frac_index(lo, a.lon0, a.sx, nx);
         const float iy = frac_index(la, a.lat0, a.sy, ny);
-        const float iz = level_index(lev, nz, zz);
-        const Corner3 c = corners3(ix, iy, iz, nx, ny, nz, Rec8<BF16>::kBytes);
+        const float iz = level_index(lev, nz, a.ltop, zz);
+        const Corner3 c = corners3(ix, iy, iz, nx, ny, nz, WinRec<BF16>::kCellUnit);
         float v[6];
-        sample_rec<BF16, 6>(x.wind, c, v);
+        sample_wind<BF16>(x.wind, a.planeb, c, v);
         const float u = v[0] * (1.0f - al) + v[3] * al;
         const float vv = v[1] * (1.0f - al) + v[4] * al;
         const float w = v[2] * (1.0f - al) + v[5] * al;
@@ -184,11 +188,14 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         const float ix = frac_index(lon, a.lon0, a.sx, nx);
         const float iy = frac_index(lat, a.lat0, a.sy, ny);
 
-        // aux record at z_eval: FT pair, density pair, meander pair.
+        // support record at z_eval: FT pair, density pair (and, bfloat16, the
+        // meander pair; float32 samples its plane after the substeps, when
+        // it is needed, so its loads are not in flight with these)
         float ax[6];
+        const float iz_eval = level_index(lev, nz, a.ltop, z_eval);
         {
-            const Corner3 c = corners3(ix, iy, level_index(lev, nz, z_eval), nx, ny, nz, Rec8<BF16>::kBytes);
-            sample_rec<BF16, 6>(x.aux, c, ax);
+            const Corner3 c = corners3(ix, iy, iz_eval, nx, ny, nz, WinRec<BF16>::kCellUnit);
+            sample_aux_a<BF16>(x.aux, c, ax);
         }
 
         // Column turbulence at z_eval (hanna.py:1500-1596).
@@ -225,10 +232,8 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
             float sw = in_bl_sw(zq, sf);
             if (sl_over && zq < 0.1f * sf.blh) sw = surface_layer_sw(zq, sf.ustar, sf.L);
             if (zq > sf.blh) {
-                const Corner3 c = corners3(ix, iy, level_index(lev, nz, zq), nx, ny, nz, Rec8<BF16>::kBytes);
-                float f0, f1;
-                sample_rec2<BF16>(x.aux, c, f0, f1);
-                sw = f0;
+                const Corner3 c = corners3(ix, iy, level_index(lev, nz, a.ltop, zq), nx, ny, nz, WinRec<BF16>::kCellUnit);
+                sw = sample_sigma_ft<BF16>(x.aux, c);
             }
             return fmaxf(sw, kSigmaMin);
         };
@@ -246,8 +251,9 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         // Substep schedule (hanna.py:1162-1186).
         // IEEE division: the ratio sits exactly on integers for the T_L floors.
         const float ratio = dt / (static_cast<float>(p.substep_c) * fmaxf(t.tw, 1e-3f));
-        int k = p.max_substeps;
-        if (ratio < static_cast<float>(p.max_substeps)) k = max(static_cast<int>(ceilf(ratio)), 1);
+        // (a device table row's max_substeps below 1 still takes one substep)
+        int k = max(p.max_substeps, 1);
+        if (ratio < static_cast<float>(k)) k = max(static_cast<int>(ceilf(ratio)), 1);
         const float sdt = dt / static_cast<float>(k);
 
         const float su2 = t.su * t.su, sv2 = t.sv * t.sv, sw2 = t.sw * t.sw;
@@ -255,17 +261,21 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         const uint32_t uid = s.uid ? ldo(s.uid, i4) : static_cast<uint32_t>(i);
         const uint32_t k0 = static_cast<uint32_t>(p.seed), k1 = static_cast<uint32_t>(p.seed >> 32);
         const uint32_t step = static_cast<uint32_t>(p.step_index);
+        // sub_dt, sigma and T_L hold for every substep (hanna.py:1300-1320), so
+        // the OU decay and noise amplitude are formed once per particle.
+        const OuCoef cu = ou_coef(t.tu, su2, sdt), cv = ou_coef(t.tv, sv2, sdt), cw = ou_coef(t.tw, sw2, sdt);
+        const float inv_sw_sq = __builtin_amdgcn_rcpf(sw_sq);
         for (int q = 0; q < k; ++q) {
-            const U4 r = philox4x32_10(uid, step, static_cast<uint32_t>(q), 0u, k0, k1);
+            const U4 r = philox4x32_10(uid, step, static_cast<uint32_t>(q), x.noise_hi, k0, k1);
             float n0, n1, n2, n3;
             box_muller(u01(r.x), u01(r.y), n0, n1);
             box_muller(u01(r.z), u01(r.w), n2, n3);
             (void)n3;
-            const float factor = 1.0f + fdiv(wp * wp, sw_sq);
+            const float factor = 1.0f + (wp * wp) * inv_sw_sq;
             const float drift = factor * half_b + dens_b;
-            float un = ou(up, t.tu, su2, sdt, n0, 0.0f);
-            float vn = ou(vp, t.tv, sv2, sdt, n1, 0.0f);
-            float wn = ou(wp, t.tw, sw2, sdt, n2, drift * sdt);
+            float un = ou_apply(up, cu, n0, 0.0f);
+            float vn = ou_apply(vp, cv, n1, 0.0f);
+            float wn = ou_apply(wp, cw, n2, drift * sdt);
             wn = fminf(fmaxf(wn, -wmax), wmax);
             un = fminf(fmaxf(un, -umax), umax);
             vn = fminf(fmaxf(vn, -vmax), vmax);
@@ -283,12 +293,18 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         }
 
         if (meander) {
-            const U4 r = philox4x32_10(uid, step, kStreamMeander, 0u, k0, k1);
+            if constexpr (!BF16) {
+                const Corner3 c = corners3(ix, iy, iz_eval, nx, ny, nz, WinRec<BF16>::kCellUnit);
+                sample_aux_b(x.aux, a.planeb, c, ax[4], ax[5]);
+            }
+            const U4 r = philox4x32_10(uid, step, kStreamMeander, x.noise_hi, k0, k1);
             float n0, n1;
             box_muller(u01(r.x), u01(r.y), n0, n1);
             const float tm = static_cast<float>(p.meander_timescale_s);
-            um = ou(um, tm, ax[4] * ax[4], dt, n0, 0.0f);
-            vm = ou(vm, tm, ax[5] * ax[5], dt, n1, 0.0f);
+            const OuCoef mu = ou_coef(tm, ax[4] * ax[4], dt);
+            const OuCoef mv{mu.a, fsqrt(fmaxf(ax[5] * ax[5] * (1.0f - mu.a * mu.a), 0.0f))};  // same decay
+            um = ou_apply(um, mu, n0, 0.0f);
+            vm = ou_apply(vm, mv, n1, 0.0f);
             const float ml = m_per_deg_lon(lat);
             lon = lon + fdiv((-um) * dt, ml);
             lat = lat + (-vm) * dt * (1.0f / kMPerDegLat);
@@ -352,11 +368,11 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // scalar registers; the spare ones spill to vector lanes).
 template <bool BF16, int MODE, bool FIXED>
 __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
-    __shared__ float lev[kMaxLevels];
-    for (int k = threadIdx.x; k < a.nz; k += blockDim.x) lev[k] = a.levels[k];
+    __shared__ float lev[kLadder];
+    fill_ladder(lev, a.levels, a.nz);
     __syncthreads();
     GlideStepParams p = a.p;
-    StepCtx x{a.wind, a.aux, a.surf, a.e.escaped, -1};
+    StepCtx x{a.wind, a.aux, a.surf, a.e.escaped, -1, 0u};
     unsigned long long* n_active = a.n_active;
     const int64_t i =
         static_cast<int64_t>(MODE == kModeSlots ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * blockDim.x +
@@ -401,6 +417,7 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
                 x.surf = static_cast<const char*>(a.ring.surf) + h * a.ring.surf_stride;
                 x.escaped = &sl->escaped;
                 x.slot = slot;
+                x.noise_hi = __builtin_amdgcn_readfirstlane(sl->noise_hi);
                 n_active = &sl->active_steps;
             }
         }
@@ -469,6 +486,14 @@ __global__ void footprint_key_kernel(int64_t n, int64_t ps, const float* lon, co
     vals[i] = (k == g.n_keys) ? 0.0f : weight[i] * dt;
 }
 
+// Table-mode step over no particles: the cursor advances by one when it
+// names a row, as a step kernel's last workgroup would advance it.
+__global__ void row_advance_kernel(int32_t* row, int32_t n_rows) {
+    if (threadIdx.x != 0) return;
+    const int32_t r = row[0];
+    if (r >= 0 && r < n_rows) row[0] = r + 1;
+}
+
 static int fill_grid(StepArgs& a, const GlideGrid& g, const GlideWindow& w) {
     GLIDE_REQUIRE(g.nx >= 2 && g.ny >= 2 && g.nz >= 2, "met grid needs >= 2 points per axis");
     GLIDE_REQUIRE(g.nz <= kMaxLevels, "at most %d met levels supported (got %d)", kMaxLevels, g.nz);
@@ -480,6 +505,8 @@ static int fill_grid(StepArgs& a, const GlideGrid& g, const GlideWindow& w) {
     a.nx = g.nx;
     a.ny = g.ny;
     a.nz = g.nz;
+    a.ltop = ladder_top(g.nz);
+    a.planeb = static_cast<uint32_t>(static_cast<uint64_t>(g.nz) * g.ny * g.nx * 16u);
     a.lon0 = static_cast<float>(g.lon0);
     a.lat0 = static_cast<float>(g.lat0);
     a.sx = static_cast<float>((g.nx - 1) / (g.lon1 - g.lon0));
@@ -557,10 +584,18 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
         a.fp = make_fp(*fg);
         a.e = *e;
     }
-    if (s->n == 0) return GLIDE_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (s->n == 0) {
+        // No particle to step, but a table-mode call still advances its cursor
+        // once (a row in range), so the replays after it read the right rows.
+        if (table != nullptr) {
+            hipLaunchKernelGGL(row_advance_kernel, dim3(1), dim3(64), 0, st, row, n_rows);
+            GLIDE_HIP(hipGetLastError());
+        }
+        return GLIDE_OK;
+    }
     const int64_t blocks = (s->n + bs - 1) / bs;
     GLIDE_REQUIRE(blocks < (1ll << 31), "too many particles");
-    hipStream_t st = static_cast<hipStream_t>(stream);
     const dim3 grid(static_cast<unsigned>(blocks)), block(static_cast<unsigned>(bs));
     const bool bf16 = w->dtype == GLIDE_MET_BF16;
     const bool fixed = (p->flags & (GLIDE_F_EMIT | GLIDE_F_FIXED)) == (GLIDE_F_EMIT | GLIDE_F_FIXED);

@@ -129,9 +129,8 @@ class DeviceMet:
             first = max(hours) if ingest.direction < 0 else min(hours)
             self.zp = ingest.get(first)["src3"].shape[1]
         ncell = self.nz * self.ny * self.nx
-        esize = 2 if met_dtype == L.GLIDE_MET_BF16 else 4
-        self.wind = torch.empty(ncell * 8 * esize, dtype=torch.uint8, device=device)
-        self.aux = torch.empty(ncell * 8 * esize, dtype=torch.uint8, device=device)
+        self.wind = torch.empty(L.window_bytes(ncell, met_dtype), dtype=torch.uint8, device=device)
+        self.aux = torch.empty(L.window_bytes(ncell, met_dtype), dtype=torch.uint8, device=device)
         self.surf = torch.empty(self.ny * self.nx * 12 * 4, dtype=torch.uint8, device=device)  # float in both
         self.ring = {}
         self.window_hour = None
@@ -365,6 +364,8 @@ class BatchEngine:
         # schedule step (the flags of a plain emitting step), and the device
         # cursor that the graph advances.
         flags = self.h.flags() | L.F_EMIT
+        if self.h.max_substeps < 1 or self.h.substep_c <= 0 or any(st.delta_s <= 0 for st in self.sched):
+            raise ValueError("step table rows need max_substeps >= 1, substep_c > 0 and dt > 0")
         rows = (L.GlideStepParams * max(len(self.sched), 1))(*[self.step_params(st, flags) for st in self.sched])
         self.step_table = torch.frombuffer(bytearray(bytes(rows)), dtype=torch.uint8).to(d)
         self.step_tmpl = self.step_params(self.sched[0], flags) if self.sched else None

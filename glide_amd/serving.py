@@ -42,6 +42,21 @@ from glide_amd.synthetic import SyntheticMet
 BLOCK = 64  # particles per wavefront; a slot owns whole blocks
 
 
+def served_noise_hi(req_id: int, n_particles: int) -> int:
+    """Philox counter word 3 of a served request's particles. A request of at
+    most 2^20 particles has ids (req_id % 4096) << 20 | k, so word 3 carries
+    the rest of its id (req_id >> 12); a larger request has ids k and word 3
+    is req_id with the top bit set. Two requests served together therefore
+    never share a (particle id, word 3) pair, and a request with req_id <
+    4096 draws the noise of the same release in a batch."""
+
+    if req_id < 0 or req_id >= (1 << 31):
+        raise ValueError("req_id must be in [0, 2^31)")
+    if n_particles <= (1 << 20):
+        return int(req_id) >> 12
+    return (1 << 31) | int(req_id)
+
+
 @dataclass
 class Request:
     """One footprint request (a point release)."""
@@ -90,9 +105,8 @@ class MetRing:
         self.dm = dm
         self.nx, self.ny, self.nz = dm.nx, dm.ny, dm.nz
         ncell = self.nz * self.ny * self.nx
-        esize = 2 if met_dtype == L.GLIDE_MET_BF16 else 4
-        self.wind_stride = ncell * 8 * esize
-        self.aux_stride = ncell * 8 * esize
+        self.wind_stride = L.window_bytes(ncell, met_dtype)
+        self.aux_stride = L.window_bytes(ncell, met_dtype)
         self.surf_stride = self.ny * self.nx * 12 * 4
         self.wind = torch.empty(n_hours * self.wind_stride, dtype=torch.uint8, device=device)
         self.aux = torch.empty(n_hours * self.aux_stride, dtype=torch.uint8, device=device)
@@ -372,6 +386,7 @@ class Server:
         uid = np.zeros(m, np.uint32)
         uid[:n] = ((np.uint32(req.req_id % 4096) << np.uint32(20)) + np.arange(n, dtype=np.uint32)) \
             if n <= (1 << 20) else np.arange(n, dtype=np.uint32)
+        noise_hi = served_noise_hi(req.req_id, n)
         alive = np.zeros(m, np.uint8)
         alive[:n] = 1
         host = dict(lon=np.full(m, req.lon, np.float32), lat=np.full(m, req.lat, np.float32),
@@ -385,7 +400,7 @@ class Server:
         # slot entry
         end = float(req.release_time_s + req.duration_s)
         sl = L.GlideSlot(t=end, term=end - float(req.length_s), rel_time=float(req.release_time_s),
-                         state=L.GLIDE_SLOT_RUNNING, k=0)
+                         state=L.GLIDE_SLOT_RUNNING, k=0, noise_hi=noise_hi)
         raw = np.frombuffer(bytes(sl), dtype=np.uint8)
         self.slots[slot * self.slot_bytes:(slot + 1) * self.slot_bytes].copy_(put(raw), non_blocking=True)
         self.wave_slot_h[b0:b0 + nb] = slot

@@ -70,9 +70,8 @@ class TorchPlain:
     def _window(self, stream: int) -> None:
         m = self.eng.met
         nz, ny, nx = m.nz, m.ny, m.nx
-        # column-major records [ny][nx][nz] -> [nz][ny][nx] fields
-        wind = m.wind.view(torch.float32).view(ny, nx, nz, 8).permute(2, 0, 1, 3)
-        aux = m.aux.view(torch.float32).view(ny, nx, nz, 8).permute(2, 0, 1, 3)
+        # packed column-major window -> [nz][ny][nx] fields
+        wind, aux = L.unpack_window(m.wind, m.aux, nz, ny, nx, m.met_dtype)
         surf = m.surf.view(torch.float32).view(ny, nx, 12)
         self.uvw = wind[..., :6].permute(3, 0, 1, 2).unsqueeze(0).contiguous()  # [1, 6, Z, Y, X]
         self.ft = aux[..., 0:2].permute(3, 0, 1, 2).unsqueeze(0).contiguous()

@@ -203,12 +203,11 @@ class HannaScheme(TurbulenceScheme):
         s2, e2 = stack2(mw.hour_start), stack2(mw.hour_end)
         p = torch.as_tensor(np.asarray(md.pressure_level_hpa, np.float64), device=device)
         lev = torch.as_tensor(level.astype(np.float32), device=device)
-        es = 2 if self.met_dtype == L.GLIDE_MET_BF16 else 4
         if self._win is not None and self._win["shape"] == (Z, Y, X) and self._win["device"] == device:
             wind, aux, surf = self._win["wind"], self._win["aux"], self._win["surf"]  # address-stable
         else:
-            wind = torch.empty(Z * Y * X * 8 * es, dtype=torch.uint8, device=device)
-            aux = torch.empty(Z * Y * X * 8 * es, dtype=torch.uint8, device=device)
+            wind = torch.empty(L.window_bytes(Z * Y * X, self.met_dtype), dtype=torch.uint8, device=device)
+            aux = torch.empty(L.window_bytes(Z * Y * X, self.met_dtype), dtype=torch.uint8, device=device)
             surf = torch.empty(Y * X * 12 * 4, dtype=torch.uint8, device=device)  # float in both layouts
         L.check(L.lib().glide_build_window(
             s3.data_ptr(), e3.data_ptr(), s2.data_ptr(), e2.data_ptr(), p.data_ptr(), lev.data_ptr(), Z, Y, X,

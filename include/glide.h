@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define GLIDE_ABI_VERSION 5
+#define GLIDE_ABI_VERSION 6
 
 #define GLIDE_OK 0
 #define GLIDE_ERR_INVALID (-1)
@@ -85,14 +85,23 @@ typedef struct GlideGrid {
 } GlideGrid;
 
 /*
- * One packed met window on the device (built by glide_build_window).
- *   wind: [ny][nx][nz] records of 8: u_s v_s w_s u_e v_e w_e 0 0
- *   aux : [ny][nx][nz] records of 8: sigma_w_ft T_Lw_ft rho drho_dz sigma_mu sigma_mv 0 0
- *   (column-major: a column's levels are contiguous, so particles binned by
- *   column gather from one short run of memory whatever their levels)
- *   surf: [ny][nx] records of 12: blh sp ustar shf t0 (hour start), same (hour end), 0 0
- * wind and aux records hold float (GLIDE_MET_F32) or bfloat16
- * (GLIDE_MET_BF16) values; surf records are float in both.
+ * One packed met window on the device (built by glide_build_window). Cells
+ * are column-major, cell = (y * nx + x) * nz + level: a column's levels are
+ * contiguous, so particles binned by column gather from one short run of
+ * memory whatever their levels.
+ *   GLIDE_MET_BF16: one 16 B record of 8 bfloat16 per cell,
+ *     wind: u_s v_s w_s u_e v_e w_e 0 0
+ *     aux : sigma_w_ft T_Lw_ft rho drho_dz sigma_mu sigma_mv 0 0
+ *   GLIDE_MET_F32: two float planes over the cell index, plane A (16 B per
+ *     cell) then plane B (8 B per cell, from byte ncell * 16), 24 B per cell:
+ *     wind: A = u_s v_s u_e v_e, B = w_s w_e
+ *     aux : A = sigma_w_ft T_Lw_ft rho drho_dz, B = sigma_mu sigma_mv
+ *     (a gather's 16 B and 8 B entries of one column's levels share fewer
+ *     cache lines than 32 B records would)
+ *   surf: [ny][nx] records of 12 float: blh sp ustar shf t0 (hour start), same
+ *     (hour end), 0 0, in both precisions.
+ * Buffer sizes: wind and aux ncell * 16 B (bfloat16) or ncell * 24 B (float),
+ * surf ny * nx * 48 B.
  */
 typedef struct GlideWindow {
     const void* wind;
@@ -232,7 +241,10 @@ int glide_step(const GlideStepParams* p, const GlideParticles* s, const GlideGri
  * device int32: the cursor, then finished-workgroup counters that must be 0
  * at launch (they are left 0; glide_row_set sets them). tmpl is a host row
  * whose flags every step takes (the rows' flags fields are not read); the
- * argument checks use it. */
+ * argument checks use it, so the rows must meet the same constraints as tmpl
+ * (dt >= 0, substep_c > 0, max_substeps >= 1; a row with max_substeps below 1
+ * takes one substep). A call over no particles (s->n == 0) still advances an
+ * in-range cursor by one. */
 #define GLIDE_ROW_WORDS 66
 int glide_step_table(const GlideStepParams* tmpl, const GlideStepParams* table, int32_t n_rows, int32_t* row,
                      const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
@@ -455,6 +467,9 @@ typedef struct GlideSlot {
     double rel_time;     /* release time: offsets are relative to it */
     int32_t state;
     int32_t k;           /* steps taken */
+    uint32_t noise_hi;   /* Philox counter word 3 of the slot's particles (0 elsewhere): with
+                            the particle ids it keeps concurrent requests' noise apart */
+    int32_t reserved;
     /* per step (glide_slots_advance) */
     double last_d;       /* length of the step taken last */
     float cursor_off, alpha, dt;

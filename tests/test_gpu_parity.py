@@ -76,9 +76,8 @@ def test_window_fields_match_oracle(cfg0):
     dm.build_window(0, _stream())
     torch.cuda.synchronize()
     nz, ny, nx = dm.nz, dm.ny, dm.nx
-    # column-major records [ny][nx][nz]
-    wind = dm.wind.view(torch.float32).cpu().numpy().reshape(ny, nx, nz, 8).transpose(2, 0, 1, 3)
-    aux = dm.aux.view(torch.float32).cpu().numpy().reshape(ny, nx, nz, 8).transpose(2, 0, 1, 3)
+    # packed column-major window (float32 planes) -> [nz][ny][nx][channel]
+    wind, aux = (t.cpu().numpy() for t in L.unpack_window(dm.wind, dm.aux, nz, ny, nx, dm.met_dtype))
     surf = dm.surf.view(torch.float32).cpu().numpy().reshape(ny, nx, 12)
     h0, h1 = _oracle_hour(met, 0), _oracle_hour(met, 1)
     win = ORUN.build_window(h0, h1, met.spec.agl_levels, met.lon, met.lat, np.float64)
