@@ -36,6 +36,7 @@ GLIDE_SLOT_FREE = 0
 GLIDE_SLOT_RUNNING = 1
 GLIDE_SLOT_FINISHED = 2
 GLIDE_ROW_WORDS = 66  # int32 words of the step cursor buffer (glide_step_table)
+GLIDE_NOALLOC_MAX_PAIRS = 1 << 20  # pairs glide_footprint_accumulate_noalloc serves
 
 vp = C.c_void_p
 
@@ -134,6 +135,7 @@ SIGNATURES = {
     "glide_footprint_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "glide_footprint_accumulate": (C.c_int, [vp, vp, C.c_int64, C.c_uint32, C.c_uint32, vp, vp, vp, vp,
                                              C.c_size_t, vp]),
+    "glide_footprint_accumulate_noalloc": (C.c_int, [vp, vp, C.c_int64, C.c_uint32, C.c_uint32, vp, vp, vp, vp]),
     "glide_sort_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "glide_sort_pairs": (C.c_int, [vp, vp, C.c_int64, C.c_int32, vp, vp, vp, C.c_size_t, vp]),
     "glide_reorder_keys": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), C.c_double, C.c_int32,

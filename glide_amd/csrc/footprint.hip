@@ -17,6 +17,8 @@
 // launched for the upper bound and tiles past the count exit.
 #include "sort.h"
 
+#include <mutex>
+
 namespace glide {
 
 constexpr int kRedThreads = 256;
@@ -391,4 +393,61 @@ static int accumulate(const uint32_t* keys, const float* vals, int64_t n, uint32
     hipLaunchKernelGGL(seg_fixup_kernel, dim3(rel ? 2u : 1u), dim3(kScanThreads), 0, st, sw.count, ct, rt, wc, wr);
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// glide_footprint_accumulate_noalloc: the same accumulation with its
+// workspace in a file-scope device array, for callers that cannot allocate.
+// Each GPU holds its own copy of the array (at its own address) until the
+// process ends; every call on a GPU uses that GPU's copy, so calls that
+// overlap on one GPU race. Nothing in it must start at zero: every pass writes
+// what the next one reads (the same contract as a caller's fresh workspace).
+// ---------------------------------------------------------------------------
+namespace glide {
+
+constexpr size_t kNoallocScratchBytes = 19u << 20;  // footprint_layout(GLIDE_NOALLOC_MAX_PAIRS) = 18.03 MiB
+
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+constexpr int kNoallocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kNoallocMaxDevices] = {};
+
+// Device address of the current device's scratch array: resolved once per
+// device (with that device current) and kept. The array's name is a host-side
+// stand-in in host code, never a device address.
+static int noalloc_scratch(void** out) {
+    int dev = 0;
+    GLIDE_HIP(hipGetDevice(&dev));
+    GLIDE_REQUIRE(dev >= 0 && dev < kNoallocMaxDevices, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) return check_hip(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                              "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GLIDE_OK;
+}
+
+}  // namespace glide
+
+extern "C" int glide_footprint_accumulate_noalloc(const uint32_t* keys, const float* vals, int64_t n,
+                                                  uint32_t n_cells, uint32_t cells_per_release, float* slab,
+                                                  float* comp, double* release_total, void* stream) {
+    GLIDE_REQUIRE(n >= 0, "n must be >= 0");
+    GLIDE_REQUIRE(n <= GLIDE_NOALLOC_MAX_PAIRS,
+                  "glide_footprint_accumulate_noalloc: %lld pairs exceed the static workspace's maximum of %d",
+                  static_cast<long long>(n), GLIDE_NOALLOC_MAX_PAIRS);
+    const size_t need = footprint_layout(n, nullptr, nullptr, nullptr, nullptr);
+    GLIDE_REQUIRE(need <= kNoallocScratchBytes, "static workspace too small: need %zu bytes, have %zu", need,
+                  kNoallocScratchBytes);
+    if (n == 0) return GLIDE_OK;
+    void* ws = nullptr;
+    const int rc = noalloc_scratch(&ws);
+    if (rc != GLIDE_OK) return rc;
+    return accumulate(keys, vals, n, n_cells, cells_per_release, nullptr, 1u, slab, comp, release_total, ws,
+                      kNoallocScratchBytes, stream);
 }

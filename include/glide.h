@@ -296,6 +296,20 @@ int glide_footprint_accumulate(const uint32_t* keys, const float* vals, int64_t 
                                float* comp, double* release_total, void* workspace,
                                size_t workspace_bytes, void* stream);
 
+/* glide_footprint_accumulate without a caller workspace and without any
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The workspace is a static device array of
+ * the library (19 MiB per GPU, held until the process ends) that serves up to
+ * GLIDE_NOALLOC_MAX_PAIRS pairs; larger n is rejected (GLIDE_ERR_INVALID).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (concurrent streams or threads) race. Same results as
+ * glide_footprint_accumulate, bit for bit. */
+#define GLIDE_NOALLOC_MAX_PAIRS (1 << 20)
+int glide_footprint_accumulate_noalloc(const uint32_t* keys, const float* vals, int64_t n, uint32_t n_cells,
+                                       uint32_t cells_per_release, float* slab, float* comp,
+                                       double* release_total, void* stream);
+
 /* glide_footprint_accumulate for release-binned keys (GlideFootprintGrid.
  * t_bin): key k < n_r * cells_per_bin belongs to release r = k / cells_per_bin
  * and adds to slab cell (r * n_t + t_bin[r]) * cells_per_bin + k % cells_per_bin
