@@ -31,6 +31,7 @@ F_MASK_INPUT = 0x10
 F_NO_TURB = 0x20
 F_RELEASE_TERMINUS = 0x40
 F_FIXED = 0x80
+F_PRESCRIBED = 0x100
 
 GLIDE_SLOT_FREE = 0
 GLIDE_SLOT_RUNNING = 1
@@ -150,6 +151,8 @@ SIGNATURES = {
                                         C.c_double, C.c_double, C.c_double, C.c_double, vp, vp, vp, vp, vp]),
     "glide_convect": (C.c_int, [P(GlideStepParams), P(GlideParticles), P(GlideGrid), vp, vp, vp, vp, vp, vp,
                                 P(GlideFootprintGrid), P(GlideEmit), vp]),
+    "glide_hanna_eval": (C.c_int, [C.c_int32, C.c_int64, vp, vp, vp]),
+    "glide_windowed_std": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp]),
     "glide_graph_begin": (C.c_int, [vp]),
     "glide_graph_end": (C.c_int, [vp, P(vp)]),
     "glide_graph_launch": (C.c_int, [vp, vp]),

@@ -1,6 +1,6 @@
 // Hourly met kernels: terrain-following resample (one thread per column and
 // target level, coalesced along x) and the per-window packed-record build.
-#include "common.h"
+#include "closure.h"
 
 namespace glide {
 
@@ -190,71 +190,7 @@ struct WindowArgs {
     void* surf;
 };
 
-constexpr float kKappa = 287.05f / 1005.0f;
-
 __device__ __forceinline__ float safe_den(float d) { return fabsf(d) < 1e-6f ? 1e-6f : d; }
-
-// Two-pass population std of 0.5 * (s + e) over the edge-clipped stencil.
-__device__ float local_std(const float* __restrict__ s, const float* __restrict__ e, int y, int x, int ny,
-                           int nx, int r) {
-    if (r == 1) {
-        // the shipped 3 x 3 stencil: the midpoints stay in registers between
-        // the passes (the same values, added in the same order)
-        float v[9];
-        bool in[9];
-        int cnt = 0;
-        float sum = 0.0f;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            const int yy = y + q / 3 - 1, xx = x + q % 3 - 1;
-            in[q] = yy >= 0 && yy < ny && xx >= 0 && xx < nx;
-            v[q] = 0.0f;
-            if (in[q]) {
-                const int64_t c = static_cast<int64_t>(yy) * nx + xx;
-                v[q] = 0.5f * (s[c] + e[c]);
-                sum += v[q];
-                ++cnt;
-            }
-        }
-        const float mean = sum / static_cast<float>(cnt);
-        float ss = 0.0f;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            if (in[q]) {
-                const float d = v[q] - mean;
-                ss += d * d;
-            }
-        }
-        return sqrtf(ss / static_cast<float>(cnt));
-    }
-    float sum = 0.0f;
-    int cnt = 0;
-    for (int dy = -r; dy <= r; ++dy) {
-        const int yy = y + dy;
-        if (yy < 0 || yy >= ny) continue;
-        for (int dx = -r; dx <= r; ++dx) {
-            const int xx = x + dx;
-            if (xx < 0 || xx >= nx) continue;
-            const int64_t c = static_cast<int64_t>(yy) * nx + xx;
-            sum += 0.5f * (s[c] + e[c]);
-            ++cnt;
-        }
-    }
-    const float mean = sum / static_cast<float>(cnt);
-    float ss = 0.0f;
-    for (int dy = -r; dy <= r; ++dy) {
-        const int yy = y + dy;
-        if (yy < 0 || yy >= ny) continue;
-        for (int dx = -r; dx <= r; ++dx) {
-            const int xx = x + dx;
-            if (xx < 0 || xx >= nx) continue;
-            const int64_t c = static_cast<int64_t>(yy) * nx + xx;
-            const float d = 0.5f * (s[c] + e[c]) - mean;
-            ss += d * d;
-        }
-    }
-    return sqrtf(ss / static_cast<float>(cnt));
-}
 
 // Levels per workgroup of the window build: one 128 B run of records per
 // column (bfloat16: 8 records of 16 B; float32: 8 plane-A entries of 16 B, and
@@ -290,7 +226,7 @@ __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
     auto pres = [&](int zz) { return static_cast<float>(a.p_hpa[zz]) * 100.0f; };
     auto theta = [&](int zz) {
         const float tm = 0.5f * (at(a.s3, 3, zz) + at(a.e3, 3, zz));
-        return tm * powf(100000.0f / pres(zz), kKappa);
+        return potential_temperature(tm, pres(zz));
     };
     auto umid = [&](int zz) { return 0.5f * (at(a.s3, 0, zz) + at(a.e3, 0, zz)); };
     auto vmid = [&](int zz) { return 0.5f * (at(a.s3, 1, zz) + at(a.e3, 1, zz)); };
@@ -308,22 +244,13 @@ __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
         const float du = (umid(zh) - umid(zl)) / dh;
         const float dv = (vmid(zh) - vmid(zl)) / dh;
         const float drho = (rho(zh) - rho(zl)) / dh;
-        const float n2 = (9.80665f / fmaxf(th, 1.0f)) * dth;
+        const float n2 = brunt_vaisala_squared(th, dth);
         const float shear_sq = du * du + dv * dv;
         const float shear_mag = sqrtf(fmaxf(shear_sq, 1e-8f));
-        const float ri = n2 / fmaxf(shear_sq, 1e-8f);
-        const float hz = fmaxf(a.lev[z], 1.0f);
-        const float ml = (0.4f * hz) / (1.0f + 0.4f * hz / 100.0f);
-        float fri;
-        if (ri < 0.0f) {
-            fri = sqrtf(fmaxf(1.0f - 16.0f * ri, 0.0f));
-        } else {
-            const float q = fmaxf(1.0f - ri / 0.25f, 0.0f);
-            fri = q * q;
-        }
-        const float kz = fminf(fmaxf(ml * ml * shear_mag * fri, 0.1f), 50.0f);
-        const float tl = (n2 > 1e-6f) ? fminf(fmaxf(0.5f / sqrtf(fmaxf(n2, 1e-6f)), 1.0f), 1000.0f) : 100.0f;
-        const float sw = sqrtf(fmaxf(kz / tl, 1e-6f));
+        const float ri = gradient_richardson(n2, shear_sq);
+        const float kz = free_trop_diffusivity(a.lev[z], shear_mag, ri);
+        float sw, tl;
+        free_trop_sigma_tl(kz, n2, sw, tl);
         const int64_t zoff = static_cast<int64_t>(z) * ncol;
         const float smu = a.coeff * local_std(a.s3 + zoff, a.e3 + zoff, y, x, a.ny, a.nx, a.radius);
         const float smv = a.coeff * local_std(a.s3 + (static_cast<int64_t>(nz) + z) * ncol,

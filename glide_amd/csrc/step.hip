@@ -182,6 +182,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         sf.fcor = 2.0f * kOmega * fsin_deg(lat);
 
         const bool tl_floors = (p.flags & GLIDE_F_TL_FLOORS) != 0;
+        const bool prescribed = (p.flags & GLIDE_F_PRESCRIBED) != 0;  // the support record at every height
         const bool sl_over = (p.flags & GLIDE_F_SL_OVERRIDE) != 0;
         const float zu = static_cast<float>(p.z_ubl_m);
         const float z_eval = fmaxf(z, zu);
@@ -216,7 +217,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
             t.tv = fmaxf(t.tv, 10.0f);
             t.tw = fmaxf(t.tw, 30.0f);
         }
-        if (z_eval > sf.blh) {
+        if (z_eval > sf.blh || prescribed) {
             t.su = t.sv = t.sw = ax[0];
             t.tu = t.tv = t.tw = ax[1];
         }
@@ -231,7 +232,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         auto sw_probe = [&](float zq) {
             float sw = in_bl_sw(zq, sf);
             if (sl_over && zq < 0.1f * sf.blh) sw = surface_layer_sw(zq, sf.ustar, sf.L);
-            if (zq > sf.blh) {
+            if (zq > sf.blh || prescribed) {
                 const Corner3 c = corners3(ix, iy, level_index(lev, nz, a.ltop, zq), nx, ny, nz, WinRec<BF16>::kCellUnit);
                 sw = sample_sigma_ft<BF16>(x.aux, c);
             }

@@ -145,6 +145,14 @@ class HannaScheme(TurbulenceScheme):
         self._win = None
         self._uid = None
 
+    @staticmethod
+    def _windowed_std(field: torch.Tensor, radius: int) -> torch.Tensor:
+        """The meander stencil std (hanna.py:1786-1830), on the GPU."""
+
+        from glide_amd.hanna import windowed_std
+
+        return windowed_std(field, radius)
+
     def required_met_keys(self) -> tuple[str, ...]:
         return ("t", "ustar", "shf")
 

@@ -68,6 +68,11 @@ extern "C" {
                                          is stepped exactly as when it runs alone (no counterpart in the
                                          reference, whose batch terminus gives non-earliest releases one
                                          more step, main.py:1383) */
+#define GLIDE_F_PRESCRIBED 0x100   /* verification mode: sigma_u = sigma_v = sigma_w and T_L come from the
+                                      support record (sigma_w_ft, T_Lw_ft) at every height, the column
+                                      turbulence's free-troposphere branch taken everywhere; the engine-level
+                                      counterpart of driving the OU primitives with given (sigma_w, T_L)
+                                      (gpu_engine.py:345-507) */
 #define GLIDE_F_FIXED 0x80         /* with GLIDE_F_EMIT: add each particle's residence to a
                                       fixed-point accumulator in the step (GlideEmit.fixed) instead of
                                       writing keys/values; replaces the reference's scatter_add_
@@ -522,6 +527,43 @@ int glide_slots_advance(GlideSlot* slots, int32_t n_slots, double dt, void* stre
 int glide_step_slots(const GlideStepParams* tmpl, GlideSlot* slots, const int32_t* wave_slot,
                      const GlideParticles* s, const GlideGrid* g, const GlideMetRing* ring,
                      const GlideFootprintGrid* fg, const GlideEmit* e, void* stream);
+
+/*
+ * The Hanna (1982) / FLEXPART free functions element-wise over device float
+ * arrays, evaluated with the step kernel's own float code and the window
+ * build's closure (replaces the reference's free functions,
+ * turbulence/hanna.py:142-189, 192-337, 340-426, for their unit tests and for
+ * diagnostics). in/out are host arrays of device pointers (n floats each):
+ *   CORIOLIS            in: lat (deg)                    out: f
+ *   AIR_DENSITY         in: sp (Pa), t (K)               out: rho
+ *   OBUKHOV             in: ustar, shf, t, sp            out: L (+inf when |kappa g H| <= 1e-10)
+ *   CONVECTIVE_VELOCITY in: blh, shf, t, sp              out: w*
+ *   IN_BL               in: z, blh, ustar, w*, h/L, lat  out: sigma_u, sigma_v, sigma_w, T_Lu, T_Lv, T_Lw
+ *   SURFACE_LAYER_SIGMA_W in: z, ustar, L                out: sigma_w
+ *   POTENTIAL_TEMPERATURE in: t, p (Pa)                  out: theta
+ *   BRUNT_VAISALA       in: theta, dtheta/dz             out: N^2
+ *   RICHARDSON          in: N^2, S^2                     out: Ri
+ *   FT_DIFFUSIVITY      in: z, |S|, Ri                   out: K_z
+ *   FT_SIGMA_TL         in: K_z, N^2                     out: sigma_w, T_Lw
+ */
+#define GLIDE_HANNA_CORIOLIS 1
+#define GLIDE_HANNA_AIR_DENSITY 2
+#define GLIDE_HANNA_OBUKHOV 3
+#define GLIDE_HANNA_CONVECTIVE_VELOCITY 4
+#define GLIDE_HANNA_IN_BL 5
+#define GLIDE_HANNA_SURFACE_LAYER_SIGMA_W 6
+#define GLIDE_HANNA_POTENTIAL_TEMPERATURE 7
+#define GLIDE_HANNA_BRUNT_VAISALA 8
+#define GLIDE_HANNA_RICHARDSON 9
+#define GLIDE_HANNA_FT_DIFFUSIVITY 10
+#define GLIDE_HANNA_FT_SIGMA_TL 11
+int glide_hanna_eval(int32_t op, int64_t n, const float* const* in, float* const* out, void* stream);
+
+/* HannaScheme._windowed_std (hanna.py:1786-1830): out[k][j][i] = population
+ * std of field[k] over the edge-clipped (2r+1)^2 stencil around (j, i), the
+ * meander stencil of glide_build_window. field, out: device float[nz][ny][nx]. */
+int glide_windowed_std(const float* field, int32_t nz, int32_t ny, int32_t nx, int32_t radius, float* out,
+                       void* stream);
 
 /*
  * Step-body graph capture on a caller stream: glide_graph_begin starts stream

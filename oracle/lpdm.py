@@ -51,6 +51,9 @@ class HannaParams:
     z_ubl_m: float = 2.0
     flexpart_tl_floors: bool = True
     surface_layer_override: bool = False
+    # prescribed turbulence: the free-troposphere branch (sigma and T_L from the
+    # support stack) at every height (the engine's GLIDE_F_PRESCRIBED)
+    prescribed: bool = False
 
 
 @dataclass
@@ -328,7 +331,7 @@ def column_turbulence(z, blh, ustar, wstar, hol, L, lat, lon, win: Window, p: Ha
         tu = np.maximum(tu, T_LU_FLOOR)
         tv = np.maximum(tv, T_LV_FLOOR)
         tw = np.maximum(tw, T_LW_FLOOR)
-    above = z > blh
+    above = (z > blh) | bool(p.prescribed)
     ft = sample_stack(win.ft, lon, lat, z, win)
     su = np.where(above, ft[0], su)
     sv = np.where(above, ft[0], sv)
