@@ -13,7 +13,7 @@ from pathlib import Path
 _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libglide_hip.so"
 
-ABI_VERSION = 6  # GLIDE_ABI_VERSION of include/glide.h
+ABI_VERSION = 7  # GLIDE_ABI_VERSION of include/glide.h
 
 GLIDE_OK = 0
 GLIDE_ERR_INVALID = -1
@@ -72,6 +72,7 @@ class GlideFootprintGrid(C.Structure):
         ("n_x", C.c_int32), ("reserved", C.c_int32),
         ("dom_lon_min", C.c_double), ("dom_lon_max", C.c_double), ("dom_lat_min", C.c_double),
         ("dom_lat_max", C.c_double), ("dom_alt_max", C.c_double), ("t_bin", vp),
+        ("slab_off", vp), ("slab_nt", vp), ("slab_cells", C.c_int64),
     ]
 
 
@@ -126,6 +127,8 @@ SIGNATURES = {
     "glide_means_workspace_bytes": (C.c_size_t, []),
     "glide_fixed_workspace_bytes": (C.c_size_t, [C.c_int32]),
     "glide_fixed_finalize": (C.c_int, [vp, C.c_int32, C.c_int64, C.c_uint32, vp, vp, vp, vp, C.c_size_t, vp]),
+    "glide_fixed_finalize_slabs": (C.c_int, [vp, C.c_int32, C.c_int64, vp, vp, C.c_uint32, C.c_int32, vp, C.c_int32,
+                                             C.c_int32, vp, vp, vp, C.c_size_t, vp]),
     "glide_particle_means": (C.c_int, [P(GlideParticles), vp, C.c_int32, vp, vp, C.c_size_t, vp]),
     "glide_field_means": (C.c_int, [vp, C.c_int64, C.c_int32, vp, vp, C.c_size_t, vp]),
     "glide_slots_advance": (C.c_int, [vp, C.c_int32, C.c_double, vp]),

@@ -294,7 +294,7 @@ __global__ void cloud_prepare_kernel(const int32_t* i_lcl, const int32_t* i_lnb,
 // fixed-point value go to fkey/fval (the caller's dropped key and 0 stay for an
 // inactive particle) instead of memory.
 __device__ __forceinline__ void convect_particle(const ConvectArgs& a, const float* lev, const float* lm,
-                                                 int64_t i, uint32_t& fkey, long long& fval) {
+                                                 int64_t i, uint64_t& fkey, long long& fval) {
     const GlideParticles& s = a.s;
     const GlideStepParams& p = a.p;
     const bool emit = (p.flags & GLIDE_F_EMIT) != 0;
@@ -388,13 +388,12 @@ __device__ __forceinline__ void convect_particle(const ConvectArgs& a, const flo
     if (emit) {
         const int64_t tb = time_bin(s.we_off[i], co, a.fp.n_t);
         const int64_t rb = s.rel_idx ? s.rel_idx[i] : 0;
-        const uint32_t key = footprint_key(a.fp, true, lon, lat, z, tb, rb);
         const float val = s.weight[i] * static_cast<float>(p.dt);
         if (fixed) {
-            fkey = key;
-            fval = key < a.fp.n_keys ? to_fixed(val, a.e.fixed_exp[rb]) : 0;
+            fkey = fixed_cell(a.fp, lon, lat, z, tb, rb);
+            fval = fkey != kNoCell ? to_fixed(val, a.e.fixed_exp[rb]) : 0;
         } else {
-            a.e.keys[i] = key;
+            a.e.keys[i] = footprint_key(a.fp, true, lon, lat, z, tb, rb);
             a.e.vals[i] = val;
         }
         const bool esc = !within_domain(a.fp, lon, lat, z);
@@ -412,11 +411,11 @@ __global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
     }
     __syncthreads();
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    uint32_t fkey = a.fp.n_keys;
+    uint64_t fkey = kNoCell;
     long long fval = 0;
     if (i < a.s.n) convect_particle(a, lev, lm, i, fkey, fval);
     if ((a.p.flags & (GLIDE_F_EMIT | GLIDE_F_FIXED)) == (GLIDE_F_EMIT | GLIDE_F_FIXED))
-        wave_fixed_add(fkey, fval, a.fp.n_keys, fixed_copy(a.e, a.fp.n_cells));  // every lane is here
+        wave_fixed_add(fkey, fval, fixed_copy(a.e, a.fp.acc_cells));  // every lane is here
 }
 
 // Mass conservation per column in float64: the float entries
@@ -604,7 +603,14 @@ extern "C" int glide_convect(const GlideStepParams* p, const GlideParticles* s, 
         }
         GLIDE_REQUIRE(s->we_off && s->weight && s->alive, "we_off/weight/alive needed to emit");
         const uint64_t cells = static_cast<uint64_t>(fg->n_r) * fg->n_t * fg->n_z * fg->n_y * fg->n_x;
-        GLIDE_REQUIRE(cells > 0 && cells < 0xffffffffull, "bad footprint size");
+        GLIDE_REQUIRE(cells > 0, "bad footprint size");
+        if (p->flags & GLIDE_F_FIXED) {
+            GLIDE_REQUIRE(fg->slab_off == nullptr || (fg->slab_nt != nullptr && fg->slab_cells > 0),
+                          "per-release slabs need slab_nt and slab_cells");
+        } else {
+            GLIDE_REQUIRE(cells < 0xffffffffull, "footprint has too many cells for 32-bit keys");
+            GLIDE_REQUIRE(fg->slab_off == nullptr, "per-release slabs need GLIDE_F_FIXED");
+        }
         a.fp = make_fp(*fg);
         a.e = *e;
     }

@@ -28,22 +28,42 @@ __device__ __forceinline__ long long wave_sum_i64(long long v) {
     return v;
 }
 
-// Block (bx, r): cells bx*256 + t (+ grid stride) of release r's slab.
-__global__ __launch_bounds__(kFinThreads) void fixed_finalize_kernel(const long long* fixed, int copies,
-                                                                      uint64_t n_cells, uint32_t cpr,
+// Where release r's cells lie: flat (r * cpr, cpr cells, out at the same
+// index) or per-release slabs (slab_off[r], slab_nt[r] * cpb cells, out from
+// slab_off[r] - slab_off[r0]: the range's slabs back to back).
+struct FinLayout {
+    uint64_t acc_cells;  // stride between the accumulator's copies
+    uint32_t cpr;        // flat: cells per release
+    const int64_t* slab_off;
+    const int32_t* slab_nt;
+    uint32_t cpb;        // slabs: cells per time bin
+    int32_t r0;          // first release of the range
+};
+
+// Block (bx, j): cells bx*256 + t (+ grid stride) of release r0 + j.
+__global__ __launch_bounds__(kFinThreads) void fixed_finalize_kernel(const long long* fixed, int copies, FinLayout L,
                                                                       const int32_t* exp, float* slab,
                                                                       long long* rel) {
     __shared__ long long part[kFinThreads / 64];
-    const uint32_t r = blockIdx.y;
+    const int32_t r = L.r0 + static_cast<int32_t>(blockIdx.y);
     const int e = exp[r];
-    const uint64_t base = static_cast<uint64_t>(r) * cpr;
+    uint64_t base, cnt, obase;
+    if (L.slab_off != nullptr) {
+        base = static_cast<uint64_t>(L.slab_off[r]);
+        cnt = static_cast<uint64_t>(L.slab_nt[r]) * L.cpb;
+        obase = base - static_cast<uint64_t>(L.slab_off[L.r0]);
+    } else {
+        base = static_cast<uint64_t>(r) * L.cpr;
+        cnt = L.cpr;
+        obase = base;
+    }
     long long sum = 0;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kFinThreads;
-    for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * kFinThreads + threadIdx.x; c < cpr; c += stride) {
+    for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * kFinThreads + threadIdx.x; c < cnt; c += stride) {
         long long v = 0;
-        for (int k = 0; k < copies; ++k) v += fixed[static_cast<uint64_t>(k) * n_cells + base + c];
+        for (int k = 0; k < copies; ++k) v += fixed[static_cast<uint64_t>(k) * L.acc_cells + base + c];
         sum += v;
-        slab[base + c] = ldexpf(static_cast<float>(v), -e);
+        slab[obase + c] = ldexpf(static_cast<float>(v), -e);
     }
     if (rel == nullptr) return;
     sum = wave_sum_i64(sum);
@@ -56,9 +76,14 @@ __global__ __launch_bounds__(kFinThreads) void fixed_finalize_kernel(const long 
     }
 }
 
-__global__ void fixed_totals_kernel(const long long* rel, const int32_t* exp, int n_r, double* out) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n_r) out[r] = ldexp(static_cast<double>(rel[r]), -exp[r]);
+__global__ void fixed_totals_kernel(const long long* rel, const int32_t* exp, int r0, int r1, double* out) {
+    const int r = r0 + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r < r1) out[r] = ldexp(static_cast<double>(rel[r]), -exp[r]);
+}
+
+__global__ void fixed_zero_range_kernel(long long* w, int r0, int r1) {
+    const int r = r0 + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r < r1) w[r] = 0;
 }
 
 }  // namespace glide
@@ -91,12 +116,46 @@ extern "C" int glide_fixed_finalize(const long long* fixed, int32_t copies, int6
     const uint64_t per_block = static_cast<uint64_t>(kFinThreads) * kFinItems;
     uint64_t bx = (cells_per_release + per_block - 1) / per_block;
     if (bx > 4096) bx = 4096;
+    const FinLayout lay{static_cast<uint64_t>(n_cells), cells_per_release, nullptr, nullptr, 0u, 0};
     hipLaunchKernelGGL(fixed_finalize_kernel, dim3(static_cast<unsigned>(bx), static_cast<unsigned>(n_r)),
-                       dim3(kFinThreads), 0, st, fixed, static_cast<int>(copies), static_cast<uint64_t>(n_cells),
-                       cells_per_release, fixed_exp, slab, rel);
+                       dim3(kFinThreads), 0, st, fixed, static_cast<int>(copies), lay, fixed_exp, slab, rel);
     if (release_total != nullptr) {
         hipLaunchKernelGGL(fixed_totals_kernel, dim3(static_cast<unsigned>((n_r + 255) / 256)), dim3(256), 0, st, rel,
-                           fixed_exp, static_cast<int>(n_r), release_total);
+                           fixed_exp, 0, static_cast<int>(n_r), release_total);
+    }
+    GLIDE_HIP(hipGetLastError());
+    return GLIDE_OK;
+}
+
+extern "C" int glide_fixed_finalize_slabs(const long long* fixed, int32_t copies, int64_t slab_cells,
+                                          const int64_t* slab_off, const int32_t* slab_nt, uint32_t cells_per_bin,
+                                          int32_t max_nt, const int32_t* fixed_exp, int32_t r0, int32_t r1, float* out,
+                                          double* release_total, void* workspace, size_t workspace_bytes,
+                                          void* stream) {
+    GLIDE_REQUIRE(fixed && slab_off && slab_nt && fixed_exp && out, "glide_fixed_finalize_slabs: NULL argument");
+    GLIDE_REQUIRE(slab_cells > 0, "bad slab_cells");
+    GLIDE_REQUIRE(copies >= 0 && copies <= 1024, "copies must be in [0, 1024]");
+    if (copies == 0) copies = 1;
+    GLIDE_REQUIRE(cells_per_bin > 0 && max_nt > 0, "bad slab geometry");
+    GLIDE_REQUIRE(r0 >= 0 && r1 >= r0 && r1 - r0 <= 65535, "bad release range [%d, %d)", r0, r1);
+    if (r1 == r0) return GLIDE_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    long long* rel = nullptr;
+    const unsigned nb = static_cast<unsigned>((r1 - r0 + 255) / 256);
+    if (release_total != nullptr) {
+        GLIDE_REQUIRE(workspace != nullptr && workspace_bytes >= glide_fixed_workspace_bytes(r1),
+                      "workspace too small: need %zu bytes", glide_fixed_workspace_bytes(r1));
+        rel = static_cast<long long*>(workspace);
+        hipLaunchKernelGGL(fixed_zero_range_kernel, dim3(nb), dim3(256), 0, st, rel, r0, r1);
+    }
+    const uint64_t per_block = static_cast<uint64_t>(kFinThreads) * kFinItems;
+    uint64_t bx = (static_cast<uint64_t>(max_nt) * cells_per_bin + per_block - 1) / per_block;
+    if (bx > 4096) bx = 4096;
+    const FinLayout lay{static_cast<uint64_t>(slab_cells), 0u, slab_off, slab_nt, cells_per_bin, r0};
+    hipLaunchKernelGGL(fixed_finalize_kernel, dim3(static_cast<unsigned>(bx), static_cast<unsigned>(r1 - r0)),
+                       dim3(kFinThreads), 0, st, fixed, static_cast<int>(copies), lay, fixed_exp, out, rel);
+    if (release_total != nullptr) {
+        hipLaunchKernelGGL(fixed_totals_kernel, dim3(nb), dim3(256), 0, st, rel, fixed_exp, r0, r1, release_total);
     }
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;

@@ -10,8 +10,11 @@ struct FpDev {
     float lon_min, lat_min, span_x, span_y;  // float32 values, as torch sees them
     float fnx, fny;
     int n_r, n_t, n_z, n_y, n_x;
-    uint32_t n_cells;  // flat cells n_r*n_t*n_z*n_y*n_x
+    uint32_t n_cells;  // flat cells n_r*n_t*n_z*n_y*n_x (the 32-bit key paths)
     uint32_t n_keys;   // key space; the dropped key (n_cells, or n_r*n_z*n_y*n_x when binned)
+    uint64_t acc_cells;  // cells of the fixed-point accumulator (the flat cells, or the slabs' total)
+    const int64_t* slab_off;  // per-release slab offsets (NULL: r * n_t * cells per bin)
+    const int32_t* slab_nt;   // per-release time bins (NULL: n_t)
     const float* z_edges;
     float dom_lon_min, dom_lon_max, dom_lat_min, dom_lat_max, dom_alt_max;
     int32_t* t_bin;  // release-binned keys: per-release time bin of this step (NULL: flat keys)
@@ -32,7 +35,11 @@ inline FpDev make_fp(const GlideFootprintGrid& g) {
     d.n_z = g.n_z;
     d.n_y = g.n_y;
     d.n_x = g.n_x;
-    d.n_cells = static_cast<uint32_t>(static_cast<uint64_t>(g.n_r) * g.n_t * g.n_z * g.n_y * g.n_x);
+    const uint64_t flat = static_cast<uint64_t>(g.n_r) * g.n_t * g.n_z * g.n_y * g.n_x;
+    d.n_cells = static_cast<uint32_t>(flat < 0xffffffffull ? flat : 0xffffffffull);
+    d.slab_off = g.slab_off;
+    d.slab_nt = g.slab_nt;
+    d.acc_cells = g.slab_off ? static_cast<uint64_t>(g.slab_cells) : flat;
     d.t_bin = g.t_bin;
     d.n_keys = g.t_bin ? static_cast<uint32_t>(static_cast<uint64_t>(g.n_r) * g.n_z * g.n_y * g.n_x) : d.n_cells;
     d.z_edges = g.z_edges;
@@ -77,6 +84,30 @@ __device__ __forceinline__ uint32_t footprint_key(const FpDev& g, bool active, f
     return static_cast<uint32_t>(key);
 }
 
+// Accumulator cell of one particle for the fixed-point footprint: release
+// r's slab offset plus its flat in-slab cell ((t * n_z + z) * n_y + y) * n_x +
+// x, 64-bit; kNoCell when the particle is outside any dimension or its time
+// bin is past its release's slab (slab_nt). The bin arithmetic is
+// footprint_key's, bit for bit.
+constexpr uint64_t kNoCell = ~0ull;
+
+__device__ __forceinline__ uint64_t fixed_cell(const FpDev& g, float lon, float lat, float z, int64_t t_idx,
+                                               int64_t r_idx) {
+    if (r_idx < 0 || r_idx >= g.n_r || t_idx < 0 || t_idx >= g.n_t) return kNoCell;
+    if (g.slab_nt != nullptr && t_idx >= g.slab_nt[r_idx]) return kNoCell;
+    const int xi = bin_index(lon, g.lon_min, g.span_x, g.fnx, g.n_x);
+    const int yi = bin_index(lat, g.lat_min, g.span_y, g.fny, g.n_y);
+    if (xi < 0 || yi < 0) return kNoCell;
+    int cnt = 0;
+    for (int k = 0; k <= g.n_z; ++k) cnt += (g.z_edges[k] < z) ? 1 : 0;
+    const int zi = cnt - 1;
+    if (zi < 0 || zi >= g.n_z) return kNoCell;
+    const uint64_t local = ((static_cast<uint64_t>(t_idx) * g.n_z + zi) * g.n_y + yi) * g.n_x + xi;
+    const uint64_t base = g.slab_off != nullptr ? static_cast<uint64_t>(g.slab_off[r_idx])
+                                                : static_cast<uint64_t>(r_idx) * g.n_t * g.n_z * g.n_y * g.n_x;
+    return base + local;
+}
+
 // Time-ago bin (main.py:1459-1465) from the pre-step cursor.
 __device__ __forceinline__ int64_t time_bin(float we_off, float cursor_off, int n_t) {
 #pragma clang fp contract(off)
@@ -97,15 +128,15 @@ __device__ __forceinline__ long long to_fixed(float v, int e) {
     return static_cast<long long>(llrint(ldexp(static_cast<double>(v), e)));
 }
 
-// Adds each lane's fixed-point value to acc[key] (keys >= n_keys dropped):
-// an inclusive segmented scan over runs of equal keys in lane order, then one
+// Adds each lane's fixed-point value to acc[key] (kNoCell dropped): an
+// inclusive segmented scan over runs of equal keys in lane order, then one
 // 64-bit integer atomic per run from its last lane. Integer addition is
 // associative, so the sums do not depend on the order the atomics land in.
 // Every lane of the wavefront must call it (no lane may have returned).
-__device__ __forceinline__ void wave_fixed_add(uint32_t key, long long v, uint32_t n_keys, long long* acc) {
-    if (__ballot(key < n_keys) == 0ull) return;
+__device__ __forceinline__ void wave_fixed_add(uint64_t key, long long v, long long* acc) {
+    if (__ballot(key != kNoCell) == 0ull) return;
     const int lane = static_cast<int>(__lane_id());
-    const uint32_t prev = __shfl_up(key, 1u);
+    const uint64_t prev = __shfl_up(static_cast<unsigned long long>(key), 1u);
     const unsigned long long heads = __ballot(lane == 0 || prev != key);
     const int start = 63 - __clzll(static_cast<long long>(heads & ((2ull << lane) - 1ull)));
 #pragma unroll
@@ -114,14 +145,14 @@ __device__ __forceinline__ void wave_fixed_add(uint32_t key, long long v, uint32
         if (lane - d >= start) v += o;
     }
     const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull) != 0ull;
-    if (tail && key < n_keys && v != 0)
+    if (tail && key != kNoCell && v != 0)
         atomicAdd(reinterpret_cast<unsigned long long*>(acc + key), static_cast<unsigned long long>(v));
 }
 
 // This workgroup's copy of the fixed-point accumulator (GlideEmit.fixed_copies).
-__device__ __forceinline__ long long* fixed_copy(const GlideEmit& e, uint32_t n_cells) {
+__device__ __forceinline__ long long* fixed_copy(const GlideEmit& e, uint64_t acc_cells) {
     const uint32_t copies = e.fixed_copies > 1 ? static_cast<uint32_t>(e.fixed_copies) : 1u;
-    return e.fixed + static_cast<uint64_t>(blockIdx.x % copies) * n_cells;
+    return e.fixed + static_cast<uint64_t>(blockIdx.x % copies) * acc_cells;
 }
 
 // Adds the number of lanes with flag set to *ctr: wave ballot, one integer atomic per wave.

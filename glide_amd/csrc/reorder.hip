@@ -32,6 +32,7 @@ struct ReorderKeyArgs {
     bool terminus;  // GLIDE_F_RELEASE_TERMINUS: lo_off holds the release terminus
     uint32_t ncell, sentinel;
     uint32_t groups;  // release groups of the key (1: by cell only)
+    bool by_level;    // the cell includes the level (else the column alone)
     uint32_t* keys;
     uint32_t* idx;
     uint32_t* live;  // optional count of particles that can still become active
@@ -58,13 +59,14 @@ __device__ __forceinline__ bool key_one(const ReorderKeyArgs& a, const float* le
         const int64_t ps = s.pos_stride;
         const int ix = static_cast<int>(frac_index(s.lon[i * ps], a.lon0, a.sx, a.nx));
         const int iy = static_cast<int>(frac_index(s.lat[i * ps], a.lat0, a.sy, a.ny));
-        const int iz = static_cast<int>(level_index(lev, a.nz, ladder_top(a.nz), s.z[i * ps]));
+        const int iz = a.by_level ? static_cast<int>(level_index(lev, a.nz, ladder_top(a.nz), s.z[i * ps])) : 0;
         uint32_t grp = 0;
         if (a.groups > 1) {
             const int32_t r = s.rel_idx[i];
             grp = static_cast<uint32_t>(r < 0 ? 0 : (r >= static_cast<int32_t>(a.groups) ? a.groups - 1 : r));
         }
-        const uint32_t cell = static_cast<uint32_t>((static_cast<int64_t>(iy) * a.nx + ix) * a.nz + iz);
+        const uint32_t cell = a.by_level ? static_cast<uint32_t>((static_cast<int64_t>(iy) * a.nx + ix) * a.nz + iz)
+                                         : static_cast<uint32_t>(static_cast<int64_t>(iy) * a.nx + ix);
         key = ((later ? a.groups : 0u) + grp) * a.ncell + cell;
     }
     a.keys[i] = key;
@@ -155,9 +157,11 @@ extern "C" int glide_reorder_keys(const GlideStepParams* p, const GlideParticles
     a.length_s = static_cast<float>(p->length_s);
     a.terminus = (p->flags & GLIDE_F_RELEASE_TERMINUS) != 0;
     GLIDE_REQUIRE(!a.terminus || s->lo_off, "GLIDE_F_RELEASE_TERMINUS needs lo_off");
-    const int64_t ncell = static_cast<int64_t>(g->nz) * g->ny * g->nx;
     const int64_t groups = n_releases > 1 ? n_releases : 1;
     GLIDE_REQUIRE(groups == 1 || s->rel_idx, "release-major keys need rel_idx");
+    // (column, level) cells while the key fits 32 bits, else columns alone
+    a.by_level = 2 * groups * static_cast<int64_t>(g->nz) * g->ny * g->nx < 0xffffffffll;
+    const int64_t ncell = a.by_level ? static_cast<int64_t>(g->nz) * g->ny * g->nx : static_cast<int64_t>(g->ny) * g->nx;
     GLIDE_REQUIRE(2 * groups * ncell < 0xffffffffll, "met grid x releases too large for the reorder keys");
     a.ncell = static_cast<uint32_t>(ncell);
     a.groups = static_cast<uint32_t>(groups);

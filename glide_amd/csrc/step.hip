@@ -80,7 +80,7 @@ struct StepCtx {
 // them, the dropped key and 0, for an inactive particle) instead of memory.
 template <bool BF16, bool FIXED>
 __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* lev, int64_t i,
-                                              const GlideStepParams& p, const StepCtx& x, uint32_t& fkey,
+                                              const GlideStepParams& p, const StepCtx& x, uint64_t& fkey,
                                               long long& fval) {
     // 32-bit byte offsets of this particle (the host checks the arrays fit)
     const uint32_t i1 = static_cast<uint32_t>(i);
@@ -326,13 +326,12 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     if (emit) {
         const int64_t tb = time_bin(ldo(s.we_off, i4), co, a.fp.n_t);
         const int64_t rb = x.slot >= 0 ? x.slot : (s.rel_idx ? ldo(s.rel_idx, i4) : 0);
-        const uint32_t key = footprint_key(a.fp, true, lon, lat, z, tb, rb);
         const float val = ldo(s.weight, i4) * dt;
         if (fixed) {
-            fkey = key;
-            fval = key < a.fp.n_keys ? to_fixed(val, a.e.fixed_exp[rb]) : 0;
+            fkey = fixed_cell(a.fp, lon, lat, z, tb, rb);
+            fval = fkey != kNoCell ? to_fixed(val, a.e.fixed_exp[rb]) : 0;
         } else {
-            sto(a.e.keys, i4, key);
+            sto(a.e.keys, i4, footprint_key(a.fp, true, lon, lat, z, tb, rb));
             sto(a.e.vals, i4, val);
         }
         const bool esc = !within_domain(a.fp, lon, lat, z);
@@ -433,11 +432,11 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
         }
     }
     bool act = false;
-    uint32_t fkey = a.fp.n_keys;
+    uint64_t fkey = kNoCell;
     long long fval = 0;
     if (run) act = step_particle<BF16, FIXED>(a, lev, i, p, x, fkey, fval);
     if (FIXED && (p.flags & GLIDE_F_EMIT))
-        wave_fixed_add(fkey, fval, a.fp.n_keys, fixed_copy(a.e, a.fp.n_cells));  // every lane is here
+        wave_fixed_add(fkey, fval, fixed_copy(a.e, a.fp.acc_cells));  // every lane is here
     if constexpr (MODE == kModeSlots) {
         if (n_active != nullptr) count_flags(act, n_active);  // per wavefront: its slot's counter
     } else if (n_active != nullptr) {
@@ -580,7 +579,13 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
         GLIDE_REQUIRE(fg->n_r >= 1 && fg->n_t >= 1 && fg->n_z >= 1 && fg->n_y >= 1 && fg->n_x >= 1,
                       "All footprint dimensions must be > 0");
         const uint64_t cells = static_cast<uint64_t>(fg->n_r) * fg->n_t * fg->n_z * fg->n_y * fg->n_x;
-        GLIDE_REQUIRE(cells < 0xffffffffull, "footprint has too many cells for 32-bit keys");
+        if (p->flags & GLIDE_F_FIXED) {
+            GLIDE_REQUIRE(fg->slab_off == nullptr || (fg->slab_nt != nullptr && fg->slab_cells > 0),
+                          "per-release slabs need slab_nt and slab_cells");
+        } else {
+            GLIDE_REQUIRE(cells < 0xffffffffull, "footprint has too many cells for 32-bit keys");
+            GLIDE_REQUIRE(fg->slab_off == nullptr, "per-release slabs need GLIDE_F_FIXED");
+        }
         GLIDE_REQUIRE(fg->n_z <= 64, "at most 64 footprint z bins");
         a.fp = make_fp(*fg);
         a.e = *e;

@@ -192,7 +192,8 @@ class BatchEngine:
                  device_met: DeviceMet | None = None, reorder_every: int = 0, use_graph: bool = False,
                  compact: bool = False, release_terminus: bool = False, diagnostics: bool = False,
                  ingest=None, step_block: int = 0, overlap: bool = False, binned_keys: bool = True,
-                 accumulate: str = "fixed", fixed_copies: int = 0, bin_by_release: bool | None = None):
+                 accumulate: str = "fixed", fixed_copies: int = 0, bin_by_release: bool | None = None,
+                 slabs: bool | None = None):
         self.cfg = cfg
         # accumulate: how each step's residence reaches the footprint.
         #   "fixed": inside the step kernel, as exact 64-bit fixed-point integer
@@ -279,6 +280,37 @@ class BatchEngine:
             dom_lat_max=dom["lat_max"], dom_alt_max=dom["alt_max"])
         self.n_cells = g["n_r"] * g["n_t"] * g["n_z"] * g["n_y"] * g["n_x"]
         self.cells_per_release = self.n_cells // g["n_r"]
+        # slabs: each release's footprint slab holds only its own hourly time
+        # bins (ceil(length / 3600)), addressed by 64-bit offsets, so batches
+        # beyond 2^32 flat cells fit (fixed-point accumulation only); None:
+        # on when the flat layout would not fit 32-bit keys
+        if slabs is None:
+            slabs = self.fixed_mode and self.n_cells >= 0xFFFFFFFF
+        if slabs and not self.fixed_mode:
+            raise ValueError("per-release slabs need the fixed-point accumulation (accumulate='fixed')")
+        if not slabs and self.n_cells >= 0xFFFFFFFF:
+            raise ValueError(f"{self.n_cells} footprint cells exceed 32-bit keys; use slabs=True "
+                             "(fixed-point accumulation)")
+        self.slabs = bool(slabs)
+        self.cells_per_bin = g["n_z"] * g["n_y"] * g["n_x"]
+        if self.slabs:
+            # a particle is active until the cursor reaches its r_off - length
+            # (under the batch terminus), so its time-ago reaches at most
+            # length + duration: hourly bins 0 .. floor((length + duration) / 3600)
+            nt = np.zeros(g["n_r"], np.int64)
+            for r in rels:
+                nt[r.release_idx] = max(nt[r.release_idx], (int(r.length_s) + int(r.duration_s)) // 3600 + 1)
+            self.slab_nt_np = np.clip(nt, 1, g["n_t"]).astype(np.int32)
+            sizes = self.slab_nt_np.astype(np.int64) * self.cells_per_bin
+            self.slab_off_np = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+            self.acc_cells = int(sizes.sum())
+            self.slab_off = _dev(self.slab_off_np, None, device)
+            self.slab_nt = _dev(self.slab_nt_np, None, device)
+            self.fg.slab_off = self.slab_off.data_ptr()
+            self.fg.slab_nt = self.slab_nt.data_ptr()
+            self.fg.slab_cells = self.acc_cells
+        else:
+            self.acc_cells = self.n_cells
         self.compensated = compensated
         self._alloc()
         self.reset()
@@ -322,7 +354,7 @@ class BatchEngine:
         # per-step active counts, each spread over ACTIVE_COPIES counters (one
         # atomic per workgroup on a single counter would queue); read as sums
         self._active = torch.zeros((max(len(self.sched), 1), ACTIVE_COPIES), dtype=torch.int64, device=d)
-        self._footprint = torch.zeros(self.n_cells, dtype=f32, device=d)
+        self._footprint = torch.zeros(self.acc_cells, dtype=f32, device=d)
         # Neumaier terms of the sort accumulation; the fixed-point sums are exact
         self.comp = torch.zeros(self.n_cells, dtype=f32, device=d) if self.compensated and not self.fixed_mode else None
         self._release_total = torch.zeros(self.grid_cfg["n_r"], dtype=torch.float64, device=d)
@@ -331,10 +363,10 @@ class BatchEngine:
             copies = self.fixed_copies_req
             if copies <= 0:
                 copies = 1
-                while copies < 64 and 2 * copies * 8 * self.n_cells <= (512 << 20):
+                while copies < 64 and 2 * copies * 8 * self.acc_cells <= (512 << 20):
                     copies *= 2
             self.fixed_copies = copies
-            self.fixed = torch.zeros(copies * self.n_cells, dtype=torch.int64, device=d)
+            self.fixed = torch.zeros(copies * self.acc_cells, dtype=torch.int64, device=d)
             self.fixed_exp = _dev(self.fixed_exponents(), None, d)
             self.fixed_ws = torch.empty(max(8, L.lib().glide_fixed_workspace_bytes(self.grid_cfg["n_r"])),
                                         dtype=torch.uint8, device=d)
@@ -349,7 +381,10 @@ class BatchEngine:
         by_rel = self.bin_by_release if self.bin_by_release is not None else (
             self.fixed_mode and self.grid_cfg["n_r"] > 1)
         self.bin_groups = int(self.grid_cfg["n_r"]) if by_rel else 1
-        # cells (per release group), the later bucket's, the sentinel
+        # cells (per release group), the later bucket's, the sentinel; the
+        # library drops the level from the cell when that key would not fit 32 bits
+        if 2 * self.bin_groups * ncell >= 0xFFFFFFFF:
+            ncell = self.met.ny * self.met.nx
         self.ro_bits = max(1, int(2 * self.bin_groups * ncell).bit_length())
         nc = self.met.ny * self.met.nx
         self.i_lcl = torch.empty(nc, dtype=torch.int32, device=d)
@@ -417,10 +452,18 @@ class BatchEngine:
     def _finalize_fixed(self, stream: int) -> None:
         """Fixed-point sums to the float32 footprint and float64 totals."""
 
-        L.check(L.lib().glide_fixed_finalize(
-            self.fixed.data_ptr(), self.fixed_copies, self.n_cells, self.cells_per_release, self.fixed_exp.data_ptr(),
-            self._footprint.data_ptr(), self._release_total.data_ptr(), self.fixed_ws.data_ptr(),
-            self.fixed_ws.numel(), stream))
+        if self.slabs:
+            g = self.grid_cfg
+            L.check(L.lib().glide_fixed_finalize_slabs(
+                self.fixed.data_ptr(), self.fixed_copies, self.acc_cells, self.slab_off.data_ptr(),
+                self.slab_nt.data_ptr(), self.cells_per_bin, int(self.slab_nt_np.max()), self.fixed_exp.data_ptr(),
+                0, g["n_r"], self._footprint.data_ptr(), self._release_total.data_ptr(), self.fixed_ws.data_ptr(),
+                self.fixed_ws.numel(), stream))
+        else:
+            L.check(L.lib().glide_fixed_finalize(
+                self.fixed.data_ptr(), self.fixed_copies, self.n_cells, self.cells_per_release,
+                self.fixed_exp.data_ptr(), self._footprint.data_ptr(), self._release_total.data_ptr(),
+                self.fixed_ws.data_ptr(), self.fixed_ws.numel(), stream))
         self._fixed_stale = False
 
     @property
@@ -434,13 +477,14 @@ class BatchEngine:
         """The fixed-point sums per cell (int64 [n_cells], summed over the
         accumulator's copies): cell c holds its residence times 2^fixed_exp[r]."""
 
-        return self.fixed.view(self.fixed_copies, self.n_cells).sum(dim=0)
+        return self.fixed.view(self.fixed_copies, self.acc_cells).sum(dim=0)
 
     @property
     def footprint(self) -> torch.Tensor:
-        """The footprint, float32 [n_r * n_t * n_z * n_y * n_x] on the device,
-        holding every step run so far (ordered before later work on the
-        current stream)."""
+        """The footprint, float32 on the device, holding every step run so far
+        (ordered before later work on the current stream): [n_r * n_t * n_z *
+        n_y * n_x], or with per-release slabs the releases' slabs [T_r][Z][Y][X]
+        back to back (``release_footprint(r)``)."""
 
         if self._fixed_stale:
             self._finalize_fixed(L.stream_ptr())
@@ -684,13 +728,32 @@ class BatchEngine:
 
     def footprint_numpy(self) -> np.ndarray:
         """The footprint as (R, T, Z, Y, X) float32 on the host (the
-        compensation terms added when the footprint is compensated)."""
+        compensation terms added when the footprint is compensated; with
+        per-release slabs, each release's bins past its own length are 0)."""
 
         self.join()  # after every step's accumulation (fixed mode: the float footprint made)
         torch.cuda.synchronize()
         g = self.grid_cfg
         f = self.footprint if self.comp is None else self.footprint + self.comp
-        return f.cpu().numpy().reshape(g["n_r"], g["n_t"], g["n_z"], g["n_y"], g["n_x"])
+        if not self.slabs:
+            return f.cpu().numpy().reshape(g["n_r"], g["n_t"], g["n_z"], g["n_y"], g["n_x"])
+        out = np.zeros((g["n_r"], g["n_t"], g["n_z"], g["n_y"], g["n_x"]), np.float32)
+        host = f.cpu().numpy()
+        for r in range(g["n_r"]):
+            out[r, : self.slab_nt_np[r]] = self._slab(host, r)
+        return out
+
+    def _slab(self, flat, r: int):
+        g = self.grid_cfg
+        o = int(self.slab_off_np[r]) if self.slabs else r * self.cells_per_release
+        nt = int(self.slab_nt_np[r]) if self.slabs else g["n_t"]
+        return flat[o: o + nt * self.cells_per_bin].reshape(nt, g["n_z"], g["n_y"], g["n_x"])
+
+    def release_footprint(self, r: int) -> torch.Tensor:
+        """Release ``r``'s footprint [T_r, Z, Y, X] float32 on the device (T_r its
+        own hourly bins with per-release slabs, else n_t)."""
+
+        return self._slab(self.footprint, r)
 
     def write_outputs(self, out_dir, *, epoch_iso: str = "2024-01-01T00:00:00", release_start: int = 0,
                       store=None) -> dict:

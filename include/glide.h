@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define GLIDE_ABI_VERSION 6
+#define GLIDE_ABI_VERSION 7
 
 #define GLIDE_OK 0
 #define GLIDE_ERR_INVALID (-1)
@@ -155,7 +155,9 @@ typedef struct GlideParticles {
 /*
  * Footprint geometry (FootprintGridder.__init__, footprint_gridder.py:50-115)
  * and the met domain of the escape kill (main.py:690-711).
- * n_cells = n_r*n_t*n_z*n_y*n_x must be < 2^32 - 1.
+ * The 32-bit key paths (keys/values, the sort accumulation, the release-binned
+ * keys) need n_cells = n_r*n_t*n_z*n_y*n_x < 2^32 - 1; the fixed-point
+ * accumulation takes any count (64-bit cells, per-release slabs).
  */
 typedef struct GlideFootprintGrid {
     double lon_min, lon_max, lat_min, lat_max;
@@ -170,6 +172,16 @@ typedef struct GlideFootprintGrid {
      * the time bin is a property of the release: the keys are shorter (fewer
      * sort passes) and sort in the same order as the flat cell index. */
     int32_t* t_bin;
+    /* Per-release slabs of the fixed-point accumulator (GLIDE_F_FIXED only; both
+     * device arrays may be NULL): release r's cells are slab_off[r] ..
+     * slab_off[r] + slab_nt[r] * n_z * n_y * n_x - 1, laid out [t][z][y][x], and
+     * its time bins t >= slab_nt[r] are dropped; slab_cells is the accumulator's
+     * total. Slabs sized by each release's length let a batch exceed the 2^32
+     * cells of the flat key (2,688 releases x 24-120 hourly bins x 160 x 240).
+     * NULL slab_off: the flat layout, slab_off[r] = r * n_t * n_z * n_y * n_x. */
+    const int64_t* slab_off;
+    const int32_t* slab_nt;
+    int64_t slab_cells;
 } GlideFootprintGrid;
 
 /*
@@ -339,6 +351,19 @@ int glide_fixed_finalize(const long long* fixed, int32_t copies, int64_t n_cells
                          const int32_t* fixed_exp, float* slab, double* release_total, void* workspace,
                          size_t workspace_bytes, void* stream);
 
+/* glide_fixed_finalize for per-release slabs (GlideFootprintGrid.slab_off):
+ * releases r0 .. r1-1, release r's slab_nt[r] * cells_per_bin cells from
+ * slab_off[r] of fixed (copies back to back, slab_cells apart), to float
+ * out[slab_off[r] - slab_off[r0] + c] (the range's slabs back to back when they
+ * are contiguous in the accumulator); release_total[r] for r in [r0, r1) (may
+ * be NULL) overwritten with the exact sums. max_nt bounds slab_nt (the launch
+ * shape). Workspace: glide_fixed_workspace_bytes(r1). Converting a range of
+ * finished releases lets a batch stream its footprints out per release. */
+int glide_fixed_finalize_slabs(const long long* fixed, int32_t copies, int64_t slab_cells, const int64_t* slab_off,
+                               const int32_t* slab_nt, uint32_t cells_per_bin, int32_t max_nt,
+                               const int32_t* fixed_exp, int32_t r0, int32_t r1, float* out, double* release_total,
+                               void* workspace, size_t workspace_bytes, void* stream);
+
 /* Workspace bytes glide_sort_pairs needs for n pairs. */
 size_t glide_sort_workspace_bytes(int64_t n);
 
@@ -364,6 +389,8 @@ int glide_sort_pairs(const uint32_t* keys, const uint32_t* vals, int64_t n, int3
  * active before the horizon is added to *live (zero it first). Sort with
  * glide_sort_pairs, then permute: those particles come first, then the ones
  * released further back. horizon_off below every r_off keeps one bucket.
+ * When 2*G*ncell does not fit 32 bits (thousands of releases), the cell is the
+ * column alone, ncell = ny*nx (a release's particles then sort by column).
  */
 int glide_reorder_keys(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,
                        double horizon_off, int32_t n_releases, uint32_t* keys, uint32_t* idx, uint32_t* live,

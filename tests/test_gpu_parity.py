@@ -14,6 +14,8 @@ from __future__ import annotations
 
 import ctypes as C
 
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -406,6 +408,72 @@ def test_full_run_cpu_workload_footprint(cfg0):
     assert abs(fp.sum() - ref.footprint.sum()) <= 1e-4 * ref.footprint.sum()
     assert int(esc.sum()) == int(sum(ref.escaped_per_step))
     print(f"full run: rel L2 {rel_l2:.3e}, total {tot_dev:.6f} vs {ref.footprint.sum():.6f}")
+
+
+def oracle_convection(met, hours, seed):
+    """The oracle's per-column convection for ORUN.run: on the first step of
+    each met hour (after the step, before the gridding), every active particle
+    is redistributed through its nearest column's matrix with the same trigger,
+    closure and Philox stream as the engine."""
+
+    levels = met.spec.agl_levels
+
+    def convect(P, active, bh, alpha, noise_step):
+        s3, _, p_hpa = hours[bh]
+        e3, _, _ = hours[bh + 1]
+        conv = met.hour(bh).conv
+        lcl, lnb, mb = OC.column_descriptors(conv, s3[3].astype(np.float32), e3[3].astype(np.float32), alpha,
+                                             p_hpa, levels.astype(np.float32))
+        ny, nx = lcl.shape
+        col = OC.nearest_column(P["lon"].astype(np.float32), P["lat"].astype(np.float32), met.lon[0], met.lon[-1],
+                                met.lat[0], met.lat[-1], nx, ny)
+        out = dict(P)
+        out["z"] = OC.convect_columns(P["z"], active, col, lcl, lnb, mb.astype(np.float64), p_hpa, levels,
+                                      seed, P["uid"], noise_step).astype(P["z"].dtype)
+        return out
+
+    return convect
+
+
+def test_full_run_cpu_workload_with_convection_footprint():
+    """The 10k-particle / 3 h run with per-column convection on every new met
+    hour, against the float64 oracle on the same noise: footprint within 1 %
+    relative L2, escapes equal. Total residence: within 1e-4, or within twice
+    the deviation of the oracle's own float32 run from its float64 run,
+    whichever is larger. At this release site particles hover at the 100 m top
+    of the output layer, where rounding moves residence across the layer edge,
+    and a convection draw is a discrete choice that a float32 position can
+    settle differently: the float32 oracle itself deviates by about 1.1e-3 here
+    (1.1e-3 without convection too), while at the workload's own release site
+    (test_full_run_cpu_workload_footprint) both stay below 1e-4."""
+
+    cfg = workload_config("plain-cpu-10k-3h")
+    # the same release inside a region whose columns fire every hour (the
+    # workload's own release column has no CAPE): 9.5 E 58.25 N
+    cfg = dict(cfg, convection=True, releases=[dataclasses.replace(cfg["releases"][0], lon=9.5, lat=58.25)])
+    met = SyntheticMet(cfg["spec"])
+    eng = BatchEngine(cfg, device="cuda", met=met, convection=ConvectionOptions())
+    eng.run()
+    torch.cuda.synchronize()
+    fp = eng.footprint.cpu().numpy().astype(np.float64)
+    hours = {}
+    ref = ORUN.run(cfg, dty=np.float64, met=met, hours_cache=hours,
+                   convection=oracle_convection(met, hours, cfg["seed"]))
+    hours32 = {}
+    f32 = ORUN.run(cfg, dty=np.float32, met=met, hours_cache=hours32,
+                   convection=oracle_convection(met, hours32, cfg["seed"]))
+    total = ref.footprint.sum()
+    rel_l2 = np.linalg.norm(fp - ref.footprint) / np.linalg.norm(ref.footprint)
+    assert rel_l2 <= 0.01, rel_l2
+    tot_dev = float(eng.release_total.cpu().numpy().sum())
+    envelope = abs(float(f32.footprint.astype(np.float64).sum()) - total) / total
+    assert abs(tot_dev - total) / total <= max(1e-4, 2.0 * envelope), (tot_dev, total, envelope)
+    assert int(eng.escaped.cpu().numpy().sum()) == int(sum(ref.escaped_per_step))
+    # convection moved particles: the run differs from the same run without it
+    plain = ORUN.run(dict(cfg, convection=False), dty=np.float64, met=met, hours_cache=hours)
+    assert np.linalg.norm(plain.footprint - ref.footprint) / np.linalg.norm(ref.footprint) > 0.01
+    print(f"full run with convection: rel L2 {rel_l2:.3e}, total {abs(tot_dev - total) / total:.2e} "
+          f"(float32 oracle {envelope:.2e})")
 
 
 def test_reordering_leaves_every_particle_bit_identical(cfg0):
