@@ -1,19 +1,35 @@
 """Throughput benchmark of the backward-LPDM step engine on one MI355X.
 
-Workload ("plain-gpu-1M-24h"): one release of 2^20 particles, 24 h backward at
-dt = 60 s (1,440 steps), full physics (Hanna three-regime turbulence with the
-Richardson closure aloft, meander on, per-column Emanuel convection once per
-met hour), synthetic met on a 240 x 160 grid at 0.25 deg with 60 AGL levels and
-25 hourly snapshots, footprint 24 time-ago bins x 1 layer x 160 x 240 in float32.
+Headline workload ("plain-gpu-1M-24h"): one release of 2^20 particles, 24 h
+backward at dt = 60 s (1,440 steps), full physics (Hanna three-regime
+turbulence with the Richardson closure aloft, meander on, per-column Emanuel
+convection once per met hour), synthetic met on a 240 x 160 grid at 0.25 deg
+with 60 AGL levels and 25 hourly snapshots, footprint 24 time-ago bins x 1
+layer x 160 x 240 in float32.
 
 A "step" is one model time step of that run over all 2^20 particles: the fused
 step kernel (which, by default, also adds every particle's residence to the
 footprint's exact fixed-point sums), the re-binning of the particles by met
 cell every few steps and, on a bracket change, the hourly resample, window
 build and convection. The float32 footprint is produced from the sums at the
-end of the timed region, inside it. The met source hours are
-resident in HBM before the timed region starts. ``value`` counts active
-particle-steps per second of wall time over the timed steps.
+end of the timed region, inside it. The met source hours are resident in HBM
+before the timed region starts. ``value`` counts active particle-steps per
+second of wall time over the timed steps.
+
+Beside the headline the same JSON line carries:
+
+* ``plain_torch``: the plain FP32 method as PyTorch ops on the same GPU and
+  workload (one met hour, convection included), eager and captured as a graph;
+* ``batched`` / ``batched_bf16``: the whole 2,688-release network batch
+  (56 sites x 48 hourly releases, 2k-20k particles, 24-120 h back, 168 met
+  hours) in one engine, float32 met and the bfloat16 precision layout;
+* ``serving``: Poisson footprint requests on a bfloat16 met ring;
+* ``ingest``: the headline workload with its met streamed from the host;
+* ``cpu_baseline``: the same plain method as PyTorch ops on the host's CPU
+  cores (the 10k-particle, 3 h workload, the reference's dynamic CPU path);
+* ``error_vs_fp64``: errors of the engine's results against the float64
+  oracle on the same noise (per-step positions over the first met hour;
+  full-run footprint and residence total of sampled releases of the batch).
 
 Prints one JSON line. ``--gpus N`` with N > 1 exits: this engine runs on one GPU.
 """
@@ -22,6 +38,9 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
+import statistics
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -32,16 +51,22 @@ if str(ROOT) not in sys.path:
 
 METRIC = "particle-steps/s, batched vs plain at stated error; p50/p99 footprint latency"
 WORKLOAD = "plain-gpu-1M-24h"
+CPU_WORKLOAD = "plain-cpu-10k-3h"
+BATCHED = "batched-gpu-2688r"
+BATCHED_BF16 = "batched-gpu-2688r-bf16"
+SERVING = "serving-gpu-10d"
 PEAK_HBM_GBS = 8000.0
+# VALU issue model of MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock, a
+# wave64 VALU instruction occupies its SIMD for 4 cycles
+N_SIMD, CLOCK_HZ, VALU_CYCLES = 1024, 2.4e9, 4
 # Algorithmic HBM bytes per active particle-step of the fused step kernel
-# (float32 structure-of-arrays state): read lon lat z u' v' w' u_m v_m, weight,
-# r_off, we_off, rel_idx, alive (49 B); write lon lat z u' v' w' u_m v_m, alive
-# (33 B). Noise is generated in the kernel (0 B). With the sort accumulation
-# the kernel also writes a 4 B cell key and a 4 B value (8 B; total 90 B); with
-# the fixed-point accumulation it adds to the footprint sums instead, one 8 B
-# integer atomic per run of equal cells in a wavefront, not per particle
-# (total 82 B, the atomics not counted).
-BYTES_PER_PARTICLE_STEP = {"fixed": 82, "sort": 90}
+# (float32 structure-of-arrays state): read lon lat z u' v' w' u_m v_m,
+# weight, r_off, we_off, rel_idx, alive (49 B); write lon lat z u' v' w' u_m
+# v_m, alive (33 B); the footprint emission, a 4 B cell key and a 4 B value
+# (8 B). Noise is generated in the kernel (0 B). Total 90 B. (The fixed-point
+# accumulation replaces the emission with one 8 B integer atomic per run of
+# equal cells in a wavefront, at most 8 B per particle-step.)
+BYTES_PER_PARTICLE_STEP = 90
 
 
 def parse():
@@ -52,47 +77,48 @@ def parse():
     ap.add_argument("--met", choices=("f32", "bf16"), default="f32")
     ap.add_argument("--reorder-every", type=int, default=32,
                     help="re-bin the particles by met cell every N steps (0 = never; 32 is the autotuner's "
-                         "choice on this workload with the fixed-point accumulation: binned particles gather "
-                         "from fewer cache lines and sum longer runs per atomic, and the gain decays as they "
-                         "move)")
+                         "choice on this workload with the fixed-point accumulation)")
     ap.add_argument("--step-block", type=int, default=128,
                     help="threads per workgroup of the step launch (64..512; 128 is the autotuner's choice on "
                          "this workload)")
     ap.add_argument("--graph", action="store_true",
                     help="time the captured step body (graph replay) as the headline run instead of "
                          "eager launches (both are always measured)")
-    ap.add_argument("--time-every", type=int, default=7,
-                    help="time the step kernel with HIP events on every Nth timed step (in graph "
-                         "mode those steps launch eagerly); 7 is coprime with the re-binning "
-                         "intervals, so the samples cover every phase of the binning cycle")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-batched", action="store_true",
-                    help="skip the network-batch leg (batched-gpu-336r, bfloat16 met, compensated footprint)")
-    ap.add_argument("--batched-steps", type=int, default=0,
-                    help="time only the first N steps of the network batch (0 = all)")
-    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
-    ap.add_argument("--no-ingest", action="store_true", help="skip the streamed-met leg")
-    ap.add_argument("--no-autotune", action="store_true", help="batched leg with the default launch parameters")
-    ap.add_argument("--batched-block", type=int, default=0, help="batched leg: step workgroup size (no tuning)")
-    ap.add_argument("--batched-reorder", type=int, default=-1, help="batched leg: re-binning interval (no tuning)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="tune the headline run's launch parameters too (its trials then show in a kernel trace)")
+    ap.add_argument("--kernel-steps", type=int, default=288,
+                    help="steps of the separate kernel-timing pass (HIP events around every step-kernel "
+                         "launch; at least --steps)")
     ap.add_argument("--accumulate", choices=("fixed", "sort"), default="fixed",
                     help="footprint accumulation: exact fixed-point sums inside the step kernel, or keys and "
                          "values sorted by cell and reduced in a fixed order")
     ap.add_argument("--no-other-accumulation", action="store_true",
                     help="skip timing the headline steps once more with the other footprint accumulation")
     ap.add_argument("--no-overlap", action="store_true",
-                    help="sort accumulation: accumulate each step's footprint on the step's own stream "
-                         "(no second stream)")
-    ap.add_argument("--serve-out", default="", help="keep the serving leg's Zarr stores in this directory")
+                    help="sort accumulation: accumulate each step's footprint on the step's own stream")
+    ap.add_argument("--autotune", action="store_true", help="tune the headline run's launch parameters too")
     ap.add_argument("--no-plain-torch", action="store_true", help="skip the PyTorch-ops plain method leg")
-    ap.add_argument("--torch-steps", type=int, default=60, help="timed steps of the PyTorch-ops plain method")
+    ap.add_argument("--torch-steps", type=int, default=60,
+                    help="timed steps of the PyTorch-ops plain method (60 = one met hour, convection included)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 2,688-release network batch legs")
+    ap.add_argument("--no-batched-bf16", action="store_true", help="skip the bfloat16 network batch leg")
+    ap.add_argument("--batched-steps", type=int, default=0, help="time only the first N steps of the batch")
+    ap.add_argument("--no-autotune", action="store_true", help="batched legs with the default launch parameters")
+    ap.add_argument("--batched-block", type=int, default=0, help="batched legs: step workgroup size (no tuning)")
+    ap.add_argument("--batched-reorder", type=int, default=-1, help="batched legs: re-binning interval (no tuning)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=120.0,
+                    help="seconds of Poisson request arrivals (20 requests/s) in the serving leg (600 = the "
+                         "full 10-minute stream)")
+    ap.add_argument("--serve-out", default="", help="keep the serving leg's Zarr stores in this directory")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the streamed-met leg")
     ap.add_argument("--ingest-workers", type=int, default=8, help="host decode threads of the streamed-met leg")
-    ap.add_argument("--serve-seconds", type=float, default=30.0,
-                    help="seconds of Poisson request arrivals (20 requests/s) in the serving leg")
-    ap.add_argument("--cpu-particles", type=int, default=262144)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU baseline and the float64 checks")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="timed CPU baseline runs per thread count (median)")
+    ap.add_argument("--no-fp64-check", action="store_true", help="skip the float64 error checks")
+    ap.add_argument("--check-releases", type=int, default=1,
+                    help="releases of the batch run alone through the float64 oracle (1-3: the smallest 24 h, "
+                         "48 h and 72 h releases)")
+    ap.add_argument("--check-particles", type=int, default=16384,
+                    help="particles of the per-step position check over the first met hour")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a float32 copy kernel of this many MiB (reads and writes that "
                          "many bytes) 5 times, to calibrate the HBM byte counters; prints one JSON line")
@@ -112,35 +138,395 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
-def cpu_baseline(cfg, n_particles: int, n_steps: int) -> dict:
-    """The oracle's float32 plain method (the reference's CPU path, restated)
-    on a bounded sample of the same workload: the first ``n_steps`` steps of
-    the run with ``n_particles`` particles of the same release on the same
-    grid. Times the step loop only."""
+# ---------------------------------------------------------------------------
+# timing helpers
+# ---------------------------------------------------------------------------
+
+def _timed_steps(eng, K: int, sptr: int) -> float:
+    """Wall seconds of the next K engine steps (the last accumulations
+    included), synchronised on both sides."""
+
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        eng.run_step(sptr)
+    eng.join(sptr)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def _kernel_pass(eng, n_steps: int, stream, sptr: int, every: int = 1) -> dict:
+    """From the release state, run ``n_steps`` steps with HIP events (on the
+    launch stream) around every ``every``-th step-kernel launch; the average
+    duration and the active particles of the timed launches."""
 
     import numpy as np
+    import torch
 
-    from oracle import lpdm as O
-    from oracle import run as ORUN
+    eng.reset()
+    torch.cuda.synchronize()
+    n = min(n_steps, len(eng.sched))
+    ev = eng.time_step_kernel(n, stream, every=every)
+    for _ in range(n):
+        eng.run_step(sptr)
+    eng.join(sptr)
+    torch.cuda.synchronize()
+    act = eng.active_steps.cpu().numpy()[:n].astype(np.float64)
+    timed = [(i, a.elapsed_time(b)) for i, (a, b) in sorted(ev.items()) if act[i] > 0]
+    t_ms = np.array([t for _, t in timed])
+    t_act = np.array([act[i] for i, _ in timed])
+    return {"launches": int(len(t_ms)), "avg_us": float(t_ms.mean() * 1e3) if len(t_ms) else float("nan"),
+            "particles_per_launch": float(t_act.mean()) if len(t_act) else 0.0,
+            "bytes_per_s": (BYTES_PER_PARTICLE_STEP * float(t_act.sum()) / (float(t_ms.sum()) * 1e-3)
+                            if len(t_ms) else float("nan"))}
 
-    rel = cfg["releases"][0]
-    sample = dict(cfg)
-    sample["releases"] = [type(rel)(**{**rel.__dict__, "n_particles": n_particles})]
-    sample["convection"] = False
-    res = ORUN.run(sample, dty=np.float32, params=O.HannaParams(), max_steps=n_steps)
-    return {
-        "value": res.particle_steps / res.step_seconds,
-        "unit": "particle-steps/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"first {n_steps} steps of {WORKLOAD} with {n_particles} particles "
-                   f"(float32 numpy restatement of the reference's plain method, "
-                   f"{res.particle_steps} particle-steps in {res.step_seconds:.2f} s, one core)"),
+
+def _counters(workload: str, met: str) -> dict:
+    """The step kernel's committed rocprofv3 counter summary for this
+    workload (HBM bytes per launch from FETCH_SIZE / WRITE_SIZE scaled by a
+    copy kernel's reading; VALU instructions per launch), if present."""
+
+    p = ROOT / "profiles" / f"step_kernel_counters_{workload}_{met}.json"
+    if not p.exists():
+        return {}
+    try:
+        return json.loads(p.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def _roofline(kp: dict, kernel: str, counters: dict) -> dict:
+    achieved = kp["bytes_per_s"] / 1e9
+    out = {
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 5),
+        "traffic": counters.get("traffic_bytes_per_launch"),
+        "kernel": kernel,
+        "kernel_avg_us": round(kp["avg_us"], 3),
+        "kernel_launches_timed": kp["launches"],
+        "particles_per_launch": round(kp["particles_per_launch"]),
+        "bytes_per_particle_step": BYTES_PER_PARTICLE_STEP,
+        "bytes_per_launch": round(BYTES_PER_PARTICLE_STEP * kp["particles_per_launch"]),
     }
+    valu = counters.get("valu_insts_per_launch")
+    if valu:
+        # issue-limited time of the launch's VALU instructions on every SIMD
+        floor_us = valu * VALU_CYCLES / (N_SIMD * CLOCK_HZ) * 1e6
+        out["valu"] = {"insts_per_launch": valu, "issue_floor_us": round(floor_us, 2),
+                       "issue_frac": round(floor_us / kp["avg_us"], 4),
+                       "model": f"{VALU_CYCLES} cycles per wave64 instruction, {N_SIMD} SIMDs at "
+                                f"{CLOCK_HZ / 1e9:.1f} GHz"}
+    if counters:
+        out["counters_source"] = counters.get("source")
+    return out
 
 
-BATCHED = "batched-gpu-336r"
-SERVING = "serving-gpu-10d"
+# ---------------------------------------------------------------------------
+# the headline: plain FP32 method, 2^20 particles, on the HIP engine
+# ---------------------------------------------------------------------------
+
+def headline_leg(args, stream, sptr: int) -> tuple[dict, dict]:
+    import numpy as np
+    import torch
+
+    from glide_amd import _lib as L
+    from glide_amd.autotune import tune
+    from glide_amd.runner import BatchEngine
+    from glide_amd.synthetic import SyntheticMet, workload_config
+
+    cfg = workload_config(WORKLOAD)
+    met = SyntheticMet(cfg["spec"])
+    met_dtype = L.GLIDE_MET_BF16 if args.met == "bf16" else L.GLIDE_MET_F32
+    kw = dict(met_dtype=met_dtype, overlap=not args.no_overlap, accumulate=args.accumulate)
+    if args.autotune:
+        tuned = tune(cfg, met, use_cache=False, **kw)
+    else:
+        tuned = {"best": {"reorder_every": args.reorder_every, "step_block": args.step_block}, "trials": []}
+    eng = BatchEngine(cfg, device="cuda", met=met, use_graph=True, **kw, **tuned["best"])
+    n_sched = len(eng.sched)
+    K = max(1, min(args.steps, n_sched))
+    W = max(0, min(args.warmup, n_sched))
+
+    def timed_run(graph: bool) -> float:
+        """Warm up (captures the step body in graph mode), reset, time K steps."""
+        eng.use_graph = graph
+        eng.step_events = None
+        eng.run(W)
+        torch.cuda.synchronize()
+        eng.reset()
+        return _timed_steps(eng, K, sptr)
+
+    other_elapsed = timed_run(not args.graph)  # the other launch mode (replay vs eager)
+    elapsed = timed_run(args.graph)  # the headline run
+    act = eng.active_steps.cpu().numpy()[:K].astype(np.float64)  # counted by the step kernel
+    psteps = int(act.sum())
+    eng.use_graph = False
+    kp = _kernel_pass(eng, max(K, args.kernel_steps), stream, sptr)
+    kernel = (f"glide::step_kernel<{str(met_dtype == L.GLIDE_MET_BF16).lower()}, 0, "
+              f"{str(args.accumulate == 'fixed').lower()}>")
+    roofline = _roofline(kp, kernel, _counters(WORKLOAD, args.met) if args.accumulate == "fixed" else {})
+    roofline["whole_step_achieved"] = round(BYTES_PER_PARTICLE_STEP * psteps / elapsed / 1e9, 2)
+    out = {
+        "metric": METRIC,
+        "value": psteps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": elapsed * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if met_dtype == L.GLIDE_MET_F32 else "bf16-met/f32",
+        "data": "synthetic (seeded met cube and release; no network)",
+        "config": {
+            "workload": WORKLOAD,
+            "particles": eng.n,
+            "dt_s": int(cfg["dt"]),
+            "steps_in_run": n_sched,
+            "grid": f"{cfg['spec'].nx}x{cfg['spec'].ny}x{cfg['spec'].za}",
+            "met_hours": len(eng.met.src),  # resident in HBM before the timed region
+            "met_storage": args.met,
+            "convection": "per-column emanuel",
+            "meander": True,
+            "max_substeps": eng.h.max_substeps,
+            "reorder_every": eng.reorder_every,
+            "step_block": eng.step_block or 256,
+            "autotune_trials": tuned["trials"],
+            "step_graph": bool(args.graph),
+            "footprint_accumulation": args.accumulate,
+            "footprint": f"1x{cfg['grid']['n_t']}x1x{cfg['grid']['n_y']}x{cfg['grid']['n_x']} f32",
+        },
+        "active_particle_steps": psteps,
+        ("eager" if args.graph else "graph_replay"): {
+            "ms_per_step": other_elapsed * 1e3 / K,
+            "value": psteps / other_elapsed,
+        },
+        "roofline": roofline,
+    }
+    if not args.no_other_accumulation:
+        # the same K steps with the other footprint accumulation, on the same met
+        other = "sort" if args.accumulate == "fixed" else "fixed"
+        oe = BatchEngine(cfg, device="cuda", met=met, device_met=eng.met, accumulate=other, overlap=True,
+                         met_dtype=met_dtype, **tuned["best"])
+        oe.run(W)
+        torch.cuda.synchronize()
+        oe.reset()
+        oel = _timed_steps(oe, K, sptr)
+        out[f"accumulate_{other}"] = {"ms_per_step": oel * 1e3 / K, "value": psteps / oel,
+                                      "method": ("keys and values written by the step, stable radix sort by cell, "
+                                                 "fixed-order float32 reduction on a second stream" if other == "sort"
+                                                 else "exact int64 fixed-point sums inside the step kernel")}
+        oe.close()
+        del oe
+    evidence = {"positions": {}}
+    if not args.no_fp64_check and not args.no_cpu_baseline:
+        for name, mdt in (("f32", L.GLIDE_MET_F32), ("bf16", L.GLIDE_MET_BF16)):
+            evidence["positions"][name] = position_snapshots(cfg, met, mdt, args.check_particles)
+    eng.close()
+    del eng
+    torch.cuda.empty_cache()
+    return out, evidence
+
+
+def position_snapshots(cfg, met, met_dtype: int, n_particles: int, n_steps: int = 60) -> dict:
+    """The headline release with ``n_particles`` particles stepped by the
+    engine over the first met hour; the particle state (in particle-id order)
+    before the first step and after each step, for the per-step check against
+    the float64 oracle."""
+
+    import dataclasses
+
+    import torch
+
+    from glide_amd.runner import BatchEngine
+
+    rel = dataclasses.replace(cfg["releases"][0], n_particles=int(n_particles))
+    c = dict(cfg, releases=[rel])
+    eng = BatchEngine(c, device="cuda", met=met, met_dtype=met_dtype)
+    snaps = [eng.state_numpy()]
+    steps = eng.sched[:n_steps]
+    for _ in steps:
+        eng.run_step()
+        snaps.append(eng.state_numpy())
+    torch.cuda.synchronize()
+    out = {"cfg": c, "sched": steps, "snaps": snaps, "length_s": eng.length_s, "seed": eng.seed,
+           "r_off": eng.ra.r_off.copy(), "lo_off": eng.ra.lo_off.copy(), "uid": eng.ra.uid.copy()}
+    eng.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the plain method as PyTorch ops on the GPU (the comparison baseline)
+# ---------------------------------------------------------------------------
+
+def plain_torch_leg(args) -> dict:
+    """The plain FP32 method as PyTorch ops on the same GPU and workload
+    (grid_sample, searchsorted, randn, the fixed substep loop, materialised
+    convection rows, scatter_add_): the first --torch-steps steps of the
+    schedule (one met hour by default, its first step convecting), eager and
+    with the step body captured as one graph, each after a 3-step warm-up and
+    a reset."""
+
+    import torch
+
+    from glide_amd.runner import BatchEngine
+    from glide_amd.synthetic import SyntheticMet, workload_config
+    from glide_amd.torch_plain import TorchPlain
+
+    cfg = workload_config(WORKLOAD)
+    met = SyntheticMet(cfg["spec"])
+    eng = BatchEngine(cfg, device="cuda", met=met)
+    K = max(1, min(args.torch_steps, len(eng.sched)))
+    res = {}
+    for mode in ("eager", "graph"):
+        tp = TorchPlain.from_engine(eng, seed=2, convection=True, graph=mode == "graph")
+        tp.run(3)
+        torch.cuda.synchronize()
+        tp.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tp.run(K)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        psteps = tp.active_steps
+        res[mode] = {"value": psteps / elapsed, "ms_per_step": elapsed * 1e3 / K, "active_particle_steps": psteps}
+        del tp
+        torch.cuda.empty_cache()
+    eng.close()
+    del eng
+    torch.cuda.empty_cache()
+    return {"workload": WORKLOAD, "value": res["graph"]["value"], "unit": "particle-steps/s", "steps": K,
+            "ms_per_step": res["graph"]["ms_per_step"], "eager": res["eager"],
+            "convection": "per column, once per met hour (the first timed step)",
+            "method": "plain FP32 as PyTorch ops (grid_sample, randn, scatter_add_), step body captured as one "
+                      "graph (value) and eager"}
+
+
+# ---------------------------------------------------------------------------
+# the network batch: 2,688 releases in one engine
+# ---------------------------------------------------------------------------
+
+BATCH_CANDIDATES = [dict(step_block=256, reorder_every=r) for r in (0, 16, 64)] + [
+    dict(step_block=128, reorder_every=16), dict(step_block=512, reorder_every=16)]
+
+
+def sampled_releases(cfg, n: int = 1) -> list[int]:
+    """Releases of the batch checked against the float64 oracle: for the
+    first ``n`` of the backward lengths 24, 48, 72 h, the release of that
+    length with the fewest particles (the oracle runs one 24 h release of
+    about 2k particles in about 20 s of one core, in float64 and again in
+    float32)."""
+
+    out = []
+    for hours in (24, 48, 72)[:n]:
+        rs = [r for r in cfg["releases"] if r.length_s == hours * 3600]
+        if rs:
+            out.append(min(rs, key=lambda r: (r.n_particles, r.release_idx)).release_idx)
+    return out
+
+
+def batched_leg(args, name: str, stream, sptr: int, shared: dict, evidence: dict) -> dict:
+    """The whole network batch in one engine on one GPU: releases of 2k-20k
+    particles and 24-120 h backward lengths, each stepped while the cursor is
+    above its own terminus, footprints in per-release slabs of exact
+    fixed-point sums (64-bit cells), float64 release totals, the live set
+    compacted every met hour. Times the whole schedule."""
+
+    import numpy as np
+    import torch
+
+    from glide_amd import _lib as L
+    from glide_amd.autotune import tune
+    from glide_amd.runner import BatchEngine, DeviceMet
+    from glide_amd.synthetic import SyntheticMet, workload_config
+
+    cfg = workload_config(name)
+    bf16 = name.endswith("-bf16")
+    met_dtype = L.GLIDE_MET_BF16 if bf16 else L.GLIDE_MET_F32
+    if "met" not in shared:
+        shared["met"] = SyntheticMet(cfg["spec"])
+    met = shared["met"]
+    from glide_amd.schedule import cursor_schedule
+
+    sched = cursor_schedule(cfg["releases"], int(cfg["dt"]))
+    hours = sorted({s.bracket_hour for s in sched} | {s.bracket_hour + 1 for s in sched})
+    t_setup = time.perf_counter()
+    dm = DeviceMet(met, hours, device="cuda", met_dtype=met_dtype, sources=shared.get("sources"))
+    shared["sources"] = dm.src
+    kw = dict(met_dtype=met_dtype, compact=True, release_terminus=True, accumulate="fixed")
+    if args.batched_block or args.batched_reorder >= 0:
+        best = {}
+        if args.batched_block:
+            best["step_block"] = args.batched_block
+        if args.batched_reorder >= 0:
+            best["reorder_every"] = args.batched_reorder
+        tuned = {"best": best, "trials": []}
+    elif args.no_autotune:
+        tuned = {"best": {}, "trials": []}
+    elif "tuned" in shared:
+        tuned = {"best": shared["tuned"], "trials": [], "from": BATCHED}
+    else:
+        tuned = tune(cfg, met, candidates=BATCH_CANDIDATES, use_cache=False, stream=stream, device_met=dm, **kw)
+        shared["tuned"] = tuned["best"]
+    eng = BatchEngine(cfg, device="cuda", met=met, device_met=dm, **kw, **tuned["best"])
+    n_sched = len(eng.sched)
+    K = n_sched if args.batched_steps <= 0 else min(args.batched_steps, n_sched)
+    eng.run(min(20, n_sched))  # warm-up, then back to the release state
+    torch.cuda.synchronize()
+    eng.reset()
+    setup_s = time.perf_counter() - t_setup
+    ev = eng.time_step_kernel(K, stream, every=7)  # every 7th launch (coprime with the re-binning interval)
+    elapsed = _timed_steps(eng, K, sptr)
+    act = eng.active_steps.cpu().numpy()[:K].astype(np.float64)
+    psteps = int(act.sum())
+    timed = [(i, a.elapsed_time(b)) for i, (a, b) in sorted(ev.items()) if act[i] > 0]
+    t_ms = np.array([t for _, t in timed])
+    t_act = np.array([act[i] for i, _ in timed])
+    kp = {"launches": int(len(t_ms)), "avg_us": float(t_ms.mean() * 1e3) if len(t_ms) else float("nan"),
+          "particles_per_launch": float(t_act.mean()) if len(t_act) else 0.0,
+          "bytes_per_s": (BYTES_PER_PARTICLE_STEP * float(t_act.sum()) / (float(t_ms.sum()) * 1e-3)
+                          if len(t_ms) else float("nan"))}
+    roofline = _roofline(kp, f"glide::step_kernel<{str(bf16).lower()}, 0, true>", {})
+    roofline["whole_step_achieved"] = round(BYTES_PER_PARTICLE_STEP * psteps / elapsed / 1e9, 2)
+    g = cfg["grid"]
+    tot = eng.release_total.cpu().numpy()
+    if K == n_sched:
+        for r in sampled_releases(cfg, args.check_releases):
+            evidence.setdefault(r, {})[name] = {
+                "footprint": eng.release_footprint(r).cpu().numpy().astype(np.float64),
+                "total": float(tot[r])}
+    out = {
+        "workload": name,
+        "value": psteps / elapsed,
+        "unit": "particle-steps/s",
+        "ms_per_step": elapsed * 1e3 / K,
+        "seconds": elapsed,
+        "steps": K,
+        "steps_in_run": n_sched,
+        "active_particle_steps": psteps,
+        "particles": eng.n,
+        "releases": len(cfg["releases"]),
+        "met_hours": len(hours),
+        "met_storage": "bf16" if bf16 else "f32",
+        "footprint": (f"{g['n_r']} per-release slabs x up to {g['n_t']} time-ago bins x 1 x {g['n_y']} x {g['n_x']} "
+                      f"f32 from exact int64 fixed-point sums ({eng.acc_cells} cells)"),
+        "release_totals": "f64",
+        "setup_s": setup_s,
+        "device_gb_allocated": round(torch.cuda.max_memory_allocated() / 1e9, 1),
+        "autotune": {"chosen": tuned["best"], "trials": tuned["trials"], **({"from": tuned["from"]}
+                                                                            if "from" in tuned else {})},
+        "roofline": roofline,
+    }
+    eng.close()
+    del eng, dm
+    torch.cuda.empty_cache()
+    return out
 
 
 def serving_leg(args, plain_value: float) -> dict:
@@ -212,126 +598,6 @@ def serving_leg(args, plain_value: float) -> dict:
     return out
 
 
-def batched_leg(args, sptr: int, plain_value: float) -> dict:
-    """One GPU's share of the network batch: 336 releases (7 sites x 48 hourly
-    releases, 2k-20k particles, 24-120 h back) in one engine, with the
-    precision layout of the batched method (bfloat16 met records; float32
-    footprint from exact fixed-point sums, or with Neumaier compensation under
-    the sort accumulation; float64 release totals) and the live set compacted
-    every met hour. Times the whole schedule."""
-
-    import numpy as np
-    import torch
-
-    from glide_amd import _lib as L
-    from glide_amd.runner import BatchEngine
-    from glide_amd.synthetic import SyntheticMet, workload_config
-
-    from glide_amd.autotune import tune
-
-    cfg = workload_config(BATCHED)
-    met = SyntheticMet(cfg["spec"])
-    kw = dict(met_dtype=L.GLIDE_MET_BF16, compensated=True, compact=True, overlap=not args.no_overlap,
-              accumulate=args.accumulate)
-    bpp = BYTES_PER_PARTICLE_STEP[args.accumulate]
-    if args.batched_block or args.batched_reorder >= 0:
-        best = {}
-        if args.batched_block:
-            best["step_block"] = args.batched_block
-        if args.batched_reorder >= 0:
-            best["reorder_every"] = args.batched_reorder
-        tuned = {"best": best, "trials": []}
-    elif args.no_autotune:
-        tuned = {"best": {}, "trials": []}
-    else:
-        tuned = tune(cfg, met, use_cache=False, **kw)
-    eng = BatchEngine(cfg, device="cuda", met=met, **kw, **tuned["best"])
-    n_sched = len(eng.sched)
-    K = n_sched if args.batched_steps <= 0 else min(args.batched_steps, n_sched)
-    eng.run(min(20, n_sched))  # warm-up, then back to the release state
-    torch.cuda.synchronize()
-    eng.reset()
-    torch.cuda.synchronize()
-    ev = eng.time_step_kernel(K, torch.cuda.current_stream(), every=max(1, args.time_every))
-    t0 = time.perf_counter()
-    for _ in range(K):
-        eng.run_step(sptr)
-    eng.join(sptr)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    act = eng.active_steps.cpu().numpy()[:K].astype(np.float64)
-    psteps = int(act.sum())
-    g = cfg["grid"]
-    value = psteps / elapsed
-    timed = [(i, a.elapsed_time(b)) for i, (a, b) in sorted(ev.items()) if act[i] > 0]
-    t_ms = np.array([t for _, t in timed])
-    moved = float(sum(act[i] for i, _ in timed)) * bpp
-    achieved = moved / (float(t_ms.sum()) * 1e-3) / 1e9 if len(t_ms) else float("nan")
-    out = {
-        "workload": BATCHED,
-        "value": value,
-        "unit": "particle-steps/s",
-        "ms_per_step": elapsed * 1e3 / K,
-        "steps": K,
-        "steps_in_run": n_sched,
-        "active_particle_steps": psteps,
-        "particles": eng.n,
-        "releases": len(cfg["releases"]),
-        "met_storage": "bf16",
-        "footprint": (f"{g['n_r']}x{g['n_t']}x1x{g['n_y']}x{g['n_x']} f32 "
-                      + ("from exact int64 fixed-point sums" if args.accumulate == "fixed"
-                         else "+ Neumaier compensation")),
-        "release_totals": "f64",
-        "vs_plain": value / plain_value if plain_value > 0 else None,
-        "autotune": {"chosen": tuned["best"], "trials": tuned["trials"]},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 5),
-                     "kernel": f"glide::step_kernel<true, 0, {str(args.accumulate == 'fixed').lower()}>",
-                     "kernel_launches_timed": len(t_ms),
-                     "kernel_avg_us": round(float(t_ms.mean()) * 1e3, 3) if len(t_ms) else None,
-                     "bytes_per_particle_step": bpp,
-                     "whole_step_achieved": round(psteps * bpp / elapsed / 1e9, 2)},
-        "error_bar": ("footprint within 1 % relative L2 per release of the float64 oracle on the same noise "
-                      "(tests/test_gpu_batch.py, 6-release network batch on the 64x48 grid: measured <= 1.4e-3 "
-                      "with bfloat16 met, <= 3e-4 with float32 met; residence totals within 8.5e-4 / 1.9e-4)"),
-    }
-    eng.close()
-    del eng
-    torch.cuda.empty_cache()
-    return out
-
-
-def plain_torch_leg(args, sptr: int) -> dict:
-    """The plain FP32 method as PyTorch ops on the same GPU (the reference's
-    GPU path restated: grid_sample, searchsorted, randn, the fixed substep
-    loop, scatter_add_), on the same workload: the first --torch-steps steps
-    of the schedule, after 3 warm-up steps on a separate instance."""
-
-    import torch
-
-    from glide_amd.runner import BatchEngine
-    from glide_amd.synthetic import SyntheticMet, workload_config
-    from glide_amd.torch_plain import TorchPlain
-
-    cfg = workload_config(WORKLOAD)
-    met = SyntheticMet(cfg["spec"])
-    eng = BatchEngine(cfg, device="cuda", met=met)
-    TorchPlain(eng, seed=1).run(3, sptr)
-    torch.cuda.synchronize()
-    tp = TorchPlain(eng, seed=2)
-    K = max(1, min(args.torch_steps, len(eng.sched)))
-    t0 = time.perf_counter()
-    tp.run(K, sptr)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    psteps = int(tp.active_steps.item())
-    out = {"workload": WORKLOAD, "value": psteps / elapsed, "unit": "particle-steps/s", "steps": K,
-           "ms_per_step": elapsed * 1e3 / K, "active_particle_steps": psteps,
-           "method": "plain FP32 as PyTorch ops (grid_sample, randn, scatter_add_), eager"}
-    del tp, eng
-    torch.cuda.empty_cache()
-    return out
-
 
 def ingest_leg(args, sptr: int, plain_value: float) -> dict:
     """The plain workload with its met streamed instead of resident: source
@@ -378,6 +644,233 @@ def ingest_leg(args, sptr: int, plain_value: float) -> dict:
     }
 
 
+
+# ---------------------------------------------------------------------------
+# the CPU baseline and the float64 checks (the oracle is the checker here)
+# ---------------------------------------------------------------------------
+
+def _cpu_model() -> str:
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        txt = ""
+    for line in txt.splitlines():
+        if line.startswith("Model name:"):
+            return line.split(":", 1)[1].strip()
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_leg(args) -> dict:
+    """The plain method as PyTorch ops on the host's CPU cores: the
+    reference's default CPU path (the dynamic step over the active particles,
+    max(k) substeps) restated in glide_amd.torch_plain, float32, on the whole
+    10k-particle, 3 h workload (64x48x40 met, convection off), at
+    torch.set_num_threads(n) for n = os.cpu_count() and n = 16. Per thread
+    count: one warm-up run, then the median of --cpu-runs runs of the step loop
+    (windows prepared; one run when the warm-up took over 20 s), and one run
+    of the whole run (window building from the synthetic met included)."""
+
+    import numpy as np
+    import torch
+
+    from glide_amd.synthetic import SyntheticMet, workload_config
+    from glide_amd.torch_plain import TorchPlain
+    from oracle import run as ORUN
+
+    cfg = workload_config(CPU_WORKLOAD)
+    met = SyntheticMet(cfg["spec"])
+    geo = dict(lon0=float(met.lon[0]), lon1=float(met.lon[-1]), lat0=float(met.lat[0]), lat1=float(met.lat[-1]))
+    levels = cfg["spec"].agl_levels
+    prepared = {}
+    fresh = ORUN.torch_windows(met)
+
+    def cached(bh):
+        if bh not in prepared:
+            prepared[bh] = fresh(bh)
+        return prepared[bh]
+
+    def one(windows):
+        t0 = time.perf_counter()
+        tp = TorchPlain(cfg, windows, device="cpu", levels=levels, geometry=geo, seed=3, dynamic=True)
+        t1 = time.perf_counter()
+        tp.run(len(tp.sched))
+        t2 = time.perf_counter()
+        fp = tp.footprint.numpy()
+        total = float(fp.sum())
+        return tp.active_steps, t2 - t1, time.perf_counter() - t0, total
+
+    n_cpu = os.cpu_count() or 1
+    threads = sorted({min(n_cpu, 256), 16})
+    prev = torch.get_num_threads()
+    runs = {}
+    try:
+        for n in threads:
+            torch.set_num_threads(n)
+            _, warm_s, _, _ = one(cached)  # warm-up (also prepares the windows)
+            # bounded: a thread count far above the cores this process may use
+            # runs slowly; then the median is over fewer runs
+            n_runs = max(1, args.cpu_runs) if warm_s < 20.0 else 1
+            step_s = []
+            for _ in range(n_runs):
+                ps, s1, _, _ = one(cached)
+                step_s.append(s1)
+            _, _, e2e, _ = one(ORUN.torch_windows(met))  # windows built from the synthetic met again
+            runs[n] = {"threads": n, "step_loop_s": statistics.median(step_s), "end_to_end_s": e2e,
+                       "particle_steps": ps, "value": ps / statistics.median(step_s), "end_to_end_value": ps / e2e,
+                       "runs": len(step_s), "warmup_s": warm_s}
+    finally:
+        torch.set_num_threads(prev)
+    best = max(runs.values(), key=lambda r: r["value"])
+    return {
+        "value": best["value"],
+        "unit": "particle-steps/s",
+        "cores": best["threads"],
+        "kind": "port",
+        "sample": (f"{CPU_WORKLOAD}: 10,000 particles x 180 steps ({best['particle_steps']} particle-steps), "
+                   f"the reference's dynamic CPU step as float32 PyTorch ops (glide_amd.torch_plain, "
+                   f"dynamic=True); median of {best['runs']} step-loop runs after 1 warm-up at "
+                   f"{best['threads']} threads"),
+        "by_threads": [runs[n] for n in threads],
+        "os_cpu_count": n_cpu,
+        "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+        "cpu_model": _cpu_model(),
+        "end_to_end_value": best["end_to_end_value"],
+    }
+
+
+def position_errors(ev: dict) -> dict:
+    """Per-step position errors of the engine over the first met hour: each
+    engine step from its own float32 state against one float64 oracle step
+    from the same state, on the same Philox noise (convection included on the
+    hour's first step). Relative errors |d lon| / |lon|, |d lat| / |lat| and
+    |d z| / max(|z|, |z_pre|, 10 m) of the active particles."""
+
+    import numpy as np
+
+    from oracle import lpdm as O
+    from oracle import run as ORUN
+
+    from glide_amd.synthetic import SyntheticMet
+
+    cfg = ev["cfg"]
+    met = SyntheticMet(cfg["spec"])
+    levels = cfg["spec"].agl_levels
+    hours = {}
+
+    def hour(h):
+        if h not in hours:
+            sh = met.hour(h)
+            from oracle import resample as ors
+
+            hours[h] = ors.resample_hour(sh.src3, sh.src2, sh.height_agl, met.terrain, met.lat, met.lon, levels,
+                                         sh.p_level_pa)
+        return hours[h]
+
+    conv = ORUN.column_convection(met, hours, ev["seed"]) if cfg.get("convection") else None
+    p = O.HannaParams()
+    windows = {}
+    order = np.argsort(ev["uid"], kind="stable")  # snapshots are in particle-id order
+    r_off, lo_off = ev["r_off"][order], ev["lo_off"][order]
+    worst = {"lon": 0.0, "lat": 0.0, "z": 0.0}
+    sums = {"lon": 0.0, "lat": 0.0, "z": 0.0}
+    n_tot = 0
+    frac_ok = []
+    last_bracket = None
+    for st, pre, post in zip(ev["sched"], ev["snaps"][:-1], ev["snaps"][1:]):
+        bh = st.bracket_hour
+        if bh not in windows:
+            windows[bh] = ORUN.build_window(hour(bh), hour(bh + 1), levels, met.lon, met.lat, np.float64)
+        co = np.float32(st.cursor_off)
+        active = (r_off >= co) & (lo_off <= co) & pre["alive"]
+        S = {k: pre[k].astype(np.float64) for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm")}
+        S["uid"] = pre["uid"]
+        ref = O.step(S, active, windows[bh], st.alpha, float(st.delta_s), p, seed=ev["seed"],
+                     step_index=st.noise_step)
+        if bh != last_bracket:
+            if conv is not None:
+                ref = conv(ref, active, bh, st.alpha, st.noise_step)
+            last_bracket = bh
+        if not active.any():
+            continue
+        e = {"lon": np.abs(post["lon"][active] - ref["lon"][active]) / np.abs(ref["lon"][active]),
+             "lat": np.abs(post["lat"][active] - ref["lat"][active]) / np.abs(ref["lat"][active]),
+             "z": np.abs(post["z"][active] - ref["z"][active])
+             / np.maximum(np.maximum(np.abs(ref["z"][active]), np.abs(S["z"][active])), 10.0)}
+        for k in e:
+            worst[k] = max(worst[k], float(e[k].max()))
+            sums[k] += float(e[k].sum())
+        n_tot += int(active.sum())
+        frac_ok.append(float((np.maximum(np.maximum(e["lon"], e["lat"]), e["z"]) <= 1e-5).mean()))
+    return {"steps": len(frac_ok), "particles": int(ev["snaps"][0]["lon"].size),
+            "mean_rel": {k: sums[k] / max(n_tot, 1) for k in sums}, "max_rel": worst,
+            "min_frac_within_1e-5": min(frac_ok) if frac_ok else None,
+            "mean_frac_within_1e-5": float(np.mean(frac_ok)) if frac_ok else None}
+
+
+def release_errors(r: int, results: dict) -> dict:
+    """One release of the network batch run alone through the float64 oracle
+    (its own terminus, its particle ids, the batch's seed and per-column
+    convection): relative L2 of the engine's footprint slab and the relative
+    error of its residence total, per engine run."""
+
+    import dataclasses
+
+    import numpy as np
+
+    from oracle import run as ORUN
+
+    from glide_amd.synthetic import SyntheticMet, workload_config
+
+    cfg = workload_config(BATCHED)
+    rel = cfg["releases"][r]
+    one = dataclasses.replace(rel, release_idx=0, noise_id=rel.release_idx)
+    c1 = dict(cfg, releases=[one], grid=dict(cfg["grid"], n_r=1))
+    met = SyntheticMet(cfg["spec"])
+    hours = {}
+    t0 = time.perf_counter()
+    ref = ORUN.run(c1, np.float64, met=met, hours_cache=hours, convection=ORUN.column_convection(met, hours, c1["seed"]),
+                   release_terminus=True)
+    g = cfg["grid"]
+    rf = ref.footprint.reshape(g["n_t"], g["n_z"] * g["n_y"] * g["n_x"])
+    out = {"release": r, "particles": one.n_particles, "length_h": one.length_s // 3600,
+           "oracle_seconds": round(time.perf_counter() - t0, 1), "oracle_particle_steps": ref.particle_steps}
+    rtot = float(ref.footprint.sum())
+    # the float32 oracle on the same noise: how far float32 arithmetic alone
+    # carries the full run from the float64 one (the envelope of a float32 engine)
+    h32 = {}
+    f32 = ORUN.run(c1, np.float32, met=met, hours_cache=h32, convection=ORUN.column_convection(met, h32, c1["seed"]),
+                   release_terminus=True)
+    ff = f32.footprint.astype(np.float64)
+    out["float32_oracle"] = {"footprint_rel_l2": float(np.linalg.norm(ff - ref.footprint) / np.linalg.norm(ref.footprint)),
+                             "total_rel_err": abs(float(ff.sum()) - rtot) / rtot}
+    for name, res in results.items():
+        fp = res["footprint"].reshape(-1, g["n_z"] * g["n_y"] * g["n_x"])
+        nt = fp.shape[0]
+        beyond = float(np.abs(rf[nt:]).sum())  # residence in bins past the slab (none expected)
+        l2 = float(np.linalg.norm(fp - rf[:nt]) / np.linalg.norm(rf[:nt]))
+        out[name] = {"footprint_rel_l2": l2, "total_rel_err": abs(res["total"] - rtot) / rtot,
+                     "oracle_residence_beyond_slab": beyond}
+    return out
+
+
+def fp64_checks(evidence: dict) -> dict:
+    out = {}
+    for name, ev in evidence.get("positions", {}).items():
+        out[f"positions_first_met_hour_{name}_met"] = position_errors(ev)
+    rel = [release_errors(r, res) for r, res in sorted(evidence.get("releases", {}).items())]
+    if rel:
+        out["batched_releases"] = rel
+    out["method"] = ("float64 oracle (oracle/) on the same Philox noise; positions: one oracle step from each "
+                     "engine step's float32 start state; releases: the release alone, full run")
+    return out
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -386,175 +879,51 @@ def main() -> int:
         return 2
     if args.copy_calibration_mib:
         return copy_calibration(args.copy_calibration_mib)
-    import numpy as np
     import torch
-
-    from glide_amd import _lib as L
-    from glide_amd.runner import BatchEngine
-    from glide_amd.synthetic import SyntheticMet, workload_config
 
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
     torch.cuda.set_device(0)
-    cfg = workload_config(WORKLOAD)
-    met = SyntheticMet(cfg["spec"])
-    met_dtype = L.GLIDE_MET_BF16 if args.met == "bf16" else L.GLIDE_MET_F32
     stream = torch.cuda.Stream()  # default priority; graph capture needs a non-default stream
     sptr = int(stream.cuda_stream)
     torch.cuda.set_stream(stream)
-    from glide_amd.autotune import tune
 
-    kw = dict(met_dtype=met_dtype, overlap=not args.no_overlap, accumulate=args.accumulate)
-    if args.autotune:
-        tuned = tune(cfg, met, use_cache=False, **kw)
-    else:
-        tuned = {"best": {"reorder_every": args.reorder_every, "step_block": args.step_block}, "trials": []}
-    eng = BatchEngine(cfg, device="cuda", met=met, use_graph=True, **kw, **tuned["best"])
-    n_sched = len(eng.sched)
-    K = max(1, min(args.steps, n_sched))
-    W = max(0, min(args.warmup, n_sched))
+    def note(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-    def timed_run(graph: bool) -> float:
-        """Warm up (captures the step body in graph mode), reset, time K steps."""
-        eng.use_graph = graph
-        eng.step_events = None
-        eng.run(W)
-        torch.cuda.synchronize()
-        eng.reset()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(K):
-            eng.run_step(sptr)
-        eng.join(sptr)  # the last steps' accumulations are part of the timed work
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
-
-    # the other launch mode first (replay vs eager), then the headline run
-    other_elapsed = timed_run(not args.graph)
-    eng.use_graph = args.graph
-    eng.run(W)
-    torch.cuda.synchronize()
-    eng.reset()
-    torch.cuda.synchronize()
-
-    every = max(1, args.time_every)
-    ev = eng.time_step_kernel(K, stream, every=every)  # HIP events around sampled step-kernel launches
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(K):
-        eng.run_step(sptr)
-    eng.join(sptr)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    act = eng.active_steps.cpu().numpy()[:K].astype(np.float64)  # counted by the step kernel
-    psteps = int(act.sum())
-    timed = [(i, a.elapsed_time(b)) for i, (a, b) in sorted(ev.items()) if act[i] > 0]
-    t_ms = np.array([t for _, t in timed])
-    t_act = np.array([act[i] for i, _ in timed])
-    avg_ms = float(t_ms.mean()) if len(t_ms) else float("nan")
-    bpp = BYTES_PER_PARTICLE_STEP[args.accumulate]
-    bytes_launch = bpp * float(t_act.mean()) if len(t_act) else 0.0
-    achieved = (bpp * float(t_act.sum()) / (float(t_ms.sum()) * 1e-3) / 1e9
-                if len(t_ms) else float("nan"))
-    roofline = {
-        "bound": "hbm",
-        "achieved": round(achieved, 2),
-        "peak": PEAK_HBM_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / PEAK_HBM_GBS, 5),
-        "traffic": None,
-        "kernel": (f"glide::step_kernel<{str(met_dtype == L.GLIDE_MET_BF16).lower()}, 0, "
-                   f"{str(args.accumulate == 'fixed').lower()}>"),
-        "kernel_avg_us": round(avg_ms * 1e3, 3),
-        "kernel_launches_timed": len(t_ms),
-        "bytes_per_launch": round(bytes_launch),
-        "bytes_per_particle_step": bpp,
-    }
-    traffic_file = ROOT / "profiles" / f"traffic_{WORKLOAD}_{args.met}_{args.accumulate}.json"
-    if traffic_file.exists():
-        try:
-            roofline["traffic"] = json.loads(traffic_file.read_text()).get("bytes_per_launch")
-        except (OSError, ValueError):
-            roofline["traffic"] = None
-    out = {
-        "metric": METRIC,
-        "value": psteps / elapsed,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": elapsed * 1e3 / K,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if met_dtype == L.GLIDE_MET_F32 else "bf16-met/f32",
-        "data": "synthetic (seeded met cube and release; no network)",
-        "config": {
-            "workload": WORKLOAD,
-            "particles": eng.n,
-            "dt_s": int(cfg["dt"]),
-            "steps_in_run": n_sched,
-            "grid": f"{cfg['spec'].nx}x{cfg['spec'].ny}x{cfg['spec'].za}",
-            "met_hours": len(eng.met.src),  # resident in HBM before the timed region
-            "met_storage": args.met,
-            "convection": "per-column emanuel",
-            "meander": True,
-            "max_substeps": eng.h.max_substeps,
-            "reorder_every": eng.reorder_every,
-            "step_block": eng.step_block or 256,
-            "autotune_trials": tuned["trials"],
-            "step_graph": bool(args.graph),
-            "footprint_accumulation": args.accumulate,
-            "accumulate_on_second_stream": args.accumulate == "sort" and not args.no_overlap,
-            "footprint": f"1x{cfg['grid']['n_t']}x1x{cfg['grid']['n_y']}x{cfg['grid']['n_x']} f32",
-        },
-        "active_particle_steps": psteps,
-        ("eager" if args.graph else "graph_replay"): {
-            "ms_per_step": other_elapsed * 1e3 / K,
-            "value": psteps / other_elapsed,
-        },
-        "roofline": roofline,
-    }
-    if not args.no_other_accumulation:
-        # the same K steps with the other footprint accumulation, on the same met
-        other = "sort" if args.accumulate == "fixed" else "fixed"
-        oe = BatchEngine(cfg, device="cuda", met=met, device_met=eng.met, accumulate=other, overlap=True,
-                         **{k: v for k, v in kw.items() if k not in ("accumulate", "overlap")}, **tuned["best"])
-        oe.run(W)
-        torch.cuda.synchronize()
-        oe.reset()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(K):
-            oe.run_step(sptr)
-        oe.join(sptr)
-        torch.cuda.synchronize()
-        oel = time.perf_counter() - t0
-        out[f"accumulate_{other}"] = {"ms_per_step": oel * 1e3 / K, "value": psteps / oel,
-                                      "method": ("keys and values written by the step, stable radix sort by cell, "
-                                                 "fixed-order float32 reduction on a second stream" if other == "sort"
-                                                 else "exact int64 fixed-point sums inside the step kernel")}
-        oe.close()
-        del oe
-    eng.close()
-    del eng
-    torch.cuda.empty_cache()
+    out, evidence = headline_leg(args, stream, sptr)
+    note(f"headline {out['value']:.4g} particle-steps/s")
     if not args.no_plain_torch:
-        out["plain_torch"] = plain_torch_leg(args, sptr)
+        out["plain_torch"] = plain_torch_leg(args)
         pt = out["plain_torch"]["value"]
         out["vs_plain_torch"] = out["value"] / pt if pt > 0 else None
+        note(f"plain torch {pt:.4g}")
     if not args.no_batched:
-        out["batched"] = batched_leg(args, sptr, out["value"])
-        if "plain_torch" in out:
-            out["batched"]["vs_plain_torch"] = out["batched"]["value"] / pt if pt > 0 else None
+        shared: dict = {}
+        evidence["releases"] = {}
+        names = [BATCHED] + ([] if args.no_batched_bf16 else [BATCHED_BF16])
+        for name in names:
+            key = "batched_bf16" if name.endswith("-bf16") else "batched"
+            out[key] = batched_leg(args, name, stream, sptr, shared, evidence["releases"])
+            out[key]["vs_plain"] = out[key]["value"] / out["value"]
+            if "plain_torch" in out:
+                out[key]["vs_plain_torch"] = out[key]["value"] / out["plain_torch"]["value"]
+            note(f"{name} {out[key]['value']:.4g}")
+        shared.clear()
+        torch.cuda.empty_cache()
     if not args.no_serving:
         out["serving"] = serving_leg(args, out["value"])
+        note("serving done")
     if not args.no_ingest:
         out["ingest"] = ingest_leg(args, sptr, out["value"])
+        note("ingest done")
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_particles, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline_leg(args)
+        note(f"cpu baseline {out['cpu_baseline']['value']:.4g}")
+        if not args.no_fp64_check:
+            out["error_vs_fp64"] = fp64_checks(evidence)
+            note("fp64 checks done")
     print(json.dumps(out), flush=True)
     return 0
 

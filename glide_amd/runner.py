@@ -94,7 +94,7 @@ class DeviceMet:
     window, for a :class:`SyntheticMet` grid."""
 
     def __init__(self, met: SyntheticMet, hours, *, device="cuda", met_dtype: int = L.GLIDE_MET_F32,
-                 meander_coeff: float = 0.16, meander_radius: int = 1, ingest=None):
+                 meander_coeff: float = 0.16, meander_radius: int = 1, ingest=None, sources: dict | None = None):
         self.met = met
         # ingest (glide_amd.ingest.MetIngest): source hours stream in ahead of
         # the cursor instead of being made resident up front
@@ -114,9 +114,13 @@ class DeviceMet:
         self.terrain = _dev(met.terrain, None, device)
         mlon = np.clip(111320.0 * np.cos(np.deg2rad(self.lat)), 1.0, None)
         self.mlon = _dev(mlon, torch.float64, device)
-        self.src = {}
+        # sources: another DeviceMet's resident source hours (the same met in
+        # another window precision), shared instead of uploaded again
+        self.src = sources if sources is not None else {}
         if ingest is None:
             for h in hours:
+                if h in self.src:
+                    continue
                 sh = met.hour(h)
                 self.src[h] = dict(
                     src3=_dev(sh.src3, None, device), src2=_dev(sh.src2, None, device),
@@ -299,7 +303,8 @@ class BatchEngine:
             # length + duration: hourly bins 0 .. floor((length + duration) / 3600)
             nt = np.zeros(g["n_r"], np.int64)
             for r in rels:
-                nt[r.release_idx] = max(nt[r.release_idx], (int(r.length_s) + int(r.duration_s)) // 3600 + 1)
+                loc = int(r.release_idx) - int(rels[0].release_idx)  # batch-local slab (release_arrays)
+                nt[loc] = max(nt[loc], (int(r.length_s) + int(r.duration_s)) // 3600 + 1)
             self.slab_nt_np = np.clip(nt, 1, g["n_t"]).astype(np.int32)
             sizes = self.slab_nt_np.astype(np.int64) * self.cells_per_bin
             self.slab_off_np = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)

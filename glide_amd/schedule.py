@@ -93,11 +93,16 @@ def particle_ids(release) -> np.ndarray:
 
 def release_arrays(releases) -> ReleaseArrays:
     """Particle arrays for a batch of point releases (weight 1/N per release,
-    offsets relative to the earliest release in the batch)."""
+    offsets relative to the earliest release in the batch). ``rel_idx`` is
+    batch-local: a release's ``release_idx`` minus the first release's
+    (main.py:1299-1302), so the batches of one expansion, whose indices are
+    global, each fill footprint slabs from 0; particle ids keep the global
+    index (the noise of a release does not depend on its batch)."""
 
     if not releases:
         raise ValueError("ReleaseBatch must contain at least one release")
     batch_start = min(r.release_time_s for r in releases)
+    first_idx = int(releases[0].release_idx)
     cols = {k: [] for k in ("lon", "lat", "z", "weight", "r_off", "we_off", "rel_idx", "lo_off", "te_off")}
     for r in releases:
         n = int(r.n_particles)
@@ -113,7 +118,75 @@ def release_arrays(releases) -> ReleaseArrays:
         cols["lo_off"].append(ro - np.float32(r.length_s))
         cols["te_off"].append(np.full(n, np.float32(off + r.duration_s) - np.float32(r.length_s), np.float32))
         cols["we_off"].append(np.full(n, off + r.duration_s, np.float32))
-        cols["rel_idx"].append(np.full(n, r.release_idx, np.int32))
+        cols["rel_idx"].append(np.full(n, int(r.release_idx) - first_idx, np.int32))
     a = {k: np.concatenate(v) for k, v in cols.items()}
     uid = np.concatenate([particle_ids(r) for r in releases])
     return ReleaseArrays(uid=uid, batch_start_s=batch_start, **a)
+
+
+@dataclass(frozen=True)
+class Site:
+    """One measurement site of a multi-site release (config.py: sites)."""
+
+    lon: float
+    lat: float
+    alt_agl_m: float
+    name: str = ""
+
+
+def _per_release(v, idx: int):
+    return v[idx] if isinstance(v, (list, tuple, np.ndarray)) else v
+
+
+def expand_to_batches(kind: str, *, duration_s: int, n_particles, seed: int | None, max_releases_per_batch: int,
+                      length_s, start_time_s: int = 0, period_s: int = 0, n_releases: int = 1, times_s=(),
+                      lon: float | None = None, lat: float | None = None, alt_agl_m: float | None = None,
+                      sites=()) -> list[list]:
+    """The release schedule as execution batches (reference
+    RunConfig.expand_to_batches, config.py:530-621).
+
+    ``kind``: "point" (one release at ``start_time_s``), "periodic_point"
+    (``n_releases`` every ``period_s`` from ``start_time_s``), "point_schedule"
+    (one release per time in ``times_s``) at (``lon``, ``lat``,
+    ``alt_agl_m``); "multi_point_periodic": one release per (time, site) in
+    TIME-MAJOR order (every site of one release time, then the next time), so
+    chunks of ``max_releases_per_batch`` consecutive releases span contiguous
+    times across all sites. Release ``i`` (its index in that order) has seed
+    ``seed + i`` (None when ``seed`` is None; a single point release keeps
+    ``seed``). Releases are chunked into batches of at most
+    ``max_releases_per_batch``. ``n_particles`` and ``length_s`` (the backward
+    length; the reference has one per run) may be sequences indexed by
+    release. Times are seconds after the met epoch. Returns a list of batches,
+    each a list of ``glide_amd.synthetic.Release``."""
+
+    from glide_amd.synthetic import Release
+
+    if max_releases_per_batch < 1:
+        raise ValueError("max_releases_per_batch must be >= 1")
+    if kind == "point":
+        return [[Release(lon=float(lon), lat=float(lat), alt_agl_m=float(alt_agl_m),
+                         n_particles=int(_per_release(n_particles, 0)), release_time_s=int(start_time_s),
+                         duration_s=int(duration_s), length_s=int(_per_release(length_s, 0)),
+                         seed=0 if seed is None else int(seed), release_idx=0)]]
+    if kind in ("periodic_point", "multi_point_periodic"):
+        times = [int(start_time_s) + i * int(period_s) for i in range(int(n_releases))]
+    elif kind == "point_schedule":
+        times = [int(t) for t in times_s]
+    else:
+        raise TypeError(f"Unsupported release variant: {kind}")
+    if kind == "multi_point_periodic":
+        places = [(float(st.lon), float(st.lat), float(st.alt_agl_m)) for st in sites]
+        if not places:
+            raise ValueError("multi_point_periodic needs sites")
+    else:
+        places = [(float(lon), float(lat), float(alt_agl_m))]
+    concretes = []
+    idx = 0
+    for t in times:  # time-major: every site of one release time, then the next time
+        for plon, plat, palt in places:
+            concretes.append(Release(lon=plon, lat=plat, alt_agl_m=palt,
+                                     n_particles=int(_per_release(n_particles, idx)), release_time_s=t,
+                                     duration_s=int(duration_s), length_s=int(_per_release(length_s, idx)),
+                                     seed=(int(seed) + idx) if seed is not None else 0, release_idx=idx))
+            idx += 1
+    return [concretes[i:i + max_releases_per_batch] for i in range(0, len(concretes), max_releases_per_batch)]

@@ -258,7 +258,7 @@ class Release:
     duration_s: int = 60
     length_s: int = 3 * 3600  # backward integration length
     seed: int = 0
-    release_idx: int = 0  # footprint slab of the release within its batch
+    release_idx: int = 0  # index in the run's release order; minus the batch's first: its footprint slab
     noise_id: int | None = None  # particle ids are (noise_id % 4096) << 20 | k; None = release_idx
 
 
@@ -277,30 +277,31 @@ def _domain_and_grid(spec: MetSpec, n_r: int, n_t: int):
 
 def network_releases(spec: MetSpec, *, n_sites: int, n_hourly: int, first_hour: int, lengths_h,
                      n_range: tuple[int, int], seed: int, duration_s: int = 60) -> list[Release]:
-    """A measurement network's hourly releases (the reference's multi-site
-    batches, scripts/make_multisite_config.py): ``n_sites`` sites at seeded
-    interior positions with inlet heights uniform on [10, 300] m, one release
-    per site per hour for
-    ``n_hourly`` hours from ``first_hour``, particle counts log-uniform in
-    ``n_range``, backward lengths drawn from ``lengths_h`` (hours)."""
+    """A measurement network's hourly releases as one batch: ``n_sites`` sites
+    at seeded interior positions with inlet heights uniform on [10, 300] m,
+    one release per site per hour for ``n_hourly`` hours from ``first_hour``
+    (2 minutes before the hour), in the reference's time-major order with
+    seeds ``seed + release_idx`` (``schedule.expand_to_batches``,
+    "multi_point_periodic"); per release a particle count log-uniform on
+    ``n_range`` and a backward length drawn from ``lengths_h`` (hours)."""
+
+    from glide_amd.schedule import Site, expand_to_batches
 
     rng = np.random.default_rng(seed)
     lon, lat = spec.lon, spec.lat
-    out = []
-    idx = 0
-    for s in range(n_sites):
+    sites = []
+    for _ in range(n_sites):
         slon = float(rng.uniform(lon[len(lon) // 4], lon[3 * len(lon) // 4]))
         slat = float(rng.uniform(min(lat[len(lat) // 4], lat[3 * len(lat) // 4]),
                                  max(lat[len(lat) // 4], lat[3 * len(lat) // 4])))
-        inlet = float(rng.uniform(10.0, 300.0))
-        for k in range(n_hourly):
-            n = int(round(float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))))
-            length_h = int(rng.choice(np.asarray(lengths_h)))
-            out.append(Release(lon=slon, lat=slat, alt_agl_m=inlet, n_particles=n,
-                               release_time_s=(first_hour + k) * 3600 - 120, duration_s=duration_s,
-                               length_s=length_h * 3600, seed=seed + 1 + idx, release_idx=idx))
-            idx += 1
-    return out
+        sites.append(Site(lon=slon, lat=slat, alt_agl_m=float(rng.uniform(10.0, 300.0))))
+    n_rel = n_sites * n_hourly
+    counts = [int(round(float(np.exp(v)))) for v in rng.uniform(np.log(n_range[0]), np.log(n_range[1]), n_rel)]
+    lengths = [int(v) * 3600 for v in rng.choice(np.asarray(lengths_h), n_rel)]
+    (batch,) = expand_to_batches("multi_point_periodic", start_time_s=first_hour * 3600 - 120, period_s=3600,
+                                 n_releases=n_hourly, sites=sites, duration_s=duration_s, n_particles=counts,
+                                 length_s=lengths, seed=seed + 1, max_releases_per_batch=n_rel)
+    return batch
 
 
 def workload_config(name: str):
@@ -316,6 +317,12 @@ def workload_config(name: str):
       7 sites x 48 hourly releases, 2k-20k particles each (log-uniform), 24 to
       120 h back, 240x160x60 met over 169 hourly snapshots, convection on,
       footprint 336 releases x 120 time-ago bins x one layer.
+    * ``batched-gpu-2688r`` (float32 met) / ``batched-gpu-2688r-bf16`` (the
+      same releases; the precision layout is chosen by the engine): the full
+      network batch, 56 sites x 48 hourly releases (2,688), 2k-20k particles
+      each, 24 to 120 h back, 240x160x60 met over 169 hourly snapshots,
+      convection on, footprint 2,688 releases x up to 120 time-ago bins x one
+      layer in per-release slabs.
     * ``batched-cpu-6r``: a small network batch on the 64x48x40 grid: 2 sites x
       3 hourly releases of 8k-16k particles, 1 to 3 h back, convection off.
     """
@@ -341,6 +348,15 @@ def workload_config(name: str):
                                 n_range=(2000, 20000), seed=20240103)
         grid, domain = _domain_and_grid(spec, len(rels), 120)
         return dict(spec=spec, releases=rels, grid=grid, domain=domain, convection=True, dt=60, seed=11)
+    if name in ("batched-gpu-2688r", "batched-gpu-2688r-bf16"):
+        # The network batch at its full size: 56 sites x 48 hourly releases, one
+        # engine; 168 hourly snapshots cover the 48 release hours and the
+        # longest (120 h) backward window.
+        spec = MetSpec(nx=240, ny=160, za=60, n_hours=169, seed=20240106, lon0=2.0, lat0=72.0)
+        rels = network_releases(spec, n_sites=56, n_hourly=48, first_hour=121, lengths_h=(24, 48, 72, 96, 120),
+                                n_range=(2000, 20000), seed=20240106)
+        grid, domain = _domain_and_grid(spec, len(rels), 120)
+        return dict(spec=spec, releases=rels, grid=grid, domain=domain, convection=True, dt=60, seed=23)
     if name == "batched-cpu-6r":
         spec = MetSpec(nx=64, ny=48, za=40, n_hours=7, seed=20240104)
         rels = network_releases(spec, n_sites=2, n_hourly=3, first_hour=4, lengths_h=(1, 2, 3),

@@ -126,7 +126,7 @@ def run(cfg, dty=np.float64, params: lpdm.HannaParams | None = None, met=None,
         lo_l.append(ro - np.float32(r.length_s))  # each release's own backward length
         te_l.append(np.full(n, np.float32(off + r.duration_s) - np.float32(r.length_s), np.float32))
         weoff_l.append(np.full(n, off + r.duration_s, np.float32))
-        ridx_l.append(np.full(n, r.release_idx, np.int64))
+        ridx_l.append(np.full(n, r.release_idx - rels[0].release_idx, np.int64))  # batch-local (main.py:1299-1302)
     P = {k: np.concatenate(v) for k, v in P.items()}
     for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm"):
         P[k] = P[k].astype(dty)
@@ -190,3 +190,60 @@ def run(cfg, dty=np.float64, params: lpdm.HannaParams | None = None, met=None,
     res.step_seconds = t_step
     res.total_seconds = time.perf_counter() - t_all
     return res
+
+
+def column_convection(met, hours: dict, seed: int, levels=None):
+    """Per-column convection for ``run(convection=...)``: on the first step of
+    each met hour (after the step, before the gridding), every active particle
+    is redistributed through its nearest column's matrix with the engine's
+    trigger, closure and Philox stream (reference emanuel.py:570-805 applied
+    per column). ``hours`` must be the dict ``run`` fills (``hours_cache``)."""
+
+    from oracle import convection as oc
+
+    levels = met.spec.agl_levels if levels is None else levels
+
+    def convect(P, active, bh, alpha, noise_step):
+        s3, _, p_hpa = hours[bh]
+        e3, _, _ = hours[bh + 1]
+        conv = met.hour(bh).conv
+        lcl, lnb, mb = oc.column_descriptors(conv, s3[3].astype(np.float32), e3[3].astype(np.float32), alpha,
+                                             p_hpa, np.asarray(levels, np.float32))
+        ny, nx = lcl.shape
+        col = oc.nearest_column(P["lon"].astype(np.float32), P["lat"].astype(np.float32), met.lon[0],
+                                met.lon[-1], met.lat[0], met.lat[-1], nx, ny)
+        out = dict(P)
+        out["z"] = oc.convect_columns(P["z"], active, col, lcl, lnb, mb.astype(np.float64), p_hpa, levels, seed,
+                                      P["uid"], noise_step).astype(P["z"].dtype)
+        return out
+
+    return convect
+
+
+def torch_windows(met, levels=None, hours=None):
+    """Window provider for ``glide_amd.torch_plain.TorchPlain`` on the CPU:
+    bracket hour ``bh``'s fields from this module's float32 window builder
+    (resample, support stacks, meander), as CPU torch tensors in the layout
+    TorchPlain reads. ``hours`` caches resampled hours across calls."""
+
+    import torch
+
+    levels = met.spec.agl_levels if levels is None else levels
+    hours = {} if hours is None else hours
+
+    def hour(h):
+        if h not in hours:
+            sh = met.hour(h)
+            hours[h] = ors.resample_hour(sh.src3, sh.src2, sh.height_agl, met.terrain, met.lat, met.lon, levels,
+                                         sh.p_level_pa)
+        return hours[h]
+
+    def windows(bh: int) -> dict:
+        hs, he = hour(bh), hour(bh + 1)
+        w = build_window(hs, he, levels, met.lon, met.lat, np.float32)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32))  # noqa: E731
+        return dict(uvw=t(np.concatenate([w.uvw_s, w.uvw_e])), ft=t(w.ft), dens=t(w.dens), mea=t(w.meander),
+                    surf_s=t(w.surf_s), surf_e=t(w.surf_e), t_s=t(hs[0][3]), t_e=t(he[0][3]),
+                    conv=t(met.hour(bh).conv), p_hpa=torch.from_numpy(np.asarray(hs[2], np.float64)))
+
+    return windows

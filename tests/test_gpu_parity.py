@@ -410,31 +410,6 @@ def test_full_run_cpu_workload_footprint(cfg0):
     print(f"full run: rel L2 {rel_l2:.3e}, total {tot_dev:.6f} vs {ref.footprint.sum():.6f}")
 
 
-def oracle_convection(met, hours, seed):
-    """The oracle's per-column convection for ORUN.run: on the first step of
-    each met hour (after the step, before the gridding), every active particle
-    is redistributed through its nearest column's matrix with the same trigger,
-    closure and Philox stream as the engine."""
-
-    levels = met.spec.agl_levels
-
-    def convect(P, active, bh, alpha, noise_step):
-        s3, _, p_hpa = hours[bh]
-        e3, _, _ = hours[bh + 1]
-        conv = met.hour(bh).conv
-        lcl, lnb, mb = OC.column_descriptors(conv, s3[3].astype(np.float32), e3[3].astype(np.float32), alpha,
-                                             p_hpa, levels.astype(np.float32))
-        ny, nx = lcl.shape
-        col = OC.nearest_column(P["lon"].astype(np.float32), P["lat"].astype(np.float32), met.lon[0], met.lon[-1],
-                                met.lat[0], met.lat[-1], nx, ny)
-        out = dict(P)
-        out["z"] = OC.convect_columns(P["z"], active, col, lcl, lnb, mb.astype(np.float64), p_hpa, levels,
-                                      seed, P["uid"], noise_step).astype(P["z"].dtype)
-        return out
-
-    return convect
-
-
 def test_full_run_cpu_workload_with_convection_footprint():
     """The 10k-particle / 3 h run with per-column convection on every new met
     hour, against the float64 oracle on the same noise: footprint within 1 %
@@ -458,10 +433,10 @@ def test_full_run_cpu_workload_with_convection_footprint():
     fp = eng.footprint.cpu().numpy().astype(np.float64)
     hours = {}
     ref = ORUN.run(cfg, dty=np.float64, met=met, hours_cache=hours,
-                   convection=oracle_convection(met, hours, cfg["seed"]))
+                   convection=ORUN.column_convection(met, hours, cfg["seed"]))
     hours32 = {}
     f32 = ORUN.run(cfg, dty=np.float32, met=met, hours_cache=hours32,
-                   convection=oracle_convection(met, hours32, cfg["seed"]))
+                   convection=ORUN.column_convection(met, hours32, cfg["seed"]))
     total = ref.footprint.sum()
     rel_l2 = np.linalg.norm(fp - ref.footprint) / np.linalg.norm(ref.footprint)
     assert rel_l2 <= 0.01, rel_l2

@@ -104,3 +104,54 @@ def test_met_ingest_rejects_bad_arguments():
     for kw in (dict(workers=0), dict(lookahead=-1), dict(max_resident=1)):
         with pytest.raises(ValueError, match="need workers"):
             MetIngest(object(), **kw)
+
+
+def test_expand_to_batches_order_seeds_and_chunks():
+    """RunConfig.expand_to_batches (config.py:530-621): time-major (time, site)
+    order, seed + release_idx, chunks of max_releases_per_batch consecutive
+    releases with global indices; a later batch's particles fill footprint
+    slabs from 0 (main.py:1299-1302) and keep their global noise ids."""
+    from glide_amd.schedule import Site, expand_to_batches
+
+    sites = [Site(1.0, 50.0, 10.0, "a"), Site(2.0, 51.0, 20.0, "b"), Site(3.0, 52.0, 30.0, "c")]
+    batches = expand_to_batches("multi_point_periodic", start_time_s=7200, period_s=3600, n_releases=4, sites=sites,
+                                duration_s=60, n_particles=5, length_s=3600, seed=100, max_releases_per_batch=5)
+    flat = [r for b in batches for r in b]
+    assert [len(b) for b in batches] == [5, 5, 2]
+    assert [r.release_idx for r in flat] == list(range(12))
+    assert [r.seed for r in flat] == [100 + i for i in range(12)]
+    assert [(r.release_time_s, r.lon) for r in flat] == [(7200 + 3600 * t, s.lon) for t in range(4) for s in sites]
+    assert [r.alt_agl_m for r in flat[:3]] == [10.0, 20.0, 30.0]
+    # per-release particle counts and lengths; no seed: every release seed 0 here (None in the reference)
+    per = expand_to_batches("periodic_point", start_time_s=0, period_s=600, n_releases=3, lon=5.0, lat=6.0,
+                            alt_agl_m=7.0, duration_s=60, n_particles=[3, 4, 5], length_s=[60, 120, 180], seed=None,
+                            max_releases_per_batch=2)
+    assert [[r.n_particles for r in b] for b in per] == [[3, 4], [5]]
+    assert [r.length_s for b in per for r in b] == [60, 120, 180]
+    assert [r.release_time_s for b in per for r in b] == [0, 600, 1200]
+    sch = expand_to_batches("point_schedule", times_s=[50, 10, 30], lon=5.0, lat=6.0, alt_agl_m=7.0, duration_s=60,
+                            n_particles=2, length_s=60, seed=1, max_releases_per_batch=10)
+    assert [r.release_time_s for r in sch[0]] == [50, 10, 30] and [r.seed for r in sch[0]] == [1, 2, 3]
+    (one,) = expand_to_batches("point", start_time_s=90, lon=5.0, lat=6.0, alt_agl_m=7.0, duration_s=60,
+                               n_particles=8, length_s=60, seed=42, max_releases_per_batch=3)
+    assert len(one) == 1 and one[0].seed == 42 and one[0].release_idx == 0
+    with pytest.raises(TypeError):
+        expand_to_batches("area", duration_s=60, n_particles=1, seed=0, max_releases_per_batch=1, length_s=60)
+    # the third batch: slabs 0..1, noise ids 10..11, the same particle ids as in one batch of all 12
+    ra = release_arrays(batches[2])
+    assert np.array_equal(np.unique(ra.rel_idx), [0, 1])
+    whole = release_arrays(flat)
+    assert np.array_equal(ra.uid, whole.uid[-10:])
+    assert np.array_equal(np.unique(whole.rel_idx), np.arange(12))
+
+
+def test_network_releases_follow_expand_to_batches():
+    """The synthetic measurement network is one multi-site periodic expansion:
+    time-major, seeds seed + 1 + release_idx, one release per site per hour."""
+    cfg = workload_config("batched-cpu-6r")
+    rels = cfg["releases"]
+    assert [r.release_idx for r in rels] == list(range(6))
+    assert [r.seed for r in rels] == [rels[0].seed + i for i in range(6)]
+    times = [r.release_time_s for r in rels]
+    assert times == sorted(times) and len(set(times)) == 3  # 3 hours x 2 sites, time-major
+    assert [(r.lon, r.lat) for r in rels[:2]] == [(r.lon, r.lat) for r in rels[2:4]]
