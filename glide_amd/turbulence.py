@@ -17,7 +17,10 @@ density and meander stacks built by ``glide_build_window``.
 
 Noise is the engine's counter-based Philox stream keyed by (seed, particle id)
 and counted by (step, substep), so a step's result does not depend on which
-particles are active or on launch order.
+particles are active or on launch order. The particle id is
+``state["particle_id"]`` (int32) when the state holds one, else the row index;
+the step counter is model time (glide_amd.noise), so it does not depend on how
+many calls came before.
 """
 
 from __future__ import annotations
@@ -31,6 +34,7 @@ import torch
 
 from glide_amd import _lib as L
 from glide_amd.met import HourlyMetTensors
+from glide_amd.noise import NoiseClock
 
 TurbulenceState: TypeAlias = dict[str, torch.Tensor]
 
@@ -128,7 +132,12 @@ class HannaScheme(TurbulenceScheme):
             raise ValueError("z_ubl_m must be >= 0")
         if met_storage not in ("f32", "bf16"):
             raise ValueError("met_storage must be 'f32' or 'bf16'")
-        del static_substeps  # one path: the fused kernel (static == dynamic with counter noise)
+        # The reference's static path (a fixed max_substeps loop, finished
+        # particles advanced by sub_dt = 0) and dynamic path (each particle's own
+        # k substeps over the active subset) give the same result here: the fused
+        # kernel runs each particle's own k substeps and the counter-based noise
+        # does not depend on the loop shape. Kept for the interface.
+        self.static_substeps = static_substeps
         self.meander_enabled = bool(meander_enabled)
         self.meander_coefficient = float(meander_coefficient)
         self.meander_stencil_radius = int(meander_stencil_radius)
@@ -139,7 +148,11 @@ class HannaScheme(TurbulenceScheme):
         self.flexpart_tl_floors = bool(flexpart_tl_floors)
         self.surface_layer_override = bool(surface_layer_override)
         self.seed = int(seed)
-        self.step_index = 0  # Philox counter word 1; advanced by every step()
+        # Philox counter word 1. None (default): model time (glide_amd.noise);
+        # an int set by the caller is used, with the seed as the key, and
+        # advanced by one per step() instead.
+        self.step_index: int | None = None
+        self.clock = NoiseClock()
         self.met_dtype = L.GLIDE_MET_BF16 if met_storage == "bf16" else L.GLIDE_MET_F32
         self._win_key = None
         self._win = None
@@ -240,8 +253,8 @@ class HannaScheme(TurbulenceScheme):
 
         if particles.ndim != 2 or particles.shape[1] != 4 or particles.dtype != torch.float32:
             raise ValueError("particles must be a float32 (N, 4) [lon, lat, alt, weight] tensor")
-        if not particles.is_cuda or particles.stride(1) != 1:
-            raise ValueError("particles must be a row-contiguous CUDA tensor")
+        if not particles.is_cuda:
+            raise ValueError("particles must be a CUDA tensor")
         n = int(particles.shape[0])
         if active_mask.shape != (n,):
             raise ValueError("active_mask must have shape (N,)")
@@ -253,22 +266,38 @@ class HannaScheme(TurbulenceScheme):
             if t.shape != (n,) or t.dtype != torch.float32 or not t.is_contiguous() or t.device != device:
                 raise ValueError(f"state[{k!r}] must be a contiguous float32 (N,) tensor on {device}")
         am = active_mask.to(device=device, dtype=torch.bool).contiguous()
-        if self._uid is None or self._uid.numel() != n or self._uid.device != device:
-            self._uid = torch.arange(n, dtype=torch.int32, device=device)
-        base = particles.data_ptr()
+        pid = state.get("particle_id")
+        if pid is not None:
+            if pid.shape != (n,) or pid.dtype != torch.int32 or pid.device != device:
+                raise ValueError(f"state['particle_id'] must be an int32 (N,) tensor on {device}")
+            uid = pid.contiguous()
+        else:  # the row index
+            if self._uid is None or self._uid.numel() != n or self._uid.device != device:
+                self._uid = torch.arange(n, dtype=torch.int32, device=device)
+            uid = self._uid
+        # rows of 4 adjacent floats at any row stride; another layout through a contiguous copy
+        work = particles if particles.stride(1) == 1 else particles.contiguous()
+        if self.step_index is None:
+            step_index, key = self.clock.next(met_window.metadata, t_alpha, dt_seconds, self.seed)
+        else:
+            step_index, key = int(self.step_index), self.seed
+        base = work.data_ptr()
         parts = L.GlideParticles(
-            n=n, pos_stride=particles.stride(0), lon=base, lat=base + 4, z=base + 8,
+            n=n, pos_stride=work.stride(0), lon=base, lat=base + 4, z=base + 8,
             up=state["u_prime"].data_ptr(), vp=state["v_prime"].data_ptr(), wp=state["w_prime"].data_ptr(),
             um=state["u_meander"].data_ptr() if self.meander_enabled else None,
             vm=state["v_meander"].data_ptr() if self.meander_enabled else None,
-            weight=base + 12, r_off=None, we_off=None, rel_idx=None, uid=self._uid.data_ptr(), alive=None,
+            weight=base + 12, r_off=None, we_off=None, rel_idx=None, uid=uid.data_ptr(), alive=None,
             active=am.data_ptr())
         p = L.GlideStepParams(dt=float(dt_seconds), alpha=float(t_alpha), cursor_off=0.0, length_s=0.0,
                               substep_c=self.substep_c, z_ubl_m=self.z_ubl_m,
-                              meander_timescale_s=self.meander_timescale_seconds, seed=self.seed,
-                              step_index=self.step_index, max_substeps=self.max_substeps, flags=self._flags(),
-                              reserved=0)
+                              meander_timescale_s=self.meander_timescale_seconds, seed=key,
+                              step_index=step_index, max_substeps=self.max_substeps, flags=self._flags(),
+                              block=0)
         L.check(L.lib().glide_step(C.byref(p), C.byref(parts), C.byref(win["grid"]), C.byref(win["win"]),
                                    None, None, L.stream_ptr()))
-        self.step_index += 1
+        if work is not particles:
+            particles.copy_(work)
+        if self.step_index is not None:
+            self.step_index += 1
         return particles, state

@@ -363,7 +363,7 @@ def plume_gridder_bounds():
 def wmc_met(*, blh_m: float, ustar_m_s: float = 0.4, p_top_hpa: float = 600.0, z_top: float | None = None,
             n_lev: int = 6, n_lat: int = 4, n_lon: int = 4):
     """Spatially uniform neutral BL met window (reference _wmc_met_window and
-    _wmc_density_met_window) as (HourlyMetTensors, numpy parts)."""
+    _wmc_density_met_window) as HourlyMetTensors of CPU tensors."""
 
     import datetime as _dt
 

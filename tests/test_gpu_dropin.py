@@ -133,3 +133,56 @@ def test_resample_api_bit_exact_and_hanna_step_matches_engine_and_oracle():
 def _src(met, h):
     sh = met.hour(h)
     return (sh.src3, sh.src2, sh.height_agl, met.terrain, met.lat, met.lon, met.spec.agl_levels, sh.p_level_pa)
+
+
+def test_hanna_step_noise_keys_and_particle_layouts():
+    """HannaScheme.step keys its noise by model time (a fresh instance at t
+    equals one first called later in time; a repeated call at one model time
+    and a new backward sweep draw afresh), by ``state["particle_id"]`` when
+    given (a row permutation permutes the result), and takes the (N, 4) block
+    at any layout (a strided view and a transposed one are updated in place,
+    bit-identical to the contiguous case)."""
+    import physics_gates as G
+
+    met = G.wmc_met(blh_m=1500.0)
+    met = HourlyMetTensors(met.hour_start.cuda(), met.hour_end.cuda(), met.metadata, met.channel_names,
+                           met.height_agl_m.cuda())
+    n = 4096
+    g = torch.Generator(device="cpu").manual_seed(5)
+    base = torch.zeros((n, 4))
+    base[:, 1] = 45.0
+    base[:, 2] = torch.empty(n).uniform_(10.0, 1400.0, generator=g)
+    base[:, 3] = 1.0 / n
+    base = base.cuda()
+    active = torch.ones(n, dtype=torch.bool, device="cuda")
+    eng = GPUEngine()
+
+    def run(scheme, alpha, parts=None, state=None):
+        parts = base.clone() if parts is None else parts
+        st = scheme.initialize_state(n, device=torch.device("cuda"), dtype=torch.float32)
+        if state:
+            st.update(state)
+        out, _ = scheme.step(parts, st, met, alpha, 60.0, active, eng)
+        torch.cuda.synchronize()
+        return out
+
+    mk = lambda: HannaScheme(seed=9)  # noqa: E731
+    fresh = run(mk(), 0.25)
+    s = mk()
+    run(s, 0.75)
+    assert torch.equal(run(s, 0.25), fresh)
+    assert not torch.equal(run(s, 0.25), fresh)  # a repeat at the same model time
+    assert not torch.equal(run(s, 0.5), run(mk(), 0.5))  # a new sweep
+    perm = torch.randperm(n, generator=torch.Generator().manual_seed(4)).cuda()
+    ids = torch.arange(n, dtype=torch.int32, device="cuda")
+    a = run(mk(), 0.25, state={"particle_id": ids})
+    b = run(mk(), 0.25, parts=base[perm].contiguous(), state={"particle_id": ids[perm].contiguous()})
+    assert torch.equal(a[perm], b)
+    wide = torch.zeros((n, 7), device="cuda")
+    wide[:, 2:6] = base
+    out = run(mk(), 0.25, parts=wide[:, 2:6])
+    assert out.data_ptr() == wide[:, 2:6].data_ptr() and torch.equal(out, fresh)
+    assert torch.all(wide[:, :2] == 0) and torch.all(wide[:, 6] == 0)
+    tr = base.t().contiguous().t()  # column-major (N, 4)
+    out = run(mk(), 0.25, parts=tr)
+    assert out.data_ptr() == tr.data_ptr() and torch.equal(out, fresh)
