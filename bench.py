@@ -768,8 +768,7 @@ def position_errors(ev: dict) -> dict:
             sh = met.hour(h)
             from oracle import resample as ors
 
-            hours[h] = ors.resample_hour(sh.src3, sh.src2, sh.height_agl, met.terrain, met.lat, met.lon, levels,
-                                         sh.p_level_pa)
+            hours[h] = ors.resample_source_hour(sh, met, levels)
         return hours[h]
 
     conv = ORUN.column_convection(met, hours, ev["seed"]) if cfg.get("convection") else None

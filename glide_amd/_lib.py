@@ -37,6 +37,7 @@ GLIDE_SLOT_FREE = 0
 GLIDE_SLOT_RUNNING = 1
 GLIDE_SLOT_FINISHED = 2
 GLIDE_ROW_WORDS = 66  # int32 words of the step cursor buffer (glide_step_table)
+GLIDE_SHF_UPWARD_W_M2, GLIDE_SHF_DOWNWARD_W_M2, GLIDE_SHF_ACCUMULATED_J_M2 = 0, 1, 2
 GLIDE_NOALLOC_MAX_PAIRS = 1 << 20  # pairs glide_footprint_accumulate_noalloc serves
 
 vp = C.c_void_p
@@ -148,6 +149,8 @@ SIGNATURES = {
     "glide_resample_hour": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp, vp, C.c_int32,
                                       vp, vp, C.c_double, C.c_double, vp, C.c_int32, C.c_int32, C.c_int32,
                                       vp, vp, vp]),
+    "glide_source_units": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, vp,
+                                     vp, vp, vp, vp, C.c_int32, C.c_double, vp]),
     "glide_build_window": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                      C.c_int32, C.c_int32, vp, vp, vp, vp]),
     "glide_convect_columns": (C.c_int, [vp, vp, vp, C.c_double, vp, vp, C.c_int32, C.c_int32, C.c_int32,

@@ -15,7 +15,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libglide_hip.so"
-SOURCES = ["abi.hip", "step.hip", "sort.hip", "footprint.hip", "reorder.hip", "met.hip", "convect.hip", "diag.hip", "fixed.hip", "hanna_eval.hip"]
+SOURCES = ["abi.hip", "step.hip", "sort.hip", "footprint.hip", "reorder.hip", "met.hip", "convect.hip", "diag.hip", "fixed.hip", "hanna_eval.hip", "units.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

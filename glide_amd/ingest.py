@@ -34,6 +34,8 @@ from concurrent.futures import Future, ThreadPoolExecutor
 import numpy as np
 import torch
 
+from glide_amd.met import convert_source_units
+
 FIELDS = ("src3", "src2", "height", "p_hpa", "conv")
 _TORCH_DTYPE = {np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64}
 
@@ -65,8 +67,13 @@ class MetIngest:
     def _decode(self, h: int) -> dict:
         t0 = time.perf_counter()
         sh = self.source.hour(h)
-        arrays = dict(src3=sh.src3, src2=sh.src2, height=sh.height_agl,
-                      p_hpa=np.asarray(sh.p_level_pa, np.float64) / 100.0, conv=sh.conv)
+        arrays = dict(src3=sh.src3, src2=sh.src2, p_hpa=np.asarray(sh.p_level_pa, np.float64) / 100.0,
+                      conv=sh.conv)
+        if getattr(sh, "units", "final") == "source":  # converted on the device after the upload
+            arrays.update(geopotential=sh.geopotential, geopotential_sfc=sh.geopotential_sfc,
+                          p_pa=np.asarray(sh.p_level_pa, np.float64))
+        else:
+            arrays["height"] = sh.height_agl
         pinned = {}
         for k, a in arrays.items():
             t = torch.empty(a.shape, dtype=_TORCH_DTYPE[np.dtype(a.dtype)], pin_memory=True)
@@ -118,6 +125,10 @@ class MetIngest:
             with torch.cuda.stream(self.copy_stream):
                 arrs = {k: torch.empty(v.shape, dtype=v.dtype, device=self.device).copy_(v, non_blocking=True)
                         for k, v in pinned.items()}
+                if "geopotential" in arrs:  # a raw hour: the reader's conversions, on the copy stream
+                    arrs["height"] = convert_source_units(
+                        arrs["src3"], arrs["src2"], arrs.pop("geopotential"), arrs.pop("geopotential_sfc"),
+                        arrs.pop("p_pa"), stream=self.copy_stream)
             ev = torch.cuda.Event()
             ev.record(self.copy_stream)
             self.stats["uploads"] += 1

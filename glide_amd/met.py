@@ -99,3 +99,52 @@ def resample_hour_to_agl(src3: torch.Tensor, height_agl_p: torch.Tensor, terrain
         mlon.data_ptr(), float(dl[0]), float(dn[0]), p_d.data_ptr(), iu, iv, iw, out.data_ptr(),
         p_out.data_ptr(), L.stream_ptr(stream)))
     return out, p_out
+
+
+def convert_source_units(src3: torch.Tensor, src2: torch.Tensor, geopotential: torch.Tensor | None,
+                         geopotential_sfc: torch.Tensor | None, p_level_pa: torch.Tensor, *, w_ch: int = 2,
+                         t_ch: int = 3, shf_ch: int = 3, shf_units: int = L.GLIDE_SHF_ACCUMULATED_J_M2,
+                         accumulation_s: float = 3600.0, convert_w: bool = True, stream=None) -> torch.Tensor | None:
+    """The reader's per-hour conversions of one raw hour on the device, in
+    place (``glide_source_units``, replacing met_reader.py:1052-1078,
+    1268-1300, 1353-1385): omega in ``src3[w_ch]`` to w (when ``convert_w``),
+    ``src2[shf_ch]`` to upward W/m^2, and, when geopotential is given, the
+    per-column heights above ground, returned as a new [Zp, Y, X] tensor."""
+
+    C3, Zp, Y, X = src3.shape
+    for t, name in ((src3, "src3"), (src2, "src2")):
+        if t.dtype != torch.float32 or not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous float32")
+    p = p_level_pa.to(device=src3.device, dtype=torch.float64).contiguous()
+    height = None
+    if geopotential is not None:
+        geopotential = geopotential.to(torch.float32).contiguous()
+        geopotential_sfc = geopotential_sfc.to(torch.float32).contiguous()
+        height = torch.empty((Zp, Y, X), dtype=torch.float32, device=src3.device)
+    shf = src2[shf_ch]
+    L.check(L.lib().glide_source_units(
+        src3.data_ptr(), C3, Zp, Y, X, w_ch if convert_w else -1, t_ch, p.data_ptr(),
+        geopotential.data_ptr() if geopotential is not None else None,
+        geopotential_sfc.data_ptr() if geopotential is not None else None,
+        height.data_ptr() if height is not None else None, shf.data_ptr(), int(shf_units), float(accumulation_s),
+        L.stream_ptr(stream)))
+    return height
+
+
+def upload_source_hour(sh, device) -> dict:
+    """Device arrays (src3, src2, height, p_hpa, conv) of one source hour on
+    the current stream; an hour in the reader's raw units is converted on the
+    device."""
+
+    def dev(a, dtype=None):
+        t = torch.from_numpy(np.ascontiguousarray(a))
+        return (t.to(dtype) if dtype is not None else t).to(device)
+
+    out = dict(src3=dev(sh.src3), src2=dev(sh.src2), p_hpa=dev(np.asarray(sh.p_level_pa) / 100.0, torch.float64),
+               conv=dev(sh.conv))
+    if getattr(sh, "units", "final") == "source":
+        out["height"] = convert_source_units(out["src3"], out["src2"], dev(sh.geopotential),
+                                             dev(sh.geopotential_sfc), dev(np.asarray(sh.p_level_pa), torch.float64))
+    else:
+        out["height"] = dev(sh.height_agl)
+    return out

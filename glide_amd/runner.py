@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from glide_amd import _lib as L
+from glide_amd.met import upload_source_hour
 from glide_amd.schedule import cursor_schedule, release_arrays
 from glide_amd.synthetic import SyntheticMet
 
@@ -121,13 +122,7 @@ class DeviceMet:
             for h in hours:
                 if h in self.src:
                     continue
-                sh = met.hour(h)
-                self.src[h] = dict(
-                    src3=_dev(sh.src3, None, device), src2=_dev(sh.src2, None, device),
-                    height=_dev(sh.height_agl, None, device),
-                    p_hpa=_dev(sh.p_level_pa / 100.0, torch.float64, device),
-                    conv=_dev(sh.conv, None, device),
-                )
+                self.src[h] = upload_source_hour(met.hour(h), device)
             self.zp = next(iter(self.src.values()))["src3"].shape[1]
         else:
             first = max(hours) if ingest.direction < 0 else min(hours)

@@ -94,6 +94,11 @@ class MetSpec:
     first_layer_m: float = 10.0
     n_modes: int = 16
     wind_rms: float = 10.0
+    # source_units: hours in the reader's raw units (omega in Pa/s for w,
+    # geopotential and surface geopotential instead of heights, sensible heat
+    # flux accumulated over the hour in J/m^2, downward-positive), for the
+    # device conversions (glide_source_units)
+    source_units: bool = False
 
     @property
     def lon(self) -> np.ndarray:
@@ -117,9 +122,15 @@ class SourceHour:
 
     src3: np.ndarray
     src2: np.ndarray
-    height_agl: np.ndarray
+    height_agl: np.ndarray | None
     p_level_pa: np.ndarray
     conv: np.ndarray  # [4, Y, X] cape, lcl_h, top_h, dtv
+    # "source": the reader's raw units (w channel omega in Pa/s, shf channel
+    # J/m^2 accumulated over 3600 s and downward-positive, geopotential fields
+    # in m^2/s^2 and height_agl None); "final": the units above
+    units: str = "final"
+    geopotential: np.ndarray | None = None  # [Zp, Y, X]
+    geopotential_sfc: np.ndarray | None = None  # [Y, X]
 
 
 def _smooth_pattern(rng, ny, nx, n=4, kmax=3.0):
@@ -243,7 +254,15 @@ class SyntheticMet:
         top = np.minimum(lcl + 1000.0 + 3.0 * cape, 15000.0)
         dtv = cape / 300.0
         conv = np.stack([cape, lcl, top, dtv]).astype(np.float32)
-        return SourceHour(src3=src3, src2=src2, height_agl=agl, p_level_pa=p_hpa * 100.0, conv=conv)
+        if not s.source_units:
+            return SourceHour(src3=src3, src2=src2, height_agl=agl, p_level_pa=p_hpa * 100.0, conv=conv)
+        # the same hour in the reader's raw units (inverse conversions, float64, to float32)
+        omega = -(src3[2].astype(np.float64) * GRAVITY * p / (R_DRY * src3[3].astype(np.float64)))
+        src3[2] = omega.astype(np.float32)
+        src2[3] = (-(src2[3].astype(np.float64) * 3600.0)).astype(np.float32)
+        return SourceHour(src3=src3, src2=src2, height_agl=None, p_level_pa=p_hpa * 100.0, conv=conv,
+                          units="source", geopotential=(z_msl * GRAVITY).astype(np.float32),
+                          geopotential_sfc=(terrain * GRAVITY).astype(np.float32))
 
 
 @dataclass(frozen=True)

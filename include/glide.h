@@ -419,6 +419,28 @@ int glide_resample_hour(const float* src3, int32_t c3, int32_t zp, int32_t ny, i
                         int32_t iw, float* dst3, double* p_out_hpa, void* stream);
 
 /*
+ * Per-hour unit conversions of raw source fields, in place, before
+ * glide_resample_hour (replaces the reader's host-side conversions
+ * ArcoEra5ZarrReader._compute_level_agl_m, met_reader.py:1052-1078,
+ * _convert_vertical_velocity_to_m_s, 1268-1300, and _convert_shf_to_w_per_m2,
+ * 1353-1385), with the reader's arithmetic:
+ *   geopotential [zp][ny][nx], geopotential_sfc [ny][nx] float (m^2/s^2; may be
+ *     NULL): height_agl [zp][ny][nx] = (geopotential - sfc) / g in float32, not
+ *     clamped (negative below ground);
+ *   w_ch >= 0: src3[w_ch] holds omega (Pa/s), replaced by w = -(R_d T / (g p))
+ *     omega in float64 rounded to float32, T = src3[t_ch], p = p_level_pa[k];
+ *   shf [ny][nx] float (may be NULL with GLIDE_SHF_UPWARD_W_M2): to upward-
+ *     positive W/m^2 from downward-positive W/m^2 (negated) or from J/m^2
+ *     accumulated over accumulation_s (divided, then negated), in float32.
+ */
+#define GLIDE_SHF_UPWARD_W_M2 0
+#define GLIDE_SHF_DOWNWARD_W_M2 1
+#define GLIDE_SHF_ACCUMULATED_J_M2 2
+int glide_source_units(float* src3, int32_t c3, int32_t zp, int32_t ny, int32_t nx, int32_t w_ch, int32_t t_ch,
+                       const double* p_level_pa, const float* geopotential, const float* geopotential_sfc,
+                       float* height_agl, float* shf, int32_t shf_units, double accumulation_s, void* stream);
+
+/*
  * Window build from two resampled hours (device):
  *   h3_s, h3_e: [5][nz][ny][nx] float (u, v, w, t, q) at hour start / end;
  *   h2_s, h2_e: [4][ny][nx] float (blh, sp, ustar, shf);

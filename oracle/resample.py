@@ -139,3 +139,65 @@ def resample_hour(src3, src2, height_agl_p, terrain_m, lat, lon, agl_levels, p_l
     p3 = np.broadcast_to((np.asarray(p_level_pa, np.float64) / 100.0)[:, None, None], (Zp, Y, X))
     p_agl = apply_regrid(np.ascontiguousarray(p3)[None], lower, frac, flipped)[0]
     return out3, np.asarray(src2, np.float32), p_agl.mean(axis=(1, 2))
+
+
+# ---------------------------------------------------------------------------
+# Per-hour unit conversions of raw source fields (the reader's host side)
+# ---------------------------------------------------------------------------
+
+GRAVITY = 9.80665
+R_DRY = 287.05
+
+
+def agl_from_geopotential(geopotential, geopotential_sfc):
+    """met_reader.py:1052-1078: (z - z_sfc) / g on the float32 fields (numpy
+    float32 arithmetic, the scalar g taken as float32); not clamped."""
+
+    z = np.asarray(geopotential, np.float32)
+    zs = np.asarray(geopotential_sfc, np.float32)
+    return (z - zs[None]) / np.float32(GRAVITY)
+
+
+def omega_to_w(omega, temperature, p_level_pa):
+    """met_reader.py:1268-1300: w = -(R_d T / (g p)) omega in float64, p the
+    level pressure broadcast to [Z, Y, X]; returned as float32 (the tensor the
+    reader builds)."""
+
+    om = np.asarray(omega, np.float64)
+    t = np.asarray(temperature, np.float64)
+    p3 = np.broadcast_to(np.asarray(p_level_pa, np.float64)[:, None, None], om.shape)
+    return (-(R_DRY * t / (GRAVITY * p3)) * om).astype(np.float32)
+
+
+def shf_to_upward(shf, units: str, accumulation_seconds: float = 3600.0):
+    """met_reader.py:1353-1385: de-accumulate J/m^2 over the accumulation
+    window, then flip ECMWF's downward-positive sign; float32 arithmetic.
+    ``units``: "accumulated" (J/m^2) or "instantaneous" (W/m^2)."""
+
+    a = np.asarray(shf, np.float32)
+    if units == "accumulated":
+        a = a / np.float32(accumulation_seconds)
+    elif units != "instantaneous":
+        raise ValueError(f"unknown sensible heat flux units {units!r}")
+    return -a
+
+
+def convert_source_hour(sh, w_ch: int = 2, t_ch: int = 3, shf_ch: int = 3):
+    """(src3, src2, height_agl) of a raw source hour (``sh.units == "source"``:
+    omega in the w channel, accumulated downward sensible heat flux, geopotential
+    instead of heights) in the resample's units; other hours pass through."""
+
+    if getattr(sh, "units", "final") != "source":
+        return sh.src3, sh.src2, sh.height_agl
+    src3 = np.array(sh.src3, np.float32, copy=True)
+    src2 = np.array(sh.src2, np.float32, copy=True)
+    src3[w_ch] = omega_to_w(src3[w_ch], src3[t_ch], sh.p_level_pa)
+    src2[shf_ch] = shf_to_upward(src2[shf_ch], "accumulated")
+    return src3, src2, agl_from_geopotential(sh.geopotential, sh.geopotential_sfc)
+
+
+def resample_source_hour(sh, met, levels):
+    """resample_hour of a source hour in either units (converted first)."""
+
+    src3, src2, height = convert_source_hour(sh)
+    return resample_hour(src3, src2, height, met.terrain, met.lat, met.lon, levels, sh.p_level_pa)

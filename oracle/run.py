@@ -145,8 +145,7 @@ def run(cfg, dty=np.float64, params: lpdm.HannaParams | None = None, met=None,
     def hour(h):
         if h not in hours:
             sh = met.hour(h)
-            hours[h] = ors.resample_hour(sh.src3, sh.src2, sh.height_agl, met.terrain, met.lat,
-                                         met.lon, levels, sh.p_level_pa)
+            hours[h] = ors.resample_source_hour(sh, met, levels)
         return hours[h]
 
     windows = {}
@@ -234,8 +233,7 @@ def torch_windows(met, levels=None, hours=None):
     def hour(h):
         if h not in hours:
             sh = met.hour(h)
-            hours[h] = ors.resample_hour(sh.src3, sh.src2, sh.height_agl, met.terrain, met.lat, met.lon, levels,
-                                         sh.p_level_pa)
+            hours[h] = ors.resample_source_hour(sh, met, levels)
         return hours[h]
 
     def windows(bh: int) -> dict:

@@ -46,8 +46,7 @@ def _stream():
 
 def _oracle_hour(met, h):
     sh = met.hour(h)
-    return OR.resample_hour(sh.src3, sh.src2, sh.height_agl, met.terrain, met.lat, met.lon,
-                            met.spec.agl_levels, sh.p_level_pa)
+    return OR.resample_source_hour(sh, met, met.spec.agl_levels)
 
 
 # ---------------------------------------------------------------------------
@@ -66,6 +65,47 @@ def test_resample_bit_exact(cfg0):
         diff = np.count_nonzero(got[c] != o3[c])
         assert diff == 0, f"channel {c}: {diff} values differ"
     np.testing.assert_allclose(p.cpu().numpy(), op, rtol=1e-12)
+
+
+def test_source_unit_conversions_and_resample_bit_exact():
+    """A met hour in the reader's raw units (omega, geopotential, accumulated
+    downward sensible heat flux) converted on the device (glide_source_units)
+    and resampled, against the oracle's conversions and resample: the heights,
+    every resampled channel, the surface fields and the pressures bit for bit
+    (float32 conversions correctly rounded, the omega -> w one in float64);
+    the same through the streamed-met path (MetIngest)."""
+    import dataclasses
+
+    from glide_amd.ingest import MetIngest
+    from glide_amd.met import upload_source_hour
+    from oracle import resample as ORS
+
+    base = workload_config("plain-cpu-10k-3h")["spec"]
+    met = SyntheticMet(dataclasses.replace(base, source_units=True))
+    sh = met.hour(1)
+    assert sh.units == "source"
+    src3, src2, height = ORS.convert_source_hour(sh)
+    up = upload_source_hour(sh, "cuda")
+    torch.cuda.synchronize()
+    assert np.array_equal(up["height"].cpu().numpy(), height)
+    assert np.array_equal(up["src3"].cpu().numpy(), src3)
+    assert np.array_equal(up["src2"].cpu().numpy(), src2)
+    dm = DeviceMet(met, [1, 2], device="cuda")
+    out3, out2, p = dm.resampled(1, _stream())
+    torch.cuda.synchronize()
+    o3, o2, op = _oracle_hour(met, 1)
+    got = out3.cpu().numpy()
+    for c in range(5):
+        diff = np.count_nonzero(got[c] != o3[c])
+        assert diff == 0, f"channel {c}: {diff} values differ"
+    assert np.array_equal(out2.cpu().numpy(), o2)
+    np.testing.assert_allclose(p.cpu().numpy(), op, rtol=1e-12)
+    with MetIngest(met, workers=2, lookahead=1, max_resident=3) as ing:
+        arr = ing.get(1)
+        torch.cuda.synchronize()
+        assert np.array_equal(arr["height"].cpu().numpy(), height)
+        assert np.array_equal(arr["src3"].cpu().numpy(), src3)
+        assert np.array_equal(arr["src2"].cpu().numpy(), src2)
 
 
 # ---------------------------------------------------------------------------

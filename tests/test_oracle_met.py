@@ -87,3 +87,50 @@ def test_stretched_levels_geometry():
     r = th[1:] / th[:-1]
     assert np.allclose(r, r[0], rtol=1e-9) and r[0] > 1.0
     assert np.array_equal(lev, R.stretched_agl_levels(60, 20000.0, 10.0))
+
+
+def test_source_unit_conversions_golden_values():
+    """The reader's conversions (met_reader.py:1052-1078, 1268-1300, 1353-1385)
+    on the reference's own vectors (tests/test_met_reader.py:509-617) and
+    closed forms: a wrong sign, a missing de-accumulation or a dropped g
+    fails."""
+    from oracle import resample as OR
+
+    # sensible heat flux: ECMWF downward-positive J/m^2 or W/m^2 -> upward-positive W/m^2
+    assert np.allclose(OR.shf_to_upward(np.array([[-360_000.0]]), "accumulated"), 100.0)
+    assert np.allclose(OR.shf_to_upward(np.array([[-100.0]]), "instantaneous"), 100.0)
+    assert np.allclose(OR.shf_to_upward(np.array([[43_200.0]]), "accumulated"), -12.0)
+    assert np.allclose(OR.shf_to_upward(np.array([[360_000.0]]), "accumulated"), -100.0)
+    # omega -> w: subsidence-sign omega gives upward w; w = -omega / (rho g), rho = p / (R T)
+    p = np.array([50_000.0, 85_000.0])
+    t = np.full((2, 1, 1), 250.0, np.float32)
+    w = OR.omega_to_w(np.full((2, 1, 1), -0.1, np.float32), t, p)
+    rho = p / (287.05 * 250.0)
+    np.testing.assert_allclose(w[:, 0, 0], 0.1 / (rho * 9.80665), rtol=1e-6)
+    assert np.all(w > 0) and w.dtype == np.float32
+    # geopotential -> height above ground: 9806.65 m^2/s^2 above a 1 km surface = 1 km AGL; below ground negative
+    z = np.array([[[9806.65 * 2.0]], [[9806.65 * 0.5]]], np.float32)
+    zs = np.array([[9806.65]], np.float32)
+    h = OR.agl_from_geopotential(z, zs)
+    np.testing.assert_allclose(h[:, 0, 0], [1000.0, -500.0], rtol=1e-6)
+    assert h.dtype == np.float32
+
+
+def test_synthetic_source_units_round_trip():
+    """A met generated in source units converts back to the final-unit met
+    of the same seed (to float32 rounding of the round trip)."""
+    import dataclasses
+
+    from glide_amd.synthetic import MetSpec, SyntheticMet
+    from oracle import resample as OR
+
+    spec = MetSpec(nx=16, ny=12, za=10, n_hours=2, seed=5)
+    fin = SyntheticMet(spec).hour(1)
+    raw = SyntheticMet(dataclasses.replace(spec, source_units=True)).hour(1)
+    assert raw.units == "source" and raw.height_agl is None
+    src3, src2, height = OR.convert_source_hour(raw)
+    np.testing.assert_allclose(height, fin.height_agl, rtol=0, atol=2e-3)
+    np.testing.assert_allclose(src3[2], fin.src3[2], rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(src2[3], fin.src2[3], rtol=1e-6, atol=1e-4)
+    np.testing.assert_array_equal(np.delete(src3, 2, 0), np.delete(fin.src3, 2, 0))
+    assert not np.allclose(raw.src3[2], fin.src3[2])  # the raw w channel holds omega
