@@ -109,6 +109,7 @@ def parse():
                     help="seconds of Poisson request arrivals (20 requests/s) in the serving leg (600 = the "
                          "full 10-minute stream)")
     ap.add_argument("--serve-out", default="", help="keep the serving leg's Zarr stores in this directory")
+    ap.add_argument("--serve-no-convection", action="store_true", help="serving leg without convection")
     ap.add_argument("--no-ingest", action="store_true", help="skip the streamed-met leg")
     ap.add_argument("--ingest-workers", type=int, default=8, help="host decode threads of the streamed-met leg")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU baseline and the float64 checks")
@@ -531,33 +532,38 @@ def batched_leg(args, name: str, stream, sptr: int, shared: dict, evidence: dict
 
 def serving_leg(args, plain_value: float) -> dict:
     """Continuous-batching serving: Poisson requests (20/s) on a 240x160x60
-    met ring of 240 hourly brackets in bfloat16 (a 24 h synthetic cycle
-    repeated over the 10-day span), each request a release of 1k-50k particles,
-    6-240 h back, at one of 100 sites. Requests are admitted as slots and pool
-    blocks free up and are stepped together on their own cursors; finished
-    footprints are written to one zlib Zarr store per request on host threads,
-    and latency is arrival to footprint store on disk (the stores go to a
-    temporary directory, removed afterwards, unless --serve-out names one)."""
+    met ring of 240 distinct hourly brackets in bfloat16, each request a
+    release of 1k-50k particles, 6-240 h back, at one of 100 sites, with
+    per-column convection on each request's first step of every met hour.
+    Requests are admitted as slots, particle-pool blocks and footprint-arena
+    cells are free (pool and arena grow within free HBM), and are stepped
+    together on their own cursors; finished footprints are written to one zlib
+    Zarr store per request on host threads, and latency is arrival to
+    footprint store on disk (the stores go to a temporary directory, removed
+    afterwards, unless --serve-out names one)."""
+
+    import shutil
+    import tempfile
 
     import numpy as np
     import torch
 
     from glide_amd import _lib as L
-    import shutil
-    import tempfile
-
+    from glide_amd.runner import ConvectionOptions
     from glide_amd.serving import MetRing, ResultWriter, Server, latency_percentiles
     from glide_amd.synthetic import MetSpec, SyntheticMet, serving_requests, workload_config
 
-    spec = MetSpec(nx=240, ny=160, za=60, n_hours=25, seed=20240105, lon0=2.0, lat0=72.0)
+    t0 = time.perf_counter()
+    spec = MetSpec(nx=240, ny=160, za=60, n_hours=241, seed=20240105, lon0=2.0, lat0=72.0)
     met = SyntheticMet(spec)
-    ring = MetRing(met, first_hour=0, n_hours=240, met_dtype=L.GLIDE_MET_BF16, source_hour=lambda h: h % 24)
+    ring = MetRing(met, first_hour=0, n_hours=240, met_dtype=L.GLIDE_MET_BF16, convection=not args.serve_no_convection)
+    ring_s = time.perf_counter() - t0
     base = workload_config("plain-gpu-1M-24h")
     reqs = serving_requests(spec, rate_per_s=20.0, seconds=args.serve_seconds)
     out_dir = args.serve_out or tempfile.mkdtemp(prefix="glide_serve_")
     writer = ResultWriter(out_dir, workers=4)
-    srv = Server(ring, base["grid"], base["domain"], max_slots=160, pool_particles=6 << 20, n_t_max=240,
-                 seed=17, writer=writer)
+    srv = Server(ring, base["grid"], base["domain"], max_slots=1024, n_t_max=240, seed=17, writer=writer,
+                 convection=None if args.serve_no_convection else ConvectionOptions())
     # warm-up: one request through, not timed
     warm = serving_requests(spec, rate_per_s=20.0, seconds=0.2, seed=99)[:1]
     for r in warm:
@@ -568,6 +574,8 @@ def serving_leg(args, plain_value: float) -> dict:
     lat = latency_percentiles(res)
     day = [r for r in res if 18 * 3600 <= r.req.length_s <= 30 * 3600]
     lat_day = latency_percentiles(day)
+    tenk = [r for r in day if 5_000 <= r.req.n_particles <= 20_000]
+    lat_tenk = latency_percentiles(tenk)
     value = stats.particle_steps / stats.busy_s if stats.busy_s > 0 else float("nan")
     out = {
         "workload": SERVING,
@@ -582,11 +590,18 @@ def serving_leg(args, plain_value: float) -> dict:
         "active_particle_steps": stats.particle_steps,
         "latency_s": {"p50": lat["p50"], "p99": lat["p99"]},
         "latency_18_30h_s": {"p50": lat_day["p50"], "p99": lat_day["p99"], "n": len(day)},
-        "met_storage": "bf16 ring, 240 brackets",
+        "latency_5k_20k_particles_18_30h_s": {"p50": lat_tenk["p50"], "p99": lat_tenk["p99"], "n": len(tenk)},
+        "met_storage": "bf16 ring, 240 distinct hourly brackets",
+        "ring_gb": round(ring.nbytes / 1e9, 2),
+        "ring_build_s": round(ring_s, 1),
+        "convection": "per column, each request's first step of every met hour" if srv.conv else "off",
+        "convect_launches": srv.convect_launches,
         "latency_to": "footprint Zarr store (zlib) on disk",
         "zarr_stores": len(writer.paths),
         "zarr_bytes": sum(f.stat().st_size for p in writer.paths for f in Path(p).rglob("*") if f.is_file()),
-        "slots": srv.max_slots,
+        "admission": {"max_slots": srv.max_slots, "pool_particles": srv.cap, "arena_cells": srv.arena.size,
+                      "device_gb": round(srv.device_bytes() / 1e9, 2),
+                      "rule": "slot, pool blocks and arena cells free; pool and arena grow within free HBM"},
         "vs_plain": value / plain_value if plain_value > 0 else None,
         "mean_particles": float(np.mean([r.n_particles for r in reqs])) if reqs else 0.0,
     }
@@ -596,7 +611,6 @@ def serving_leg(args, plain_value: float) -> dict:
     del srv, ring
     torch.cuda.empty_cache()
     return out
-
 
 
 def ingest_leg(args, sptr: int, plain_value: float) -> dict:

@@ -13,7 +13,7 @@ from pathlib import Path
 _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "libglide_hip.so"
 
-ABI_VERSION = 7  # GLIDE_ABI_VERSION of include/glide.h
+ABI_VERSION = 8  # GLIDE_ABI_VERSION of include/glide.h
 
 GLIDE_OK = 0
 GLIDE_ERR_INVALID = -1
@@ -36,6 +36,7 @@ F_PRESCRIBED = 0x100
 GLIDE_SLOT_FREE = 0
 GLIDE_SLOT_RUNNING = 1
 GLIDE_SLOT_FINISHED = 2
+GLIDE_SLOTS_CONVECT = 0x1
 GLIDE_ROW_WORDS = 66  # int32 words of the step cursor buffer (glide_step_table)
 GLIDE_SHF_UPWARD_W_M2, GLIDE_SHF_DOWNWARD_W_M2, GLIDE_SHF_ACCUMULATED_J_M2 = 0, 1, 2
 GLIDE_NOALLOC_MAX_PAIRS = 1 << 20  # pairs glide_footprint_accumulate_noalloc serves
@@ -84,13 +85,14 @@ class GlideStepParams(C.Structure):
         ("meander_timescale_s", C.c_double), ("seed", C.c_uint64),
         ("step_index", C.c_int32), ("max_substeps", C.c_int32),
         ("flags", C.c_int32), ("block", C.c_int32),
+        ("noise_hi", C.c_uint32), ("reserved", C.c_int32),
     ]
 
 
 class GlideSlot(C.Structure):
     _fields_ = [
         ("t", C.c_double), ("term", C.c_double), ("rel_time", C.c_double),
-        ("state", C.c_int32), ("k", C.c_int32), ("noise_hi", C.c_uint32), ("reserved", C.c_int32),
+        ("state", C.c_int32), ("k", C.c_int32), ("noise_hi", C.c_uint32), ("convect", C.c_int32),
         ("last_d", C.c_double), ("cursor_off", C.c_float), ("alpha", C.c_float), ("dt", C.c_float),
         ("noise_step", C.c_int32), ("bracket", C.c_int32), ("stepping", C.c_int32),
         ("escaped", C.c_uint64), ("active_steps", C.c_uint64),
@@ -132,7 +134,7 @@ SIGNATURES = {
                                              C.c_int32, vp, vp, vp, C.c_size_t, vp]),
     "glide_particle_means": (C.c_int, [P(GlideParticles), vp, C.c_int32, vp, vp, C.c_size_t, vp]),
     "glide_field_means": (C.c_int, [vp, C.c_int64, C.c_int32, vp, vp, C.c_size_t, vp]),
-    "glide_slots_advance": (C.c_int, [vp, C.c_int32, C.c_double, vp]),
+    "glide_slots_advance": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, vp]),
     "glide_step_slots": (C.c_int, [P(GlideStepParams), vp, vp, P(GlideParticles), P(GlideGrid), P(GlideMetRing),
                                    P(GlideFootprintGrid), P(GlideEmit), vp]),
     "glide_footprint_keys": (C.c_int, [C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp, vp, C.c_double,

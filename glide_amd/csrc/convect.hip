@@ -366,7 +366,7 @@ __device__ __forceinline__ void convect_particle(const ConvectArgs& a, const flo
         for (int j = 0; j < nz; ++j) tot += colv(j);
         tot = fmaxf(tot, 1e-12f);
         const uint32_t uid = s.uid ? s.uid[i] : static_cast<uint32_t>(i);
-        const U4 r = philox4x32_10(uid, static_cast<uint32_t>(p.step_index), kStreamConvection, 0u,
+        const U4 r = philox4x32_10(uid, static_cast<uint32_t>(p.step_index), kStreamConvection, p.noise_hi,
                                    static_cast<uint32_t>(p.seed), static_cast<uint32_t>(p.seed >> 32));
         const float u = u01(r.x);
         float cum = 0.0f;

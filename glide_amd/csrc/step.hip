@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
     fill_ladder(lev, a.levels, a.nz);
     __syncthreads();
     GlideStepParams p = a.p;
-    StepCtx x{a.wind, a.aux, a.surf, a.e.escaped, -1, 0u};
+    StepCtx x{a.wind, a.aux, a.surf, a.e.escaped, -1, a.p.noise_hi};
     unsigned long long* n_active = a.n_active;
     const int64_t i =
         static_cast<int64_t>(MODE == kModeSlots ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * blockDim.x +
@@ -386,6 +386,7 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
         if (in_table) {
             p = load_row(a.table, r);
             p.flags = a.p.flags;  // the template's, which the host checked the pointers for
+            x.noise_hi = p.noise_hi;
             x.escaped += r;
             if (n_active != nullptr) n_active += static_cast<int64_t>(r) * active_copies(a.e);
         } else {
@@ -419,6 +420,8 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
                 x.slot = slot;
                 x.noise_hi = __builtin_amdgcn_readfirstlane(sl->noise_hi);
                 n_active = &sl->active_steps;
+                // a convecting slot's emission and escape kill follow its convection
+                if (FIXED && __builtin_amdgcn_readfirstlane(sl->convect) != 0) p.flags &= ~GLIDE_F_EMIT;
             }
         }
         if (!stepping) {
@@ -640,11 +643,12 @@ extern "C" int glide_step(const GlideStepParams* p, const GlideParticles* s, con
     return launch_step(p, nullptr, 0, nullptr, s, g, w, fg, e, stream);
 }
 
-__global__ void slots_advance_kernel(GlideSlot* slots, int32_t n, double dt) {
+__global__ void slots_advance_kernel(GlideSlot* slots, int32_t n, double dt, int32_t flags) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     GlideSlot sl = slots[j];
     sl.stepping = 0;
+    sl.convect = 0;
     if (sl.state == GLIDE_SLOT_RUNNING) {
         if (sl.k > 0) sl.t -= sl.last_d;
         if (sl.t > sl.term) {
@@ -658,6 +662,8 @@ __global__ void slots_advance_kernel(GlideSlot* slots, int32_t n, double dt) {
             sl.alpha = static_cast<float>(al);
             sl.cursor_off = static_cast<float>(sl.t - sl.rel_time);
             sl.noise_step = static_cast<int32_t>(floor(sl.t / dt));
+            // the reference convects on a new bracket (main.py:1446-1458); a slot's first step is one
+            if ((flags & GLIDE_SLOTS_CONVECT) && (sl.k == 0 || sl.bracket != static_cast<int32_t>(bh))) sl.convect = 1;
             sl.bracket = static_cast<int32_t>(bh);
             sl.stepping = 1;
             sl.k += 1;
@@ -668,12 +674,13 @@ __global__ void slots_advance_kernel(GlideSlot* slots, int32_t n, double dt) {
     slots[j] = sl;
 }
 
-extern "C" int glide_slots_advance(GlideSlot* slots, int32_t n_slots, double dt, void* stream) {
+extern "C" int glide_slots_advance(GlideSlot* slots, int32_t n_slots, double dt, int32_t flags, void* stream) {
     GLIDE_REQUIRE(slots != nullptr && n_slots >= 0, "glide_slots_advance: bad slots");
     GLIDE_REQUIRE(dt > 0, "dt must be > 0");
+    GLIDE_REQUIRE((flags & ~GLIDE_SLOTS_CONVECT) == 0, "unknown slots_advance flags 0x%x", flags);
     if (n_slots == 0) return GLIDE_OK;
     hipLaunchKernelGGL(slots_advance_kernel, dim3((n_slots + 255) / 256), dim3(256), 0,
-                       static_cast<hipStream_t>(stream), slots, n_slots, dt);
+                       static_cast<hipStream_t>(stream), slots, n_slots, dt, flags);
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
 }

@@ -95,7 +95,8 @@ class DeviceMet:
     window, for a :class:`SyntheticMet` grid."""
 
     def __init__(self, met: SyntheticMet, hours, *, device="cuda", met_dtype: int = L.GLIDE_MET_F32,
-                 meander_coeff: float = 0.16, meander_radius: int = 1, ingest=None, sources: dict | None = None):
+                 meander_coeff: float = 0.16, meander_radius: int = 1, ingest=None, sources: dict | None = None,
+                 defer: bool = False):
         self.met = met
         # ingest (glide_amd.ingest.MetIngest): source hours stream in ahead of
         # the cursor instead of being made resident up front
@@ -118,12 +119,10 @@ class DeviceMet:
         # sources: another DeviceMet's resident source hours (the same met in
         # another window precision), shared instead of uploaded again
         self.src = sources if sources is not None else {}
+        self.zp = None
         if ingest is None:
-            for h in hours:
-                if h in self.src:
-                    continue
-                self.src[h] = upload_source_hour(met.hour(h), device)
-            self.zp = next(iter(self.src.values()))["src3"].shape[1]
+            if not defer:  # defer: the caller uploads hours as it needs them (upload)
+                self.upload(hours)
         else:
             first = max(hours) if ingest.direction < 0 else min(hours)
             self.zp = ingest.get(first)["src3"].shape[1]
@@ -139,6 +138,15 @@ class DeviceMet:
                                 levels=self.levels_f.data_ptr())
         self.window = L.GlideWindow(wind=self.wind.data_ptr(), aux=self.aux.data_ptr(),
                                     surf=self.surf.data_ptr(), dtype=met_dtype, reserved=0)
+
+    def upload(self, hours) -> None:
+        """Make source hours resident (those not already)."""
+
+        for h in hours:
+            if h not in self.src:
+                self.src[h] = upload_source_hour(self.met.hour(h), self.device)
+        if self.zp is None and self.src:
+            self.zp = next(iter(self.src.values()))["src3"].shape[1]
 
     def resampled(self, h: int, stream: int):
         """Resampled hour ``h`` (out3 [5, Za, Y, X], surface [4, Y, X], p_hpa [Za])."""

@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define GLIDE_ABI_VERSION 7
+#define GLIDE_ABI_VERSION 8
 
 #define GLIDE_OK 0
 #define GLIDE_ERR_INVALID (-1)
@@ -191,6 +191,8 @@ typedef struct GlideFootprintGrid {
  *   window length; seed/step_index: Philox key and counter word 1.
  *   block: threads per workgroup of the step kernel launch, a multiple of 64
  *   in [64, 512]; 0 = 256 (a tuning parameter: results do not depend on it).
+ *   noise_hi: Philox counter word 3 of every draw (0 for a batch; a served
+ *   request's word when its particles are convected on their own).
  */
 typedef struct GlideStepParams {
     double dt, alpha, cursor_off, length_s;
@@ -198,6 +200,8 @@ typedef struct GlideStepParams {
     uint64_t seed;
     int32_t step_index, max_substeps;
     int32_t flags, block;
+    uint32_t noise_hi;
+    int32_t reserved;
 } GlideStepParams;
 
 /* Step outputs (device): per-particle footprint key (n_cells = dropped) and
@@ -537,7 +541,8 @@ typedef struct GlideSlot {
     int32_t k;           /* steps taken */
     uint32_t noise_hi;   /* Philox counter word 3 of the slot's particles (0 elsewhere): with
                             the particle ids it keeps concurrent requests' noise apart */
-    int32_t reserved;
+    int32_t convect;     /* per step: 1 when this is the first step of a met hour and
+                            convection is on (its emission is left to glide_convect) */
     /* per step (glide_slots_advance) */
     double last_d;       /* length of the step taken last */
     float cursor_off, alpha, dt;
@@ -560,8 +565,14 @@ typedef struct GlideMetRing {
     int32_t dtype, reserved;
 } GlideMetRing;
 
-/* Advance every running slot one step of its schedule (dt = step length). */
-int glide_slots_advance(GlideSlot* slots, int32_t n_slots, double dt, void* stream);
+/* Advance every running slot one step of its schedule (dt = step length).
+ * With GLIDE_SLOTS_CONVECT in flags a slot's step that is the first of a met
+ * hour for it (its first step, or its bracket changed) is marked convect = 1:
+ * glide_step_slots then steps its particles without emitting, and the caller
+ * convects them and emits (glide_convect with the slot's scalars), the
+ * reference's order (main.py:1446-1477). */
+#define GLIDE_SLOTS_CONVECT 0x1
+int glide_slots_advance(GlideSlot* slots, int32_t n_slots, double dt, int32_t flags, void* stream);
 
 /* glide_step over a particle pool shared by the slots: wave_slot[j] is the slot
  * of particles 64j .. 64j+63 (-1: unused), each slot's particles lie in whole
@@ -572,7 +583,9 @@ int glide_slots_advance(GlideSlot* slots, int32_t n_slots, double dt, void* stre
  * [0, fg->n_r) marks an unused block like -1; escape and active counts go to
  * the slot. Waves of non-stepping slots only write dropped keys. With
  * GLIDE_F_FIXED the residence goes to e->fixed as in glide_step, with
- * e->fixed_exp indexed by slot. */
+ * e->fixed_exp indexed by slot (and fg->slab_off, when given, placing each
+ * slot's slab). Waves of a slot marked convect step without emitting or the
+ * escape kill (GLIDE_F_FIXED only). */
 int glide_step_slots(const GlideStepParams* tmpl, GlideSlot* slots, const int32_t* wave_slot,
                      const GlideParticles* s, const GlideGrid* g, const GlideMetRing* ring,
                      const GlideFootprintGrid* fg, const GlideEmit* e, void* stream);

@@ -45,7 +45,7 @@ def test_library_exports_every_declared_symbol():
     lib = L.lib()
     for name in declared_functions():
         assert hasattr(lib, name), name
-    assert lib.glide_abi_version() == L.ABI_VERSION == 7
+    assert lib.glide_abi_version() == L.ABI_VERSION == 8
 
 
 def test_library_missing_fails_loudly(monkeypatch, tmp_path):
