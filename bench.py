@@ -325,21 +325,34 @@ def headline_leg(args, stream, sptr: int) -> tuple[dict, dict]:
                                                  else "exact int64 fixed-point sums inside the step kernel")}
         oe.close()
         del oe
+    # the same K steps with positions held as a grid point plus float32 offsets
+    ce = BatchEngine(cfg, device="cuda", met=met, device_met=eng.met, met_dtype=met_dtype, cell_positions=True,
+                     accumulate=args.accumulate, **tuned["best"])
+    ce.run(W)
+    torch.cuda.synchronize()
+    ce.reset()
+    cel = _timed_steps(ce, K, sptr)
+    out["positions_cell"] = {"ms_per_step": cel * 1e3 / K, "value": psteps / cel,
+                             "layout": "int32 grid point + float32 lon/lat offsets (16 B) instead of float32 "
+                                       "lon/lat/z (12 B)"}
+    ce.close()
+    del ce
     evidence = {"positions": {}}
     if not args.no_fp64_check and not args.no_cpu_baseline:
-        for name, mdt in (("f32", L.GLIDE_MET_F32), ("bf16", L.GLIDE_MET_BF16)):
-            evidence["positions"][name] = position_snapshots(cfg, met, mdt, args.check_particles)
+        for name, mdt, cell in (("f32", L.GLIDE_MET_F32, False), ("bf16", L.GLIDE_MET_BF16, False),
+                                ("f32_cell", L.GLIDE_MET_F32, True), ("bf16_cell", L.GLIDE_MET_BF16, True)):
+            evidence["positions"][name] = position_snapshots(cfg, met, mdt, args.check_particles, cell=cell)
     eng.close()
     del eng
     torch.cuda.empty_cache()
     return out, evidence
 
 
-def position_snapshots(cfg, met, met_dtype: int, n_particles: int, n_steps: int = 60) -> dict:
+def position_snapshots(cfg, met, met_dtype: int, n_particles: int, n_steps: int = 60, cell: bool = False) -> dict:
     """The headline release with ``n_particles`` particles stepped by the
-    engine over the first met hour; the particle state (in particle-id order)
-    before the first step and after each step, for the per-step check against
-    the float64 oracle."""
+    engine over the first met hour; the particle state (in particle-id order,
+    positions exactly as the layout holds them) before the first step and
+    after each step, for the per-step check against the float64 oracle."""
 
     import dataclasses
 
@@ -349,7 +362,7 @@ def position_snapshots(cfg, met, met_dtype: int, n_particles: int, n_steps: int 
 
     rel = dataclasses.replace(cfg["releases"][0], n_particles=int(n_particles))
     c = dict(cfg, releases=[rel])
-    eng = BatchEngine(c, device="cuda", met=met, met_dtype=met_dtype)
+    eng = BatchEngine(c, device="cuda", met=met, met_dtype=met_dtype, cell_positions=cell)
     snaps = [eng.state_numpy()]
     steps = eng.sched[:n_steps]
     for _ in steps:
@@ -475,7 +488,8 @@ def batched_leg(args, name: str, stream, sptr: int, shared: dict, evidence: dict
     else:
         tuned = tune(cfg, met, candidates=BATCH_CANDIDATES, use_cache=False, stream=stream, device_met=dm, **kw)
         shared["tuned"] = tuned["best"]
-    eng = BatchEngine(cfg, device="cuda", met=met, device_met=dm, **kw, **tuned["best"])
+    # the precision variant holds positions as a grid point plus float32 offsets
+    eng = BatchEngine(cfg, device="cuda", met=met, device_met=dm, cell_positions=bf16, **kw, **tuned["best"])
     n_sched = len(eng.sched)
     K = n_sched if args.batched_steps <= 0 else min(args.batched_steps, n_sched)
     eng.run(min(20, n_sched))  # warm-up, then back to the release state
@@ -522,10 +536,29 @@ def batched_leg(args, name: str, stream, sptr: int, shared: dict, evidence: dict
         "device_gb_allocated": round(torch.cuda.max_memory_allocated() / 1e9, 1),
         "autotune": {"chosen": tuned["best"], "trials": tuned["trials"], **({"from": tuned["from"]}
                                                                             if "from" in tuned else {})},
+        "positions": "int32 grid point + float32 offsets" if bf16 else "float32 lon/lat/z",
         "roofline": roofline,
     }
     eng.close()
-    del eng, dm
+    del eng
+    torch.cuda.empty_cache()
+    if K == n_sched:
+        # the sampled releases alone in the other position layout (a release alone equals it in the batch)
+        import dataclasses
+
+        for r in sampled_releases(cfg, args.check_releases):
+            rel = cfg["releases"][r]
+            one = dataclasses.replace(rel, release_idx=0, noise_id=rel.release_idx)
+            c1 = dict(cfg, releases=[one], grid=dict(cfg["grid"], n_r=1))
+            e1 = BatchEngine(c1, device="cuda", met=met, device_met=dm, release_terminus=True, met_dtype=met_dtype,
+                             cell_positions=not bf16)
+            e1.run()
+            evidence[r][f"{name} alone, {'float32 positions' if bf16 else 'cell positions'}"] = {
+                "footprint": e1.release_footprint(0).cpu().numpy().astype(np.float64),
+                "total": float(e1.release_total.cpu().numpy()[0])}
+            e1.close()
+            del e1
+    del dm
     torch.cuda.empty_cache()
     return out
 
@@ -569,7 +602,7 @@ def serving_leg(args, plain_value: float) -> dict:
     for r in warm:
         srv.admit(r)
     srv.drain()
-    stats = srv.serve(reqs)
+    stats = srv.serve(reqs, progress_every=500)
     res = stats.results
     lat = latency_percentiles(res)
     day = [r for r in res if 18 * 3600 <= r.req.length_s <= 30 * 3600]
@@ -872,11 +905,15 @@ def release_errors(r: int, results: dict) -> dict:
     return out
 
 
-def fp64_checks(evidence: dict) -> dict:
+def fp64_checks(evidence: dict, note=lambda msg: None) -> dict:
     out = {}
     for name, ev in evidence.get("positions", {}).items():
         out[f"positions_first_met_hour_{name}_met"] = position_errors(ev)
-    rel = [release_errors(r, res) for r, res in sorted(evidence.get("releases", {}).items())]
+        note(f"positions {name}: checked")
+    rel = []
+    for r, res in sorted(evidence.get("releases", {}).items()):
+        rel.append(release_errors(r, res))
+        note(f"release {r}: checked")
     if rel:
         out["batched_releases"] = rel
     out["method"] = ("float64 oracle (oracle/) on the same Philox noise; positions: one oracle step from each "
@@ -935,7 +972,7 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline_leg(args)
         note(f"cpu baseline {out['cpu_baseline']['value']:.4g}")
         if not args.no_fp64_check:
-            out["error_vs_fp64"] = fp64_checks(evidence)
+            out["error_vs_fp64"] = fp64_checks(evidence, note)
             note("fp64 checks done")
     print(json.dumps(out), flush=True)
     return 0

@@ -62,7 +62,7 @@ class GlideParticles(C.Structure):
         ("lon", vp), ("lat", vp), ("z", vp),
         ("up", vp), ("vp", vp), ("wp", vp), ("um", vp), ("vm", vp),
         ("weight", vp), ("r_off", vp), ("we_off", vp), ("rel_idx", vp), ("uid", vp),
-        ("alive", vp), ("active", vp), ("lo_off", vp),
+        ("alive", vp), ("active", vp), ("lo_off", vp), ("cell", vp),
     ]
 
 

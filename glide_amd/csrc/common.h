@@ -243,6 +243,12 @@ __device__ __forceinline__ float frac_index(float c, float c0, float scale, int 
     return fminf(fmaxf(f, 0.0f), static_cast<float>(n - 1));
 }
 
+// The same for a position held as grid point c plus an offset (GlideParticles.cell).
+__device__ __forceinline__ float cell_index(int c, float off, float scale, int n) {
+    const float f = static_cast<float>(c) + off * scale;
+    return fminf(fmaxf(f, 0.0f), static_cast<float>(n - 1));
+}
+
 // Level ladders in LDS hold kLadder entries: levels[0..nz), then +inf, so a
 // branchless binary search may read past nz (a padded entry is never < z).
 constexpr int kLadder = 128;

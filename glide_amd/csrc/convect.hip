@@ -261,6 +261,7 @@ struct ConvectArgs {
     GlideStepParams p;
     GlideParticles s;
     float lon0, lat0, sx, sy;
+    float cdlon, cdlat;  // grid spacing (cell positions: the cell's point is lon0 + cx * cdlon)
     int nx, ny, nz;
     const float* levels;
     const int32_t* i_lcl;
@@ -314,13 +315,26 @@ __device__ __forceinline__ void convect_particle(const ConvectArgs& a, const flo
         return;
     }
     const int64_t ps = s.pos_stride;
-    const float lon = s.lon[i * ps];
-    const float lat = s.lat[i * ps];
+    float lon = s.lon[i * ps];
+    float lat = s.lat[i * ps];
     float z = s.z[i * ps];
     const int nz = a.nz;
-    // Nearest column.
-    const int cx = static_cast<int>(floorf(frac_index(lon, a.lon0, a.sx, a.nx) + 0.5f));
-    const int cy = static_cast<int>(floorf(frac_index(lat, a.lat0, a.sy, a.ny) + 0.5f));
+    // Nearest column (cell positions: from the cell's point and the offsets,
+    // then the absolute coordinates for the footprint and domain formulas).
+    float gx, gy;
+    if (s.cell != nullptr) {
+        const int32_t c = s.cell[i];
+        const int pcy = c / a.nx, pcx = c - pcy * a.nx;
+        gx = cell_index(pcx, lon, a.sx, a.nx);
+        gy = cell_index(pcy, lat, a.sy, a.ny);
+        lon = __builtin_fmaf(static_cast<float>(pcx), a.cdlon, a.lon0) + lon;
+        lat = __builtin_fmaf(static_cast<float>(pcy), a.cdlat, a.lat0) + lat;
+    } else {
+        gx = frac_index(lon, a.lon0, a.sx, a.nx);
+        gy = frac_index(lat, a.lat0, a.sy, a.ny);
+    }
+    const int cx = static_cast<int>(floorf(gx + 0.5f));
+    const int cy = static_cast<int>(floorf(gy + 0.5f));
     const int64_t col = static_cast<int64_t>(min(cy, a.ny - 1)) * a.nx + min(cx, a.nx - 1);
     const int lcl = a.i_lcl[col];
     const int lnb = a.i_lnb[col];
@@ -588,6 +602,9 @@ extern "C" int glide_convect(const GlideStepParams* p, const GlideParticles* s, 
     a.lat0 = static_cast<float>(g->lat0);
     a.sx = static_cast<float>((g->nx - 1) / (g->lon1 - g->lon0));
     a.sy = static_cast<float>((g->ny - 1) / (g->lat1 - g->lat0));
+    a.cdlon = static_cast<float>((g->lon1 - g->lon0) / (g->nx - 1));
+    a.cdlat = static_cast<float>((g->lat1 - g->lat0) / (g->ny - 1));
+    GLIDE_REQUIRE(s->cell == nullptr || s->pos_stride == 1, "cell positions need pos_stride 1");
     a.levels = g->levels;
     a.i_lcl = i_lcl;
     a.i_lnb = i_lnb;

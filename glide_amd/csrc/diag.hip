@@ -107,6 +107,7 @@ extern "C" int glide_particle_means(const GlideParticles* s, const int32_t* row,
                                     void* workspace, size_t workspace_bytes, void* stream) {
     GLIDE_REQUIRE(s != nullptr, "NULL particles");
     GLIDE_REQUIRE(s->pos_stride == 1, "particle means need pos_stride 1");
+    GLIDE_REQUIRE(s->cell == nullptr, "particle means take absolute positions (cell is not NULL)");
     MeanArgs a{};
     a.x[0] = s->lon;
     a.x[1] = s->lat;

@@ -57,8 +57,16 @@ __device__ __forceinline__ bool key_one(const ReorderKeyArgs& a, const float* le
     uint32_t key = a.sentinel;
     if (!done) {
         const int64_t ps = s.pos_stride;
-        const int ix = static_cast<int>(frac_index(s.lon[i * ps], a.lon0, a.sx, a.nx));
-        const int iy = static_cast<int>(frac_index(s.lat[i * ps], a.lat0, a.sy, a.ny));
+        int ix, iy;
+        if (s.cell != nullptr) {  // a grid point plus offsets
+            const int32_t c = s.cell[i];
+            const int pcy = c / a.nx, pcx = c - pcy * a.nx;
+            ix = static_cast<int>(cell_index(pcx, s.lon[i], a.sx, a.nx));
+            iy = static_cast<int>(cell_index(pcy, s.lat[i], a.sy, a.ny));
+        } else {
+            ix = static_cast<int>(frac_index(s.lon[i * ps], a.lon0, a.sx, a.nx));
+            iy = static_cast<int>(frac_index(s.lat[i * ps], a.lat0, a.sy, a.ny));
+        }
         const int iz = a.by_level ? static_cast<int>(level_index(lev, a.nz, ladder_top(a.nz), s.z[i * ps])) : 0;
         uint32_t grp = 0;
         if (a.groups > 1) {
@@ -128,6 +136,7 @@ __global__ __launch_bounds__(256) void permute_kernel(const PermuteArgs a) {
     mv(s.uid, const_cast<uint32_t*>(d.uid), j, i);
     mv(s.alive, d.alive, j, i);
     mv(s.lo_off, const_cast<float*>(d.lo_off), j, i);
+    mv(s.cell, d.cell, j, i);
 }
 
 }  // namespace glide
@@ -142,6 +151,7 @@ extern "C" int glide_reorder_keys(const GlideStepParams* p, const GlideParticles
     GLIDE_REQUIRE(s->pos_stride >= 1 && s->n >= 0, "bad particles");
     GLIDE_REQUIRE(g->nz >= 2 && g->nz <= kMaxLev && g->nx >= 2 && g->ny >= 2, "bad grid");
     GLIDE_REQUIRE(g->lon1 != g->lon0 && g->lat1 != g->lat0, "Grid dimension has 0 span (first == last)");
+    GLIDE_REQUIRE(s->cell == nullptr || s->pos_stride == 1, "cell positions need pos_stride 1");
     if (s->n == 0) return GLIDE_OK;
     ReorderKeyArgs a{};
     a.s = *s;

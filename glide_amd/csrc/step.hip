@@ -23,6 +23,7 @@ struct StepArgs {
     GlideStepParams p;
     GlideParticles s;
     float lon0, lat0, sx, sy;  // met grid: ix = (lon - lon0) * sx
+    float cdlon, cdlat;  // grid spacing (GlideParticles.cell: the cell's point is lon0 + cx * cdlon)
     float blon0, blat0, bsx, bsy;  // surface bilinear mapping (same grid)
     int nx, ny, nz;
     int ltop;  // first step of the level search (ladder_top(nz))
@@ -78,7 +79,11 @@ struct StepCtx {
 // Advances particle i; returns whether it was active. With GLIDE_F_FIXED the
 // footprint key and fixed-point value go to fkey/fval (left as the caller set
 // them, the dropped key and 0, for an inactive particle) instead of memory.
-template <bool BF16, bool FIXED>
+// CELL: the precision layout (GlideParticles.cell): lon and lat below are the
+// offsets from the particle's cell point (rlon, rlat); every displacement
+// moves them, and the absolute coordinate rlon + lon is formed only where a
+// formula needs it (map factors, Coriolis, footprint cell, domain test).
+template <bool BF16, bool FIXED, bool CELL>
 __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* lev, int64_t i,
                                               const GlideStepParams& p, const StepCtx& x, uint64_t& fkey,
                                               long long& fval) {
@@ -112,11 +117,24 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     const float dt = static_cast<float>(p.dt);
     const float al = static_cast<float>(p.alpha);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
+    int cx = 0, cy = 0;
+    float rlon = 0.0f, rlat = 0.0f;
+    if constexpr (CELL) {
+        const int32_t c = ldo(s.cell, i4);
+        cy = c / nx;
+        cx = c - cy * nx;
+        rlon = __builtin_fmaf(static_cast<float>(cx), a.cdlon, a.lon0);
+        rlat = __builtin_fmaf(static_cast<float>(cy), a.cdlat, a.lat0);
+    }
+    // grid index and absolute coordinate of a position (an offset with CELL)
+    auto gix = [&](float lo) { return CELL ? cell_index(cx, lo, a.sx, nx) : frac_index(lo, a.lon0, a.sx, nx); };
+    auto giy = [&](float la) { return CELL ? cell_index(cy, la, a.sy, ny) : frac_index(la, a.lat0, a.sy, ny); };
+    auto alat = [&](float la) { return CELL ? rlat + la : la; };
 
     // ---- RK2 advection (backward) ----
     auto wind_at = [&](float lo, float la, float zz, float& dlo, float& dla, float& dzz) {
-        const float ix = frac_index(lo, a.lon0, a.sx, nx);
-        const float iy = frac_index(la, a.lat0, a.sy, ny);
+        const float ix = gix(lo);
+        const float iy = giy(la);
         const float iz = level_index(lev, nz, a.ltop, zz);
         const Corner3 c = corners3(ix, iy, iz, nx, ny, nz, WinRec<BF16>::kCellUnit);
         float v[6];
@@ -124,7 +142,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         const float u = v[0] * (1.0f - al) + v[3] * al;
         const float vv = v[1] * (1.0f - al) + v[4] * al;
         const float w = v[2] * (1.0f - al) + v[5] * al;
-        dlo = fdiv(u, m_per_deg_lon(la));
+        dlo = fdiv(u, m_per_deg_lon(alat(la)));
         dla = vv * (1.0f / kMPerDegLat);
         dzz = w;
     };
@@ -148,8 +166,8 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         Surface sf;
         float sp, shf, tsf;
         {
-            const float ix = frac_index(lon, a.blon0, a.bsx, nx);
-            const float iy = frac_index(lat, a.blat0, a.bsy, ny);
+            const float ix = CELL ? cell_index(cx, lon, a.bsx, nx) : frac_index(lon, a.blon0, a.bsx, nx);
+            const float iy = CELL ? cell_index(cy, lat, a.bsy, ny) : frac_index(lat, a.blat0, a.bsy, ny);
             const float fx0 = floorf(ix), fy0 = floorf(iy);
             const int x0 = static_cast<int>(fx0), y0 = static_cast<int>(fy0);
             const int x1 = min(x0 + 1, nx - 1), y1 = min(y0 + 1, ny - 1);
@@ -179,15 +197,15 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         sf.L = L;
         sf.wstar = wstar;
         sf.hol = fdiv(sf.blh, L);
-        sf.fcor = 2.0f * kOmega * fsin_deg(lat);
+        sf.fcor = 2.0f * kOmega * fsin_deg(alat(lat));
 
         const bool tl_floors = (p.flags & GLIDE_F_TL_FLOORS) != 0;
         const bool prescribed = (p.flags & GLIDE_F_PRESCRIBED) != 0;  // the support record at every height
         const bool sl_over = (p.flags & GLIDE_F_SL_OVERRIDE) != 0;
         const float zu = static_cast<float>(p.z_ubl_m);
         const float z_eval = fmaxf(z, zu);
-        const float ix = frac_index(lon, a.lon0, a.sx, nx);
-        const float iy = frac_index(lat, a.lat0, a.sy, ny);
+        const float ix = gix(lon);
+        const float iy = giy(lat);
 
         // support record at z_eval: FT pair, density pair (and, bfloat16, the
         // meander pair; float32 samples its plane after the substeps, when
@@ -281,7 +299,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
             un = fminf(fmaxf(un, -umax), umax);
             vn = fminf(fmaxf(vn, -vmax), vmax);
             z = z + (-wn) * sdt;
-            const float ml = m_per_deg_lon(lat);
+            const float ml = m_per_deg_lon(alat(lat));
             lon = lon + fdiv((-un) * sdt, ml);
             lat = lat + (-vn) * sdt * (1.0f / kMPerDegLat);
             if (z < 0.0f) {
@@ -306,12 +324,24 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
             const OuCoef mv{mu.a, fsqrt(fmaxf(ax[5] * ax[5] * (1.0f - mu.a * mu.a), 0.0f))};  // same decay
             um = ou_apply(um, mu, n0, 0.0f);
             vm = ou_apply(vm, mv, n1, 0.0f);
-            const float ml = m_per_deg_lon(lat);
+            const float ml = m_per_deg_lon(alat(lat));
             lon = lon + fdiv((-um) * dt, ml);
             lat = lat + (-vm) * dt * (1.0f / kMPerDegLat);
         }
     }
 
+    if constexpr (CELL) {
+        // to the nearest grid point: the offsets stay within about half a cell
+        const int ncx = min(max(static_cast<int>(floorf(static_cast<float>(cx) + lon * a.sx + 0.5f)), 0), nx - 1);
+        const int ncy = min(max(static_cast<int>(floorf(static_cast<float>(cy) + lat * a.sy + 0.5f)), 0), ny - 1);
+        lon = lon - static_cast<float>(ncx - cx) * a.cdlon;
+        lat = lat - static_cast<float>(ncy - cy) * a.cdlat;
+        cx = ncx;
+        cy = ncy;
+        rlon = __builtin_fmaf(static_cast<float>(cx), a.cdlon, a.lon0);
+        rlat = __builtin_fmaf(static_cast<float>(cy), a.cdlat, a.lat0);
+        sto(s.cell, i4, cy * nx + cx);
+    }
     sto(s.lon, ip, lon);
     sto(s.lat, ip, lat);
     sto(s.z, ip, z);
@@ -324,6 +354,10 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     }
 
     if (emit) {
+        if constexpr (CELL) {  // the absolute coordinates the footprint and domain formulas take
+            lon = rlon + lon;
+            lat = rlat + lat;
+        }
         const int64_t tb = time_bin(ldo(s.we_off, i4), co, a.fp.n_t);
         const int64_t rb = x.slot >= 0 ? x.slot : (s.rel_idx ? ldo(s.rel_idx, i4) : 0);
         const float val = ldo(s.weight, i4) * dt;
@@ -366,7 +400,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // FIXED: the fixed-point footprint (GLIDE_F_FIXED) as its own instantiation,
 // so neither variant keeps the other's pointers live (the kernel is short of
 // scalar registers; the spare ones spill to vector lanes).
-template <bool BF16, int MODE, bool FIXED>
+template <bool BF16, int MODE, bool FIXED, bool CELL = false>
 __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
     __shared__ float lev[kLadder];
     fill_ladder(lev, a.levels, a.nz);
@@ -437,7 +471,7 @@ __global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
     bool act = false;
     uint64_t fkey = kNoCell;
     long long fval = 0;
-    if (run) act = step_particle<BF16, FIXED>(a, lev, i, p, x, fkey, fval);
+    if (run) act = step_particle<BF16, FIXED, CELL>(a, lev, i, p, x, fkey, fval);
     if (FIXED && (p.flags & GLIDE_F_EMIT))
         wave_fixed_add(fkey, fval, fixed_copy(a.e, a.fp.acc_cells));  // every lane is here
     if constexpr (MODE == kModeSlots) {
@@ -514,6 +548,8 @@ static int fill_grid(StepArgs& a, const GlideGrid& g, const GlideWindow& w) {
     a.lat0 = static_cast<float>(g.lat0);
     a.sx = static_cast<float>((g.nx - 1) / (g.lon1 - g.lon0));
     a.sy = static_cast<float>((g.ny - 1) / (g.lat1 - g.lat0));
+    a.cdlon = static_cast<float>((g.lon1 - g.lon0) / (g.nx - 1));
+    a.cdlat = static_cast<float>((g.lat1 - g.lat0) / (g.ny - 1));
     a.blon0 = a.lon0;
     a.blat0 = a.lat0;
     a.bsx = a.sx;
@@ -528,6 +564,18 @@ static int fill_grid(StepArgs& a, const GlideGrid& g, const GlideWindow& w) {
 }  // namespace glide
 
 using namespace glide;
+
+// One launch of the step kernel variant for the met precision and the footprint mode.
+template <int MODE, bool CELL>
+static void launch_variants(bool bf16, bool fixed, dim3 grid, dim3 block, hipStream_t st, const StepArgs& a) {
+    if (fixed) {
+        if (bf16) hipLaunchKernelGGL((step_kernel<true, MODE, true, CELL>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((step_kernel<false, MODE, true, CELL>), grid, block, 0, st, a);
+    } else {
+        if (bf16) hipLaunchKernelGGL((step_kernel<true, MODE, false, CELL>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((step_kernel<false, MODE, false, CELL>), grid, block, 0, st, a);
+    }
+}
 
 static int launch_step(const GlideStepParams* p, const GlideStepParams* table, int32_t n_rows, int32_t* row,
                        const GlideParticles* s, const GlideGrid* g, const GlideWindow* w,
@@ -553,6 +601,11 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
     } else {
         GLIDE_REQUIRE(s->r_off && s->alive, "r_off/alive are NULL (no explicit mask)");
         GLIDE_REQUIRE(!(p->flags & GLIDE_F_RELEASE_TERMINUS) || s->lo_off, "GLIDE_F_RELEASE_TERMINUS needs lo_off");
+    }
+    const bool cell = s->cell != nullptr;
+    if (cell) {
+        GLIDE_REQUIRE(slots == nullptr, "cell positions are not taken by glide_step_slots");
+        GLIDE_REQUIRE(s->pos_stride == 1, "cell positions need pos_stride 1");
     }
     StepArgs a{};
     a.p = *p;
@@ -609,13 +662,8 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
     const bool bf16 = w->dtype == GLIDE_MET_BF16;
     const bool fixed = (p->flags & (GLIDE_F_EMIT | GLIDE_F_FIXED)) == (GLIDE_F_EMIT | GLIDE_F_FIXED);
     if (table != nullptr) {
-        if (fixed) {
-            if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeTable, true>), grid, block, 0, st, a);
-            else hipLaunchKernelGGL((step_kernel<false, kModeTable, true>), grid, block, 0, st, a);
-        } else {
-            if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeTable, false>), grid, block, 0, st, a);
-            else hipLaunchKernelGGL((step_kernel<false, kModeTable, false>), grid, block, 0, st, a);
-        }
+        if (cell) launch_variants<kModeTable, true>(bf16, fixed, grid, block, st, a);
+        else launch_variants<kModeTable, false>(bf16, fixed, grid, block, st, a);
     } else if (slots != nullptr) {
         if (fixed) {
             if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeSlots, true>), grid, block, 0, st, a);
@@ -625,13 +673,8 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
             else hipLaunchKernelGGL((step_kernel<false, kModeSlots, false>), grid, block, 0, st, a);
         }
     } else {
-        if (fixed) {
-            if (bf16) hipLaunchKernelGGL((step_kernel<true, kModePlain, true>), grid, block, 0, st, a);
-            else hipLaunchKernelGGL((step_kernel<false, kModePlain, true>), grid, block, 0, st, a);
-        } else {
-            if (bf16) hipLaunchKernelGGL((step_kernel<true, kModePlain, false>), grid, block, 0, st, a);
-            else hipLaunchKernelGGL((step_kernel<false, kModePlain, false>), grid, block, 0, st, a);
-        }
+        if (cell) launch_variants<kModePlain, true>(bf16, fixed, grid, block, st, a);
+        else launch_variants<kModePlain, false>(bf16, fixed, grid, block, st, a);
     }
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;

@@ -200,7 +200,7 @@ class BatchEngine:
                  compact: bool = False, release_terminus: bool = False, diagnostics: bool = False,
                  ingest=None, step_block: int = 0, overlap: bool = False, binned_keys: bool = True,
                  accumulate: str = "fixed", fixed_copies: int = 0, bin_by_release: bool | None = None,
-                 slabs: bool | None = None):
+                 slabs: bool | None = None, cell_positions: bool = False):
         self.cfg = cfg
         # accumulate: how each step's residence reaches the footprint.
         #   "fixed": inside the step kernel, as exact 64-bit fixed-point integer
@@ -237,6 +237,12 @@ class BatchEngine:
         # diagnostics: per-step mean particle position (inside the step body)
         # and hourly grid-mean winds, for the trajectory table
         self.diagnostics = bool(diagnostics)
+        # cell_positions: the precision layout, each particle's horizontal
+        # position as its nearest met grid point (GlideParticles.cell) plus
+        # float32 offsets in degrees (lon/lat hold the offsets)
+        self.cell_positions = bool(cell_positions)
+        if self.cell_positions and self.diagnostics:
+            raise ValueError("diagnostics take absolute positions; cell_positions=True has none on the device")
         self.reorder_every = int(reorder_every)
         # compact: on every new met hour, sort the particles by met cell with the
         # ones that can never be active again (finished releases, escaped) last,
@@ -332,8 +338,10 @@ class BatchEngine:
         f32 = torch.float32
         dtypes = dict(rel_idx=torch.int32, uid=torch.int32, alive=torch.uint8)
         self.bufs = []
+        keys = self.STATE + (("cell",) if self.cell_positions else ())
+        dtypes["cell"] = torch.int32
         for _ in range(2):  # double buffer for the reordering permutation
-            self.bufs.append({k: torch.empty(n, dtype=dtypes.get(k, f32), device=d) for k in self.STATE})
+            self.bufs.append({k: torch.empty(n, dtype=dtypes.get(k, f32), device=d) for k in keys})
         self.cur = 0
         self._parts = [self._make_parts(b) for b in self.bufs]
         self.keys = torch.empty(n, dtype=torch.int32, device=d)
@@ -425,7 +433,61 @@ class BatchEngine:
             vm=b["vm"].data_ptr(), weight=b["weight"].data_ptr(), r_off=b["r_off"].data_ptr(),
             we_off=b["we_off"].data_ptr(), rel_idx=b["rel_idx"].data_ptr(), uid=b["uid"].data_ptr(),
             alive=b["alive"].data_ptr(), active=None,
-            lo_off=b["lo_off"].data_ptr() if self.per_particle_length else None)
+            lo_off=b["lo_off"].data_ptr() if self.per_particle_length else None,
+            cell=b["cell"].data_ptr() if self.cell_positions else None)
+
+    # -- the cell layout: grid point + offsets <-> absolute coordinates ----
+    def _cell_consts(self):
+        g = self.met.grid
+        f = np.float32
+        return (f(g.lon0), f(g.lat0), f((g.lon1 - g.lon0) / (g.nx - 1)), f((g.lat1 - g.lat0) / (g.ny - 1)),
+                f((g.nx - 1) / (g.lon1 - g.lon0)), f((g.ny - 1) / (g.lat1 - g.lat0)), g.nx, g.ny)
+
+    def _cell_points(self, cell):
+        """The float32 grid point of each cell, as the kernels form it
+        (fma(cx, dlon, lon0), one rounding of an exact double sum)."""
+
+        lon0, lat0, dlon, dlat, _, _, nx, _ = self._cell_consts()
+        cy, cx = np.divmod(np.asarray(cell, np.int64), nx)
+        rlon = (np.float64(lon0) + cx * np.float64(dlon)).astype(np.float32)
+        rlat = (np.float64(lat0) + cy * np.float64(dlat)).astype(np.float32)
+        return rlon, rlat
+
+    def to_cell(self, lon, lat):
+        """(cell, lon offset, lat offset) of absolute positions: the nearest
+        grid point and the float32 offsets from it."""
+
+        lon0, lat0, _, _, sx, sy, nx, ny = self._cell_consts()
+        lon = np.asarray(lon, np.float64)
+        lat = np.asarray(lat, np.float64)
+        cx = np.clip(np.floor((lon - lon0) * np.float64(sx) + 0.5), 0, nx - 1).astype(np.int64)
+        cy = np.clip(np.floor((lat - lat0) * np.float64(sy) + 0.5), 0, ny - 1).astype(np.int64)
+        cell = (cy * nx + cx).astype(np.int32)
+        rlon, rlat = self._cell_points(cell)
+        return cell, (lon - rlon).astype(np.float32), (lat - rlat).astype(np.float32)
+
+    def from_cell(self, cell, off_lon, off_lat):
+        """Absolute float64 positions of a cell layout: grid point + offset, exactly."""
+
+        rlon, rlat = self._cell_points(cell)
+        return (rlon.astype(np.float64) + np.asarray(off_lon, np.float64),
+                rlat.astype(np.float64) + np.asarray(off_lat, np.float64))
+
+    def _put_positions(self, b, lon, lat) -> None:
+        if self.cell_positions:
+            cell, ol, oa = self.to_cell(lon, lat)
+            b["cell"].copy_(torch.from_numpy(cell))
+            b["lon"].copy_(torch.from_numpy(ol))
+            b["lat"].copy_(torch.from_numpy(oa))
+        else:
+            b["lon"].copy_(torch.from_numpy(np.asarray(lon, np.float32)))
+            b["lat"].copy_(torch.from_numpy(np.asarray(lat, np.float32)))
+
+    def _get_positions(self, b, order):
+        lon, lat = b["lon"].cpu().numpy()[order], b["lat"].cpu().numpy()[order]
+        if self.cell_positions:
+            return self.from_cell(b["cell"].cpu().numpy()[order], lon, lat)
+        return lon, lat
 
     @property
     def parts(self) -> L.GlideParticles:
@@ -515,10 +577,11 @@ class BatchEngine:
         self.cur = 0
         b = self.bufs[0]
         ra = self.ra
-        for k, v in (("lon", ra.lon), ("lat", ra.lat), ("z", ra.z), ("weight", ra.weight), ("r_off", ra.r_off),
+        for k, v in (("z", ra.z), ("weight", ra.weight), ("r_off", ra.r_off),
                      ("we_off", ra.we_off), ("rel_idx", ra.rel_idx), ("uid", ra.uid.view(np.int32)),
                      ("lo_off", ra.te_off if self.release_terminus else ra.lo_off)):
             b[k].copy_(torch.from_numpy(np.ascontiguousarray(v)))
+        self._put_positions(b, ra.lon, ra.lat)
         for k in ("up", "vp", "wp", "um", "vm"):
             b[k].zero_()
         b["alive"].fill_(1)
@@ -730,7 +793,8 @@ class BatchEngine:
         b = self.bufs[self.cur]
         uid = b["uid"].cpu().numpy().view(np.uint32)
         order = np.argsort(uid, kind="stable")
-        return {"lon": b["lon"].cpu().numpy()[order], "lat": b["lat"].cpu().numpy()[order],
+        lon, lat = self._get_positions(b, order)
+        return {"lon": np.asarray(lon, np.float32), "lat": np.asarray(lat, np.float32),
                 "alt": b["z"].cpu().numpy()[order], "weight": b["weight"].cpu().numpy()[order],
                 "release_idx": b["rel_idx"].cpu().numpy()[order].astype(np.int64)}
 
@@ -927,7 +991,9 @@ class BatchEngine:
         b = self.bufs[self.cur]
         uid = b["uid"].cpu().numpy().view(np.uint32)
         order = np.argsort(uid, kind="stable")
-        out = {k: b[k].cpu().numpy()[order] for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm")}
+        out = {k: b[k].cpu().numpy()[order] for k in ("z", "up", "vp", "wp", "um", "vm")}
+        # absolute positions (float64 exactly when the cell layout holds them)
+        out["lon"], out["lat"] = self._get_positions(b, order)
         out["alive"] = b["alive"].cpu().numpy().astype(bool)[order]
         out["uid"] = uid[order]
         return out
@@ -944,6 +1010,7 @@ class BatchEngine:
                      ("uid", ra.uid.view(np.int32)),
                      ("lo_off", ra.te_off if self.release_terminus else ra.lo_off)):
             b[k].copy_(torch.from_numpy(np.ascontiguousarray(v)))
-        for k in ("lon", "lat", "z", "up", "vp", "wp", "um", "vm"):
+        for k in ("z", "up", "vp", "wp", "um", "vm"):
             b[k].copy_(torch.from_numpy(np.asarray(S[k], np.float32)))
+        self._put_positions(b, S["lon"], S["lat"])
         b["alive"].fill_(1)

@@ -813,10 +813,19 @@ class Server:
         return self.take_completed()
 
     # -- request loop ------------------------------------------------------
-    def serve(self, requests: list[Request], max_wall_s: float | None = None) -> ServeStats:
+    def serve(self, requests: list[Request], max_wall_s: float | None = None, progress_every: int = 0,
+              log=None) -> ServeStats:
         """Real-time request loop: requests arrive at ``arrival_s`` (seconds
         after the start), are admitted as slots and pool blocks free up, and
-        are stepped together until all are done."""
+        are stepped together until all are done. With ``progress_every`` > 0,
+        ``log`` (default: print to stderr) gets a line each time that many
+        more requests have completed."""
+
+        if progress_every and log is None:
+            import sys
+
+            def log(msg):
+                print(msg, file=sys.stderr, flush=True)
 
         stats = ServeStats()
         queue = sorted(requests, key=lambda r: r.arrival_s)
@@ -843,7 +852,12 @@ class Server:
                 time.sleep(0.001)
             elif qi < len(queue):
                 time.sleep(max(0.0, min(0.01, queue[qi].arrival_s - self.now())))
-            stats.results.extend(self.take_completed())
+            done = self.take_completed()
+            stats.results.extend(done)
+            if progress_every and done and len(stats.results) // progress_every != \
+                    (len(stats.results) - len(done)) // progress_every:
+                log(f"[serve] {self.now():.0f} s: {len(stats.results)} of {len(queue)} requests done, "
+                    f"{len(self.running)} running, {self.step_count} steps")
             if max_wall_s is not None and self.now() > max_wall_s:
                 break
         torch.cuda.synchronize()

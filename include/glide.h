@@ -130,6 +130,16 @@ typedef struct GlideWindow {
  *            r_off - p->length_s for every particle. With
  *            GLIDE_F_RELEASE_TERMINUS it holds the release terminus instead.
  * um/vm may be NULL when meander is off.
+ *   cell:    (may be NULL) the precision layout: cell[i] = cy * nx + cx, a
+ *            point of the launch's met grid, and lon/lat hold the offsets in
+ *            degrees from that point, rlon = fma(cx, dlon, lon0) and rlat =
+ *            fma(cy, dlat, lat0) in float32 (dlon = (lon1 - lon0) / (nx - 1),
+ *            dlat likewise, rounded to float). A step moves the offsets and
+ *            leaves each particle at its nearest grid point, so the offsets
+ *            stay within about half a cell and keep float32 precision relative
+ *            to the cell instead of to the coordinate. Taken by glide_step,
+ *            glide_step_table, glide_convect, glide_reorder_keys and
+ *            glide_permute_particles (pos_stride 1); the others reject it.
  */
 typedef struct GlideParticles {
     int64_t n;
@@ -150,6 +160,7 @@ typedef struct GlideParticles {
     uint8_t* alive;
     const uint8_t* active;
     const float* lo_off;
+    int32_t* cell;
 } GlideParticles;
 
 /*
