@@ -158,24 +158,34 @@ def _timed_steps(eng, K: int, sptr: int) -> float:
     return time.perf_counter() - t0
 
 
-def _kernel_pass(eng, n_steps: int, stream, sptr: int, every: int = 1) -> dict:
-    """From the release state, run ``n_steps`` steps with HIP events (on the
-    launch stream) around every ``every``-th step-kernel launch; the average
-    duration and the active particles of the timed launches."""
+def _kernel_pass(eng, n_steps: int, stream, sptr: int) -> dict:
+    """From the release state, run ``n_steps`` eager steps, each step-kernel
+    launch recording its own start and end (HIP events of the launch, on the
+    launch stream: glide_step_timing_arm); the average duration and the active
+    particles of the launches that had active particles."""
+
+    import ctypes
 
     import numpy as np
     import torch
 
+    from glide_amd import _lib as L
+
     eng.reset()
+    eng.use_graph = False
     torch.cuda.synchronize()
     n = min(n_steps, len(eng.sched))
-    ev = eng.time_step_kernel(n, stream, every=every)
+    L.check(L.lib().glide_step_timing_arm(n))
     for _ in range(n):
         eng.run_step(sptr)
     eng.join(sptr)
     torch.cuda.synchronize()
+    ms = np.zeros(n, np.float32)
+    got = L.lib().glide_step_timing_read(ms.ctypes.data_as(ctypes.c_void_p), n)
+    if got != n:
+        raise RuntimeError(f"{got} of {n} step launches timed")
     act = eng.active_steps.cpu().numpy()[:n].astype(np.float64)
-    timed = [(i, a.elapsed_time(b)) for i, (a, b) in sorted(ev.items()) if act[i] > 0]
+    timed = [(i, float(ms[i])) for i in range(n) if act[i] > 0]
     t_ms = np.array([t for _, t in timed])
     t_act = np.array([act[i] for i, _ in timed])
     return {"launches": int(len(t_ms)), "avg_us": float(t_ms.mean() * 1e3) if len(t_ms) else float("nan"),
@@ -269,7 +279,7 @@ def headline_leg(args, stream, sptr: int) -> tuple[dict, dict]:
     eng.use_graph = False
     kp = _kernel_pass(eng, max(K, args.kernel_steps), stream, sptr)
     kernel = (f"glide::step_kernel<{str(met_dtype == L.GLIDE_MET_BF16).lower()}, 0, "
-              f"{str(args.accumulate == 'fixed').lower()}>")
+              f"{str(args.accumulate == 'fixed').lower()}, false>")
     roofline = _roofline(kp, kernel, _counters(WORKLOAD, args.met) if args.accumulate == "fixed" else {})
     roofline["whole_step_achieved"] = round(BYTES_PER_PARTICLE_STEP * psteps / elapsed / 1e9, 2)
     out = {
@@ -488,26 +498,33 @@ def batched_leg(args, name: str, stream, sptr: int, shared: dict, evidence: dict
     else:
         tuned = tune(cfg, met, candidates=BATCH_CANDIDATES, use_cache=False, stream=stream, device_met=dm, **kw)
         shared["tuned"] = tuned["best"]
-    # the precision variant holds positions as a grid point plus float32 offsets
-    eng = BatchEngine(cfg, device="cuda", met=met, device_met=dm, cell_positions=bf16, **kw, **tuned["best"])
+    eng = BatchEngine(cfg, device="cuda", met=met, device_met=dm, **kw, **tuned["best"])
     n_sched = len(eng.sched)
     K = n_sched if args.batched_steps <= 0 else min(args.batched_steps, n_sched)
     eng.run(min(20, n_sched))  # warm-up, then back to the release state
     torch.cuda.synchronize()
     eng.reset()
     setup_s = time.perf_counter() - t_setup
-    ev = eng.time_step_kernel(K, stream, every=7)  # every 7th launch (coprime with the re-binning interval)
+    import ctypes
+
+    L.check(L.lib().glide_step_timing_arm(K))  # every step launch times itself (its own events)
     elapsed = _timed_steps(eng, K, sptr)
     act = eng.active_steps.cpu().numpy()[:K].astype(np.float64)
     psteps = int(act.sum())
-    timed = [(i, a.elapsed_time(b)) for i, (a, b) in sorted(ev.items()) if act[i] > 0]
-    t_ms = np.array([t for _, t in timed])
-    t_act = np.array([act[i] for i, _ in timed])
+    ms = np.zeros(K, np.float32)
+    got = L.lib().glide_step_timing_read(ms.ctypes.data_as(ctypes.c_void_p), K)
+    if got == K:  # one launch per step
+        timed = [(i, float(ms[i])) for i in range(K) if act[i] > 0]
+        t_ms = np.array([t for _, t in timed])
+        t_act = np.array([act[i] for i, _ in timed])
+    else:  # steps over an empty live set launch nothing: pair the launches with the non-empty steps
+        t_act = act[act > 0][:got]
+        t_ms = ms[:t_act.size].astype(np.float64)
     kp = {"launches": int(len(t_ms)), "avg_us": float(t_ms.mean() * 1e3) if len(t_ms) else float("nan"),
           "particles_per_launch": float(t_act.mean()) if len(t_act) else 0.0,
           "bytes_per_s": (BYTES_PER_PARTICLE_STEP * float(t_act.sum()) / (float(t_ms.sum()) * 1e-3)
                           if len(t_ms) else float("nan"))}
-    roofline = _roofline(kp, f"glide::step_kernel<{str(bf16).lower()}, 0, true>", {})
+    roofline = _roofline(kp, f"glide::step_kernel<{str(bf16).lower()}, 0, true, {str(bf16).lower()}>", {})
     roofline["whole_step_achieved"] = round(BYTES_PER_PARTICLE_STEP * psteps / elapsed / 1e9, 2)
     g = cfg["grid"]
     tot = eng.release_total.cpu().numpy()
@@ -536,14 +553,27 @@ def batched_leg(args, name: str, stream, sptr: int, shared: dict, evidence: dict
         "device_gb_allocated": round(torch.cuda.max_memory_allocated() / 1e9, 1),
         "autotune": {"chosen": tuned["best"], "trials": tuned["trials"], **({"from": tuned["from"]}
                                                                             if "from" in tuned else {})},
-        "positions": "int32 grid point + float32 offsets" if bf16 else "float32 lon/lat/z",
+        "positions": "float32 lon/lat/z",
         "roofline": roofline,
     }
     eng.close()
     del eng
     torch.cuda.empty_cache()
+    # the other position layout (int32 grid point + float32 offsets): the first
+    # steps of the batch timed, and the sampled releases alone against the
+    # float64 oracle (a release alone equals it in the batch)
+    kc = min(2000, n_sched)
+    times = {}
+    for cell in (False, True):
+        e2 = BatchEngine(cfg, device="cuda", met=met, device_met=dm, cell_positions=cell, **kw, **tuned["best"])
+        times[cell] = _timed_steps(e2, kc, sptr)
+        e2.close()
+        del e2
+        torch.cuda.empty_cache()
+    out["positions_cell"] = {"steps": kc, "ms_per_step": times[True] * 1e3 / kc,
+                             "float32_positions_ms_per_step": times[False] * 1e3 / kc,
+                             "speed_vs_float32_positions": times[False] / times[True]}
     if K == n_sched:
-        # the sampled releases alone in the other position layout (a release alone equals it in the batch)
         import dataclasses
 
         for r in sampled_releases(cfg, args.check_releases):
@@ -551,9 +581,9 @@ def batched_leg(args, name: str, stream, sptr: int, shared: dict, evidence: dict
             one = dataclasses.replace(rel, release_idx=0, noise_id=rel.release_idx)
             c1 = dict(cfg, releases=[one], grid=dict(cfg["grid"], n_r=1))
             e1 = BatchEngine(c1, device="cuda", met=met, device_met=dm, release_terminus=True, met_dtype=met_dtype,
-                             cell_positions=not bf16)
+                             cell_positions=True)
             e1.run()
-            evidence[r][f"{name} alone, {'float32 positions' if bf16 else 'cell positions'}"] = {
+            evidence[r][f"{name} alone, cell positions"] = {
                 "footprint": e1.release_footprint(0).cpu().numpy().astype(np.float64),
                 "total": float(e1.release_total.cpu().numpy()[0])}
             e1.close()

@@ -135,6 +135,8 @@ SIGNATURES = {
     "glide_particle_means": (C.c_int, [P(GlideParticles), vp, C.c_int32, vp, vp, C.c_size_t, vp]),
     "glide_field_means": (C.c_int, [vp, C.c_int64, C.c_int32, vp, vp, C.c_size_t, vp]),
     "glide_slots_advance": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, vp]),
+    "glide_step_timing_arm": (C.c_int, [C.c_int32]),
+    "glide_step_timing_read": (C.c_int, [vp, C.c_int32]),
     "glide_step_slots": (C.c_int, [P(GlideStepParams), vp, vp, P(GlideParticles), P(GlideGrid), P(GlideMetRing),
                                    P(GlideFootprintGrid), P(GlideEmit), vp]),
     "glide_footprint_keys": (C.c_int, [C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp, vp, C.c_double,

@@ -324,7 +324,7 @@ __device__ __forceinline__ void convect_particle(const ConvectArgs& a, const flo
     float gx, gy;
     if (s.cell != nullptr) {
         const int32_t c = s.cell[i];
-        const int pcy = c / a.nx, pcx = c - pcy * a.nx;
+        const int pcy = c >> 16, pcx = c & 0xffff;
         gx = cell_index(pcx, lon, a.sx, a.nx);
         gy = cell_index(pcy, lat, a.sy, a.ny);
         lon = __builtin_fmaf(static_cast<float>(pcx), a.cdlon, a.lon0) + lon;
@@ -604,7 +604,8 @@ extern "C" int glide_convect(const GlideStepParams* p, const GlideParticles* s, 
     a.sy = static_cast<float>((g->ny - 1) / (g->lat1 - g->lat0));
     a.cdlon = static_cast<float>((g->lon1 - g->lon0) / (g->nx - 1));
     a.cdlat = static_cast<float>((g->lat1 - g->lat0) / (g->ny - 1));
-    GLIDE_REQUIRE(s->cell == nullptr || s->pos_stride == 1, "cell positions need pos_stride 1");
+    GLIDE_REQUIRE(s->cell == nullptr || (s->pos_stride == 1 && g->nx <= 65535 && g->ny <= 65535),
+                  "cell positions need pos_stride 1 and nx, ny <= 65535");
     a.levels = g->levels;
     a.i_lcl = i_lcl;
     a.i_lnb = i_lnb;

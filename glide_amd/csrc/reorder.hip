@@ -60,7 +60,7 @@ __device__ __forceinline__ bool key_one(const ReorderKeyArgs& a, const float* le
         int ix, iy;
         if (s.cell != nullptr) {  // a grid point plus offsets
             const int32_t c = s.cell[i];
-            const int pcy = c / a.nx, pcx = c - pcy * a.nx;
+            const int pcy = c >> 16, pcx = c & 0xffff;
             ix = static_cast<int>(cell_index(pcx, s.lon[i], a.sx, a.nx));
             iy = static_cast<int>(cell_index(pcy, s.lat[i], a.sy, a.ny));
         } else {
@@ -151,7 +151,8 @@ extern "C" int glide_reorder_keys(const GlideStepParams* p, const GlideParticles
     GLIDE_REQUIRE(s->pos_stride >= 1 && s->n >= 0, "bad particles");
     GLIDE_REQUIRE(g->nz >= 2 && g->nz <= kMaxLev && g->nx >= 2 && g->ny >= 2, "bad grid");
     GLIDE_REQUIRE(g->lon1 != g->lon0 && g->lat1 != g->lat0, "Grid dimension has 0 span (first == last)");
-    GLIDE_REQUIRE(s->cell == nullptr || s->pos_stride == 1, "cell positions need pos_stride 1");
+    GLIDE_REQUIRE(s->cell == nullptr || (s->pos_stride == 1 && g->nx <= 65535 && g->ny <= 65535),
+                  "cell positions need pos_stride 1 and nx, ny <= 65535");
     if (s->n == 0) return GLIDE_OK;
     ReorderKeyArgs a{};
     a.s = *s;

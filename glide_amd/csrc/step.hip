@@ -14,6 +14,10 @@
 // torch.where gate). Each active particle runs only its own k substeps: with
 // the counter-based noise a sub_dt = 0 substep is the identity, so this equals
 // the fixed max_substeps loop (hanna.py:1245-1254).
+#include <hip/hip_ext.h>
+
+#include <vector>
+
 #include "footprint_key.h"
 #include "physics.h"
 
@@ -121,8 +125,8 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     float rlon = 0.0f, rlat = 0.0f;
     if constexpr (CELL) {
         const int32_t c = ldo(s.cell, i4);
-        cy = c / nx;
-        cx = c - cy * nx;
+        cy = c >> 16;
+        cx = c & 0xffff;
         rlon = __builtin_fmaf(static_cast<float>(cx), a.cdlon, a.lon0);
         rlat = __builtin_fmaf(static_cast<float>(cy), a.cdlat, a.lat0);
     }
@@ -340,7 +344,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         cy = ncy;
         rlon = __builtin_fmaf(static_cast<float>(cx), a.cdlon, a.lon0);
         rlat = __builtin_fmaf(static_cast<float>(cy), a.cdlat, a.lat0);
-        sto(s.cell, i4, cy * nx + cx);
+        sto(s.cell, i4, (cy << 16) | cx);
     }
     sto(s.lon, ip, lon);
     sto(s.lat, ip, lat);
@@ -565,15 +569,32 @@ static int fill_grid(StepArgs& a, const GlideGrid& g, const GlideWindow& w) {
 
 using namespace glide;
 
+// Launch timing (glide_step_timing_arm/read): the next armed step launches of
+// the calling thread record their own start and end (events of the launch).
+struct StepTiming {
+    std::vector<hipEvent_t> ev;  // start, stop per recorded launch
+    int armed = 0;
+    int used = 0;
+};
+static thread_local StepTiming g_timing;
+
+template <typename K>
+static void launch_one(K kernel, dim3 grid, dim3 block, hipStream_t st, const StepArgs& a, hipEvent_t e0,
+                       hipEvent_t e1) {
+    if (e0 != nullptr) hipExtLaunchKernelGGL(kernel, grid, block, 0, st, e0, e1, 0, a);
+    else hipLaunchKernelGGL(kernel, grid, block, 0, st, a);
+}
+
 // One launch of the step kernel variant for the met precision and the footprint mode.
 template <int MODE, bool CELL>
-static void launch_variants(bool bf16, bool fixed, dim3 grid, dim3 block, hipStream_t st, const StepArgs& a) {
+static void launch_variants(bool bf16, bool fixed, dim3 grid, dim3 block, hipStream_t st, const StepArgs& a,
+                            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     if (fixed) {
-        if (bf16) hipLaunchKernelGGL((step_kernel<true, MODE, true, CELL>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((step_kernel<false, MODE, true, CELL>), grid, block, 0, st, a);
+        if (bf16) launch_one(step_kernel<true, MODE, true, CELL>, grid, block, st, a, e0, e1);
+        else launch_one(step_kernel<false, MODE, true, CELL>, grid, block, st, a, e0, e1);
     } else {
-        if (bf16) hipLaunchKernelGGL((step_kernel<true, MODE, false, CELL>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((step_kernel<false, MODE, false, CELL>), grid, block, 0, st, a);
+        if (bf16) launch_one(step_kernel<true, MODE, false, CELL>, grid, block, st, a, e0, e1);
+        else launch_one(step_kernel<false, MODE, false, CELL>, grid, block, st, a, e0, e1);
     }
 }
 
@@ -606,6 +627,7 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
     if (cell) {
         GLIDE_REQUIRE(slots == nullptr, "cell positions are not taken by glide_step_slots");
         GLIDE_REQUIRE(s->pos_stride == 1, "cell positions need pos_stride 1");
+        GLIDE_REQUIRE(g->nx <= 65535 && g->ny <= 65535, "cell positions need nx, ny <= 65535");
     }
     StepArgs a{};
     a.p = *p;
@@ -661,9 +683,20 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
     const dim3 grid(static_cast<unsigned>(blocks)), block(static_cast<unsigned>(bs));
     const bool bf16 = w->dtype == GLIDE_MET_BF16;
     const bool fixed = (p->flags & (GLIDE_F_EMIT | GLIDE_F_FIXED)) == (GLIDE_F_EMIT | GLIDE_F_FIXED);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g_timing.armed > 0 && slots == nullptr) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        GLIDE_HIP(hipStreamIsCapturing(st, &cs));
+        if (cs == hipStreamCaptureStatusNone) {  // never inside a captured graph
+            e0 = g_timing.ev[2 * g_timing.used];
+            e1 = g_timing.ev[2 * g_timing.used + 1];
+            ++g_timing.used;
+            --g_timing.armed;
+        }
+    }
     if (table != nullptr) {
-        if (cell) launch_variants<kModeTable, true>(bf16, fixed, grid, block, st, a);
-        else launch_variants<kModeTable, false>(bf16, fixed, grid, block, st, a);
+        if (cell) launch_variants<kModeTable, true>(bf16, fixed, grid, block, st, a, e0, e1);
+        else launch_variants<kModeTable, false>(bf16, fixed, grid, block, st, a, e0, e1);
     } else if (slots != nullptr) {
         if (fixed) {
             if (bf16) hipLaunchKernelGGL((step_kernel<true, kModeSlots, true>), grid, block, 0, st, a);
@@ -673,11 +706,35 @@ static int launch_step(const GlideStepParams* p, const GlideStepParams* table, i
             else hipLaunchKernelGGL((step_kernel<false, kModeSlots, false>), grid, block, 0, st, a);
         }
     } else {
-        if (cell) launch_variants<kModePlain, true>(bf16, fixed, grid, block, st, a);
-        else launch_variants<kModePlain, false>(bf16, fixed, grid, block, st, a);
+        if (cell) launch_variants<kModePlain, true>(bf16, fixed, grid, block, st, a, e0, e1);
+        else launch_variants<kModePlain, false>(bf16, fixed, grid, block, st, a, e0, e1);
     }
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;
+}
+
+extern "C" int glide_step_timing_arm(int32_t n) {
+    GLIDE_REQUIRE(n >= 0 && n <= (1 << 20), "glide_step_timing_arm: n must be in [0, 2^20]");
+    const size_t need = 2 * static_cast<size_t>(n);
+    while (g_timing.ev.size() < need) {
+        hipEvent_t e;
+        GLIDE_HIP(hipEventCreate(&e));
+        g_timing.ev.push_back(e);
+    }
+    g_timing.armed = n;
+    g_timing.used = 0;
+    return GLIDE_OK;
+}
+
+extern "C" int glide_step_timing_read(float* ms, int32_t n) {
+    GLIDE_REQUIRE(ms != nullptr || n == 0, "glide_step_timing_read: NULL output");
+    const int m = g_timing.used < n ? g_timing.used : n;
+    for (int k = 0; k < m; ++k) {
+        GLIDE_HIP(hipEventSynchronize(g_timing.ev[2 * k + 1]));
+        GLIDE_HIP(hipEventElapsedTime(ms + k, g_timing.ev[2 * k], g_timing.ev[2 * k + 1]));
+    }
+    g_timing.armed = 0;
+    return m;
 }
 
 extern "C" int glide_step(const GlideStepParams* p, const GlideParticles* s, const GlideGrid* g,

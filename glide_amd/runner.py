@@ -447,8 +447,9 @@ class BatchEngine:
         """The float32 grid point of each cell, as the kernels form it
         (fma(cx, dlon, lon0), one rounding of an exact double sum)."""
 
-        lon0, lat0, dlon, dlat, _, _, nx, _ = self._cell_consts()
-        cy, cx = np.divmod(np.asarray(cell, np.int64), nx)
+        lon0, lat0, dlon, dlat, _, _, _, _ = self._cell_consts()
+        c = np.asarray(cell, np.int64)
+        cy, cx = c >> 16, c & 0xFFFF
         rlon = (np.float64(lon0) + cx * np.float64(dlon)).astype(np.float32)
         rlat = (np.float64(lat0) + cy * np.float64(dlat)).astype(np.float32)
         return rlon, rlat
@@ -462,7 +463,7 @@ class BatchEngine:
         lat = np.asarray(lat, np.float64)
         cx = np.clip(np.floor((lon - lon0) * np.float64(sx) + 0.5), 0, nx - 1).astype(np.int64)
         cy = np.clip(np.floor((lat - lat0) * np.float64(sy) + 0.5), 0, ny - 1).astype(np.int64)
-        cell = (cy * nx + cx).astype(np.int32)
+        cell = ((cy << 16) | cx).astype(np.int32)
         rlon, rlat = self._cell_points(cell)
         return cell, (lon - rlon).astype(np.float32), (lat - rlat).astype(np.float32)
 

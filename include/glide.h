@@ -130,8 +130,8 @@ typedef struct GlideWindow {
  *            r_off - p->length_s for every particle. With
  *            GLIDE_F_RELEASE_TERMINUS it holds the release terminus instead.
  * um/vm may be NULL when meander is off.
- *   cell:    (may be NULL) the precision layout: cell[i] = cy * nx + cx, a
- *            point of the launch's met grid, and lon/lat hold the offsets in
+ *   cell:    (may be NULL) the precision layout: cell[i] = cy << 16 | cx, a
+ *            point of the launch's met grid (nx, ny <= 65535), and lon/lat hold the offsets in
  *            degrees from that point, rlon = fma(cx, dlon, lon0) and rlat =
  *            fma(cy, dlat, lat0) in float32 (dlon = (lon1 - lon0) / (nx - 1),
  *            dlat likewise, rounded to float). A step moves the offsets and
@@ -524,6 +524,18 @@ int glide_convect(const GlideStepParams* p, const GlideParticles* s, const Glide
                   const int32_t* i_lcl, const int32_t* i_lnb, const float* m_b,
                   const float* layer_mass, double* colsum_err, float* cloud, const GlideFootprintGrid* fg,
                   const GlideEmit* e, void* stream);
+
+/*
+ * Launch timing, for measurement (no counterpart in the reference):
+ * glide_step_timing_arm(n) makes the next n step-kernel launches of glide_step
+ * and glide_step_table on the calling thread (outside graph capture) record
+ * the start and end of the kernel itself (the launch's own events);
+ * glide_step_timing_read(ms, n) waits for them, writes each one's duration in
+ * milliseconds to ms[0 .. n) and returns how many launches were recorded
+ * (arming again starts a new record).
+ */
+int glide_step_timing_arm(int32_t n);
+int glide_step_timing_read(float* ms, int32_t n);
 
 /*
  * Serving: many releases (requests), each with its own cursor, stepped

@@ -85,7 +85,7 @@ def test_cell_layout_per_step_errors_against_float64(cfg0):
     b = eng.bufs[eng.cur]
     g = eng.met.grid
     cell = b["cell"].cpu().numpy()
-    assert cell.min() >= 0 and cell.max() < g.nx * g.ny
+    assert (cell & 0xFFFF).max() < g.nx and (cell >> 16).max() < g.ny and cell.min() >= 0
     dlon, dlat = (g.lon1 - g.lon0) / (g.nx - 1), abs(g.lat1 - g.lat0) / (g.ny - 1)
     alive = b["alive"].cpu().numpy() != 0
     assert np.abs(b["lon"].cpu().numpy()[alive]).max() <= 0.6 * dlon
