@@ -128,25 +128,94 @@ __device__ __forceinline__ long long to_fixed(float v, int e) {
     return static_cast<long long>(llrint(ldexp(static_cast<double>(v), e)));
 }
 
-// Adds each lane's fixed-point value to acc[key] (kNoCell dropped): an
-// inclusive segmented scan over runs of equal keys in lane order, then one
-// 64-bit integer atomic per run from its last lane. Integer addition is
-// associative, so the sums do not depend on the order the atomics land in.
+// 64-bit read of another lane's value through DPP (CTRL: the DPP control
+// word), 0 where the source lane is outside the row or the row is masked off
+// (ROWMASK). The wavefront must be fully active.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
+    const uint32_t lo = __builtin_amdgcn_update_dpp(0u, static_cast<uint32_t>(x), CTRL, ROWMASK, 0xf, false);
+    const uint32_t hi = __builtin_amdgcn_update_dpp(0u, static_cast<uint32_t>(x >> 32), CTRL, ROWMASK, 0xf, false);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+// Inclusive segmented scan over the wavefront: lane l gets the sum of lanes
+// [start, l] (start <= l, the first lane of its segment). Hillis-Steele steps
+// within rows of 16 lanes (row_shr 1, 2, 4, 8), then the last lane of row 0
+// into row 1 and of row 2 into row 3 (row_bcast:15), then lane 31 into rows 2
+// and 3 (row_bcast:31); a step adds only when its source lane is in the same
+// segment. Exact in integers.
+__device__ __forceinline__ long long seg_scan_i64(long long v, int lane, int start) {
+    unsigned long long u = static_cast<unsigned long long>(v), o;
+    o = dpp_u64<kDppRowShr1, 0xf>(u);
+    if (lane - 1 >= start) u += o;
+    o = dpp_u64<kDppRowShr2, 0xf>(u);
+    if (lane - 2 >= start) u += o;
+    o = dpp_u64<kDppRowShr4, 0xf>(u);
+    if (lane - 4 >= start) u += o;
+    o = dpp_u64<kDppRowShr8, 0xf>(u);
+    if (lane - 8 >= start) u += o;
+    o = dpp_u64<kDppRowBcast15, 0xa>(u);
+    if ((lane & ~15) - 1 >= start) u += o;
+    o = dpp_u64<kDppRowBcast31, 0xc>(u);
+    if (31 >= start) u += o;
+    return static_cast<long long>(u);
+}
+
+// Sum of v over the wavefront (the same DPP steps without segments, read
+// from lane 63).
+__device__ __forceinline__ long long wave_sum_i64_dpp(long long v) {
+    unsigned long long u = static_cast<unsigned long long>(v);
+    u += dpp_u64<kDppRowShr1, 0xf>(u);
+    u += dpp_u64<kDppRowShr2, 0xf>(u);
+    u += dpp_u64<kDppRowShr4, 0xf>(u);
+    u += dpp_u64<kDppRowShr8, 0xf>(u);
+    u += dpp_u64<kDppRowBcast15, 0xa>(u);
+    u += dpp_u64<kDppRowBcast31, 0xc>(u);
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(u), 63);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(u >> 32), 63);
+    return static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// Adds each lane's fixed-point value to acc[key] (kNoCell dropped), exactly:
+// integer addition is associative, so the sums do not depend on how the
+// lanes are grouped or the order the atomics land in.
+//
+// A wavefront's particles were binned by met column at the last re-binning
+// and have drifted since, so its keys are few but interleaved. First, up to
+// kMatchRounds times: the lowest pending lane's key, the wavefront sum of the
+// lanes holding that key, one 64-bit atomic from that lane. What is left
+// (a wavefront over many cells) goes as runs of equal keys in lane order: an
+// inclusive segmented scan, one atomic per run from its last lane.
 // Every lane of the wavefront must call it (no lane may have returned).
+constexpr int kMatchRounds = 8;
 __device__ __forceinline__ void wave_fixed_add(uint64_t key, long long v, long long* acc) {
-    if (__ballot(key != kNoCell) == 0ull) return;
+    bool pend = key != kNoCell && v != 0;
+    unsigned long long m = __ballot(pend);
+    if (m == 0ull) return;
     const int lane = static_cast<int>(__lane_id());
-    const uint64_t prev = __shfl_up(static_cast<unsigned long long>(key), 1u);
-    const unsigned long long heads = __ballot(lane == 0 || prev != key);
-    const int start = 63 - __clzll(static_cast<long long>(heads & ((2ull << lane) - 1ull)));
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const long long o = __shfl_up(v, static_cast<unsigned>(d));
-        if (lane - d >= start) v += o;
+    for (int round = 0; round < kMatchRounds && m != 0ull; ++round) {
+        const int leader = __ffsll(static_cast<long long>(m)) - 1;
+        const uint32_t klo = __builtin_amdgcn_readlane(static_cast<uint32_t>(key), leader);
+        const uint32_t khi = __builtin_amdgcn_readlane(static_cast<uint32_t>(key >> 32), leader);
+        const uint64_t kk = (static_cast<uint64_t>(khi) << 32) | klo;
+        const bool mine = pend && key == kk;
+        const long long t = wave_sum_i64_dpp(mine ? v : 0);
+        if (lane == leader)
+            atomicAdd(reinterpret_cast<unsigned long long*>(acc + kk), static_cast<unsigned long long>(t));
+        pend = pend && !mine;
+        m = __ballot(pend);
     }
+    if (m == 0ull) return;
+    const uint64_t k = pend ? key : kNoCell;
+    const uint64_t prev = __shfl_up(static_cast<unsigned long long>(k), 1u);
+    const unsigned long long heads = __ballot(lane == 0 || prev != k);
+    const int start = 63 - __clzll(static_cast<long long>(heads & ((2ull << lane) - 1ull)));
+    const long long s = seg_scan_i64(pend ? v : 0, lane, start);
     const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull) != 0ull;
-    if (tail && key != kNoCell && v != 0)
-        atomicAdd(reinterpret_cast<unsigned long long*>(acc + key), static_cast<unsigned long long>(v));
+    if (tail && k != kNoCell && s != 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(acc + k), static_cast<unsigned long long>(s));
 }
 
 // This workgroup's copy of the fixed-point accumulator (GlideEmit.fixed_copies).
