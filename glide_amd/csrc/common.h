@@ -422,22 +422,24 @@ __device__ __forceinline__ void sample_aux_b(const void* base, uint32_t planeb, 
     }
 }
 
+// sigma_w_ft (the support record's channel 0) at one corner offset.
+template <bool BF16>
+__device__ __forceinline__ float sample_sigma_ft_at(const void* base, uint32_t o) {
+    if constexpr (BF16) {
+        float v[2];
+        Rec8<true>::load2_at(base, o, v);
+        return v[0];
+    } else {
+        return plane_a0(base, o);
+    }
+}
+
 // Trilinear sample of sigma_w_ft alone (the free-troposphere probes).
 template <bool BF16>
 __device__ __forceinline__ float sample_sigma_ft(const void* base, const Corner3& c) {
     float acc = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        float v0;
-        if constexpr (BF16) {
-            float v[2];
-            Rec8<true>::load2_at(base, c.o[q], v);
-            v0 = v[0];
-        } else {
-            v0 = plane_a0(base, c.o[q]);
-        }
-        acc += c.w[q] * v0;
-    }
+    for (int q = 0; q < 8; ++q) acc += c.w[q] * sample_sigma_ft_at<BF16>(base, c.o[q]);
     return acc;
 }
 

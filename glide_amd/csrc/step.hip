@@ -251,19 +251,52 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         t.tw = fmaxf(t.tw, kTLMin);
 
         // sigma_w at the +-1 m probes.
-        auto sw_probe = [&](float zq) {
-            float sw = in_bl_sw(zq, sf);
-            if (sl_over && zq < 0.1f * sf.blh) sw = surface_layer_sw(zq, sf.ustar, sf.L);
-            if (zq > sf.blh || prescribed) {
-                const Corner3 c = corners3(ix, iy, level_index(lev, nz, a.ltop, zq), nx, ny, nz, WinRec<BF16>::kCellUnit);
-                sw = sample_sigma_ft<BF16>(x.aux, c);
-            }
-            return fmaxf(sw, kSigmaMin);
-        };
+        // Free-troposphere sigma_w at both probes from one gather: z_lo's
+        // stencil columns at its levels l, l+1, l+2 (clamped). z_hi, 2 m
+        // above, falls in level l or l+1 unless the levels are closer than
+        // that (then its own stencil is read). Same weights and order of
+        // sums as a trilinear sample of each probe.
         const float z_hi = z_eval + 1.0f;
         const float z_lo = fmaxf(z_eval - 1.0f, zu);
-        const float sw_hi = sw_probe(z_hi);
-        const float sw_lo = sw_probe(z_lo);
+        float ft_lo = 0.0f, ft_hi = 0.0f;
+        if (z_hi > sf.blh || prescribed) {
+            constexpr uint32_t ru = WinRec<BF16>::kCellUnit;
+            const float izl = level_index(lev, nz, a.ltop, z_lo);
+            const float izh = level_index(lev, nz, a.ltop, z_hi);
+            const Corner3 cl = corners3(ix, iy, izl, nx, ny, nz, ru);
+            const Corner3 ch = corners3(ix, iy, izh, nx, ny, nz, ru);
+            const int zl0 = static_cast<int>(floorf(izl)), zh0 = static_cast<int>(floorf(izh));
+            const uint32_t d1 = cl.o[4] - cl.o[0];  // level l to min(l + 1, nz - 1)
+            const uint32_t d2 = zl0 + 2 <= nz - 1 ? ru : 0u;
+            float sg[4][3];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                sg[q][0] = sample_sigma_ft_at<BF16>(x.aux, cl.o[q]);
+                sg[q][1] = sample_sigma_ft_at<BF16>(x.aux, cl.o[q] + d1);
+                sg[q][2] = sample_sigma_ft_at<BF16>(x.aux, cl.o[q] + d1 + d2);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ft_lo += cl.w[q] * sg[q][0];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ft_lo += cl.w[q + 4] * sg[q][1];
+            const int dh = zh0 - zl0;
+            if (dh == 0 || dh == 1) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) ft_hi += ch.w[q] * (dh == 0 ? sg[q][0] : sg[q][1]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) ft_hi += ch.w[q + 4] * (dh == 0 ? sg[q][1] : sg[q][2]);
+            } else {
+                ft_hi = sample_sigma_ft<BF16>(x.aux, ch);
+            }
+        }
+        auto sw_probe = [&](float zq, float ft) {
+            float sw = in_bl_sw(zq, sf);
+            if (sl_over && zq < 0.1f * sf.blh) sw = surface_layer_sw(zq, sf.ustar, sf.L);
+            if (zq > sf.blh || prescribed) sw = ft;
+            return fmaxf(sw, kSigmaMin);
+        };
+        const float sw_hi = sw_probe(z_hi, ft_hi);
+        const float sw_lo = sw_probe(z_lo, ft_lo);
         const float dsig2 = fdiv(sw_hi * sw_hi - sw_lo * sw_lo, fmaxf(z_hi - z_lo, 1.0f));
         const float sw_sq = fmaxf(t.sw * t.sw, kSigmaMin * kSigmaMin);
 

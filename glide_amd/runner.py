@@ -215,8 +215,11 @@ class BatchEngine:
         self.fixed_mode = accumulate == "fixed"
         # fixed_copies: copies of the fixed-point accumulator the step's
         # workgroups spread their atomics over (0: as many as fit in 512 MiB, at
-        # most 64; a cell crowded by a point release then takes 1/copies of the
-        # atomics per address); the cells are the sums over the copies
+        # most 16; a cell crowded by a point release then takes 1/copies of the
+        # atomics per address); the cells are the sums over the copies. The
+        # step sums each wavefront's lanes per distinct cell before its atomics,
+        # so 16 copies already keep a point release's cell from queueing, and
+        # every copy is read once to make the float footprint
         self.fixed_copies_req = int(fixed_copies)
         # bin_by_release: the re-binning keeps each release's particles
         # together (release-major keys), so a wavefront's particles share a
@@ -379,7 +382,7 @@ class BatchEngine:
             copies = self.fixed_copies_req
             if copies <= 0:
                 copies = 1
-                while copies < 64 and 2 * copies * 8 * self.acc_cells <= (512 << 20):
+                while copies < 16 and 2 * copies * 8 * self.acc_cells <= (512 << 20):
                     copies *= 2
             self.fixed_copies = copies
             self.fixed = torch.zeros(copies * self.acc_cells, dtype=torch.int64, device=d)
