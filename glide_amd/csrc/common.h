@@ -61,8 +61,11 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
 }
 
 // Uniform in (0, 1): ((x >> 9) + 0.5) * 2^-23, exact in float (24 bits).
+// Formed as the float 1 + (x >> 9) * 2^-23 (the 23 bits as a mantissa) plus
+// -(1 - 2^-24): the exact sum has 24 significant bits, so the one rounding
+// returns it unchanged.
 __device__ __forceinline__ float u01(uint32_t x) {
-    return (static_cast<float>(x >> 9) + 0.5f) * 1.1920928955078125e-07f;
+    return __uint_as_float((x >> 9) | 0x3f800000u) + (-0.99999994039535522f);
 }
 
 // Box-Muller pair from (ua, ub): (r cos(2 pi ub), r sin(2 pi ub)), on the

@@ -332,9 +332,10 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
             float un = ou_apply(up, cu, n0, 0.0f);
             float vn = ou_apply(vp, cv, n1, 0.0f);
             float wn = ou_apply(wp, cw, n2, drift * sdt);
-            wn = fminf(fmaxf(wn, -wmax), wmax);
-            un = fminf(fmaxf(un, -umax), umax);
-            vn = fminf(fmaxf(vn, -vmax), vmax);
+            // clamps to [-4 sigma, 4 sigma] (sigma > 0): one v_med3 each
+            wn = __builtin_amdgcn_fmed3f(wn, -wmax, wmax);
+            un = __builtin_amdgcn_fmed3f(un, -umax, umax);
+            vn = __builtin_amdgcn_fmed3f(vn, -vmax, vmax);
             z = z + (-wn) * sdt;
             const float ml = m_per_deg_lon(alat(lat));
             lon = lon + fdiv((-un) * sdt, ml);
