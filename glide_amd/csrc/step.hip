@@ -183,12 +183,22 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
             const uint32_t r0 = static_cast<uint32_t>(y0 * nx), r1 = static_cast<uint32_t>(y1 * nx);
             const uint32_t offs[4] = {(r0 + x0) * sb, (r0 + x1) * sb, (r1 + x0) * sb, (r1 + x1) * sb};
             const float ws[4] = {w00, w01, w10, w11};
+            float r[4][10];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                float r[10];
-                load_surf_at(x.surf, offs[q], r);
+            for (int q = 0; q < 4; ++q) load_surf_at(x.surf, offs[q], r[q]);
+            // torch_lerp's two forms, chosen once: alpha is the same for the
+            // whole launch
+            if (fabsf(al) < 0.5f) {
 #pragma unroll
-                for (int ch = 0; ch < 5; ++ch) acc[ch] += ws[q] * torch_lerp(r[ch], r[5 + ch], al);
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int ch = 0; ch < 5; ++ch) acc[ch] += ws[q] * (r[q][ch] + al * (r[q][5 + ch] - r[q][ch]));
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int ch = 0; ch < 5; ++ch)
+                        acc[ch] += ws[q] * (r[q][5 + ch] - (r[q][5 + ch] - r[q][ch]) * (1.0f - al));
             }
             sf.blh = fmaxf(acc[0], kBlhMin);
             sp = acc[1];
@@ -221,27 +231,31 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
             sample_aux_a<BF16>(x.aux, c, ax);
         }
 
-        // Column turbulence at z_eval (hanna.py:1500-1596).
-        SigTL t = in_bl(z_eval, sf);
-        if (sl_over && z_eval < 0.1f * sf.blh) {
-            const float z_sl = 0.1f * sf.blh;
-            const SigTL tc = in_bl(fmaxf(z_eval, z_sl), sf);
-            t.su = tc.su;
-            t.sv = tc.sv;
-            t.sw = surface_layer_sw(z_eval, sf.ustar, sf.L);
-            const float zt = fmaxf(z_eval, kZMin);
-            t.tu = kKarman * zt / fmaxf(t.su, kSigmaMin);
-            t.tv = kKarman * zt / fmaxf(t.sv, kSigmaMin);
-            t.tw = kKarman * zt / fmaxf(t.sw, kSigmaMin);
-        }
-        if (tl_floors) {
-            t.tu = fmaxf(t.tu, 10.0f);
-            t.tv = fmaxf(t.tv, 10.0f);
-            t.tw = fmaxf(t.tw, 30.0f);
-        }
+        // Column turbulence at z_eval (hanna.py:1500-1596): above the
+        // boundary layer (or prescribed) the support record's pair, else the
+        // Hanna profiles (a lane takes one branch, not both).
+        SigTL t;
         if (z_eval > sf.blh || prescribed) {
             t.su = t.sv = t.sw = ax[0];
             t.tu = t.tv = t.tw = ax[1];
+        } else {
+            t = in_bl(z_eval, sf);
+            if (sl_over && z_eval < 0.1f * sf.blh) {
+                const float z_sl = 0.1f * sf.blh;
+                const SigTL tc = in_bl(fmaxf(z_eval, z_sl), sf);
+                t.su = tc.su;
+                t.sv = tc.sv;
+                t.sw = surface_layer_sw(z_eval, sf.ustar, sf.L);
+                const float zt = fmaxf(z_eval, kZMin);
+                t.tu = kKarman * zt / fmaxf(t.su, kSigmaMin);
+                t.tv = kKarman * zt / fmaxf(t.sv, kSigmaMin);
+                t.tw = kKarman * zt / fmaxf(t.sw, kSigmaMin);
+            }
+            if (tl_floors) {
+                t.tu = fmaxf(t.tu, 10.0f);
+                t.tv = fmaxf(t.tv, 10.0f);
+                t.tw = fmaxf(t.tw, 30.0f);
+            }
         }
         t.su = fmaxf(t.su, kSigmaMin);
         t.sv = fmaxf(t.sv, kSigmaMin);
@@ -290,9 +304,13 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
             }
         }
         auto sw_probe = [&](float zq, float ft) {
-            float sw = in_bl_sw(zq, sf);
-            if (sl_over && zq < 0.1f * sf.blh) sw = surface_layer_sw(zq, sf.ustar, sf.L);
-            if (zq > sf.blh || prescribed) sw = ft;
+            float sw;
+            if (zq > sf.blh || prescribed) {
+                sw = ft;
+            } else {
+                sw = in_bl_sw(zq, sf);
+                if (sl_over && zq < 0.1f * sf.blh) sw = surface_layer_sw(zq, sf.ustar, sf.L);
+            }
             return fmaxf(sw, kSigmaMin);
         };
         const float sw_hi = sw_probe(z_hi, ft_hi);
