@@ -456,8 +456,15 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // FIXED: the fixed-point footprint (GLIDE_F_FIXED) as its own instantiation,
 // so neither variant keeps the other's pointers live (the kernel is short of
 // scalar registers; the spare ones spill to vector lanes).
+//
+// Occupancy: the bfloat16 kernel with float32 positions is held to 96 VGPRs,
+// 5 wavefronts per SIMD (a few registers spill, measured 3-4 us per launch
+// faster at 2^20 particles); the other instantiations keep the compiler's
+// allocation at 4 wavefronts per SIMD (held to 96, the float32 kernel spills
+// 48 bytes per lane and runs 6 us slower).
 template <bool BF16, int MODE, bool FIXED, bool CELL = false>
-__global__ __launch_bounds__(kStepBlockMax) void step_kernel(const StepArgs a) {
+__global__ __launch_bounds__(kStepBlockMax) __attribute__((amdgpu_waves_per_eu((BF16 && !CELL) ? 5 : 4)))
+void step_kernel(const StepArgs a) {
     __shared__ float lev[kLadder];
     fill_ladder(lev, a.levels, a.nz);
     __syncthreads();
