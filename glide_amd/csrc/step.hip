@@ -80,6 +80,32 @@ struct StepCtx {
     uint32_t noise_hi;  // Philox counter word 3 (a served slot's; 0 otherwise)
 };
 
+// The step's scalars in the precision the particle update uses, 32-bit
+// words only: a table row or served slot read from memory then holds a dozen
+// scalar registers instead of the row's doubles (the conversions are the ones
+// each use would make, so results do not change).
+struct StepScalars {
+    float dt, al, co, len, subc, zu, tm;
+    int32_t kmax, flags;
+    uint32_t k0, k1, step;
+};
+__device__ __forceinline__ StepScalars step_scalars(const GlideStepParams& p) {
+    StepScalars q;
+    q.dt = static_cast<float>(p.dt);
+    q.al = static_cast<float>(p.alpha);
+    q.co = static_cast<float>(p.cursor_off);
+    q.len = static_cast<float>(p.length_s);
+    q.subc = static_cast<float>(p.substep_c);
+    q.zu = static_cast<float>(p.z_ubl_m);
+    q.tm = static_cast<float>(p.meander_timescale_s);
+    q.kmax = p.max_substeps;
+    q.flags = p.flags;
+    q.k0 = static_cast<uint32_t>(p.seed);
+    q.k1 = static_cast<uint32_t>(p.seed >> 32);
+    q.step = static_cast<uint32_t>(p.step_index);
+    return q;
+}
+
 // Advances particle i; returns whether it was active. With GLIDE_F_FIXED the
 // footprint key and fixed-point value go to fkey/fval (left as the caller set
 // them, the dropped key and 0, for an inactive particle) instead of memory.
@@ -89,7 +115,7 @@ struct StepCtx {
 // formula needs it (map factors, Coriolis, footprint cell, domain test).
 template <bool BF16, bool FIXED, bool CELL>
 __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* lev, int64_t i,
-                                              const GlideStepParams& p, const StepCtx& x, uint64_t& fkey,
+                                              const StepScalars& p, const StepCtx& x, uint64_t& fkey,
                                               long long& fval) {
     // 32-bit byte offsets of this particle (the host checks the arrays fit)
     const uint32_t i1 = static_cast<uint32_t>(i);
@@ -98,7 +124,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     const GlideParticles& s = a.s;
     const bool emit = (p.flags & GLIDE_F_EMIT) != 0;
     constexpr bool fixed = FIXED;
-    const float co = static_cast<float>(p.cursor_off);
+    const float co = p.co;
 
     // The position loads go out with the mask's: one memory round trip before
     // the first gather instead of two (i < n, so they are in bounds).
@@ -109,7 +135,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     if (p.flags & GLIDE_F_MASK_INPUT) {
         active = ldo(s.active, i1) != 0;
     } else {
-        active = in_window(s, i1, co, static_cast<float>(p.length_s), p.flags);
+        active = in_window(s, i1, co, p.len, p.flags);
     }
     if (!active) {
         if (emit && !fixed) {
@@ -118,8 +144,8 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         }
         return false;
     }
-    const float dt = static_cast<float>(p.dt);
-    const float al = static_cast<float>(p.alpha);
+    const float dt = p.dt;
+    const float al = p.al;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     int cx = 0, cy = 0;
     float rlon = 0.0f, rlat = 0.0f;
@@ -153,7 +179,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     {
         float d0, d1, d2;
         wind_at(lon, lat, z, d0, d1, d2);
-        const float h = static_cast<float>(0.5 * p.dt);
+        const float h = 0.5f * dt;  // float(0.5 * dt) exactly: halving commutes with the rounding
         float e0, e1, e2;
         wind_at(lon - h * d0, lat - h * d1, z - h * d2, e0, e1, e2);
         lon = lon - dt * e0;
@@ -216,7 +242,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         const bool tl_floors = (p.flags & GLIDE_F_TL_FLOORS) != 0;
         const bool prescribed = (p.flags & GLIDE_F_PRESCRIBED) != 0;  // the support record at every height
         const bool sl_over = (p.flags & GLIDE_F_SL_OVERRIDE) != 0;
-        const float zu = static_cast<float>(p.z_ubl_m);
+        const float zu = p.zu;
         const float z_eval = fmaxf(z, zu);
         const float ix = gix(lon);
         const float iy = giy(lat);
@@ -324,17 +350,17 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
 
         // Substep schedule (hanna.py:1162-1186).
         // IEEE division: the ratio sits exactly on integers for the T_L floors.
-        const float ratio = dt / (static_cast<float>(p.substep_c) * fmaxf(t.tw, 1e-3f));
+        const float ratio = dt / (p.subc * fmaxf(t.tw, 1e-3f));
         // (a device table row's max_substeps below 1 still takes one substep)
-        int k = max(p.max_substeps, 1);
+        int k = max(p.kmax, 1);
         if (ratio < static_cast<float>(k)) k = max(static_cast<int>(ceilf(ratio)), 1);
         const float sdt = dt / static_cast<float>(k);
 
         const float su2 = t.su * t.su, sv2 = t.sv * t.sv, sw2 = t.sw * t.sw;
         const float umax = 4.0f * t.su, vmax = 4.0f * t.sv, wmax = 4.0f * t.sw;
         const uint32_t uid = s.uid ? ldo(s.uid, i4) : static_cast<uint32_t>(i);
-        const uint32_t k0 = static_cast<uint32_t>(p.seed), k1 = static_cast<uint32_t>(p.seed >> 32);
-        const uint32_t step = static_cast<uint32_t>(p.step_index);
+        const uint32_t k0 = p.k0, k1 = p.k1;
+        const uint32_t step = p.step;
         // sub_dt, sigma and T_L hold for every substep (hanna.py:1300-1320), so
         // the OU decay and noise amplitude are formed once per particle.
         const OuCoef cu = ou_coef(t.tu, su2, sdt), cv = ou_coef(t.tv, sv2, sdt), cw = ou_coef(t.tw, sw2, sdt);
@@ -375,7 +401,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
             const U4 r = philox4x32_10(uid, step, kStreamMeander, x.noise_hi, k0, k1);
             float n0, n1;
             box_muller(u01(r.x), u01(r.y), n0, n1);
-            const float tm = static_cast<float>(p.meander_timescale_s);
+            const float tm = p.tm;
             const OuCoef mu = ou_coef(tm, ax[4] * ax[4], dt);
             const OuCoef mv{mu.a, fsqrt(fmaxf(ax[5] * ax[5] * (1.0f - mu.a * mu.a), 0.0f))};  // same decay
             um = ou_apply(um, mu, n0, 0.0f);
@@ -468,7 +494,7 @@ void step_kernel(const StepArgs a) {
     __shared__ float lev[kLadder];
     fill_ladder(lev, a.levels, a.nz);
     __syncthreads();
-    GlideStepParams p = a.p;
+    StepScalars p = step_scalars(a.p);
     StepCtx x{a.wind, a.aux, a.surf, a.e.escaped, -1, a.p.noise_hi};
     unsigned long long* n_active = a.n_active;
     const int64_t i =
@@ -481,9 +507,10 @@ void step_kernel(const StepArgs a) {
         const int r = __builtin_amdgcn_readfirstlane(*a.row);
         in_table = r >= 0 && r < a.n_rows;
         if (in_table) {
-            p = load_row(a.table, r);
+            const GlideStepParams row = load_row(a.table, r);
+            p = step_scalars(row);
             p.flags = a.p.flags;  // the template's, which the host checked the pointers for
-            x.noise_hi = p.noise_hi;
+            x.noise_hi = row.noise_hi;
             x.escaped += r;
             if (n_active != nullptr) n_active += static_cast<int64_t>(r) * active_copies(a.e);
         } else {
@@ -505,10 +532,10 @@ void step_kernel(const StepArgs a) {
             const int32_t hb = __builtin_amdgcn_readfirstlane(sl->bracket) - a.ring.first_hour;
             stepping = stepping && hb >= 0 && hb < a.ring.n_hours;  // never outside the ring
             if (stepping) {
-                p.dt = sl->dt;
-                p.alpha = sl->alpha;
-                p.cursor_off = sl->cursor_off;
-                p.step_index = __builtin_amdgcn_readfirstlane(sl->noise_step);
+                p.dt = static_cast<float>(sl->dt);
+                p.al = static_cast<float>(sl->alpha);
+                p.co = static_cast<float>(sl->cursor_off);
+                p.step = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(sl->noise_step));
                 const int64_t h = hb;
                 x.wind = static_cast<const char*>(a.ring.wind) + h * a.ring.wind_stride;
                 x.aux = static_cast<const char*>(a.ring.aux) + h * a.ring.aux_stride;
