@@ -289,6 +289,44 @@ __device__ __forceinline__ float level_index(const float* __restrict__ lev, int 
     return static_cast<float>(up - 1) + fr;
 }
 
+// level_index for a height near one already looked up: u is that lookup's
+// upper level (1 <= u <= nz - 1) and becomes this one's. One step up or down
+// from u is tried first (two LDS reads), verified to bracket z as the count
+// of levels below z would (the clamped end intervals included), and the
+// binary search runs only when it does not. The result equals level_index's
+// for non-decreasing levels.
+__device__ __forceinline__ float level_index_near(const float* __restrict__ lev, int nz, int top, float z, int& u) {
+    const int u0 = u;
+    int up = u0 + (lev[u0] < z ? 1 : 0) - (lev[u0 - 1] >= z ? 1 : 0);
+    up = min(max(up, 1), nz - 1);
+    float lo = lev[up - 1];
+    float hi = lev[up];
+    if (!((up == 1 || lo < z) && (up == nz - 1 || z <= hi))) {
+        int cnt = 0;
+        for (int step = top; step > 0; step >>= 1) cnt = (lev[cnt + step - 1] < z) ? cnt + step : cnt;
+        up = min(max(cnt, 1), nz - 1);
+        lo = lev[up - 1];
+        hi = lev[up];
+    }
+    u = up;
+    float fr = (z - lo) * __builtin_amdgcn_rcpf(fmaxf(hi - lo, 1e-6f));
+    fr = fminf(fmaxf(fr, 0.0f), 1.0f);
+    return static_cast<float>(up - 1) + fr;
+}
+
+// level_index that also returns the upper level it found (for level_index_near).
+__device__ __forceinline__ float level_index_up(const float* __restrict__ lev, int nz, int top, float z, int& u) {
+    int cnt = 0;
+    for (int step = top; step > 0; step >>= 1) cnt = (lev[cnt + step - 1] < z) ? cnt + step : cnt;
+    const int up = min(max(cnt, 1), nz - 1);
+    u = up;
+    const float lo = lev[up - 1];
+    const float hi = lev[up];
+    float fr = (z - lo) * __builtin_amdgcn_rcpf(fmaxf(hi - lo, 1e-6f));
+    fr = fminf(fmaxf(fr, 0.0f), 1.0f);
+    return static_cast<float>(up - 1) + fr;
+}
+
 // Byte offsets and weights of the 8 trilinear corners (order x fastest, then
 // y, then z: 000 001 010 011 100 101 110 111).
 struct Corner3 {

@@ -162,10 +162,16 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
     auto alat = [&](float la) { return CELL ? rlat + la : la; };
 
     // ---- RK2 advection (backward) ----
+    // Level lookups after the first start from the last one's interval: the
+    // heights looked up in a step are close together.
+    int lvl = 1;
+    bool first_lookup = true;
     auto wind_at = [&](float lo, float la, float zz, float& dlo, float& dla, float& dzz) {
         const float ix = gix(lo);
         const float iy = giy(la);
-        const float iz = level_index(lev, nz, a.ltop, zz);
+        const float iz = first_lookup ? level_index_up(lev, nz, a.ltop, zz, lvl)
+                                      : level_index_near(lev, nz, a.ltop, zz, lvl);
+        first_lookup = false;
         const Corner3 c = corners3(ix, iy, iz, nx, ny, nz, WinRec<BF16>::kCellUnit);
         float v[6];
         sample_wind<BF16>(x.wind, a.planeb, c, v);
@@ -251,7 +257,7 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         // meander pair; float32 samples its plane after the substeps, when
         // it is needed, so its loads are not in flight with these)
         float ax[6];
-        const float iz_eval = level_index(lev, nz, a.ltop, z_eval);
+        const float iz_eval = level_index_near(lev, nz, a.ltop, z_eval, lvl);
         {
             const Corner3 c = corners3(ix, iy, iz_eval, nx, ny, nz, WinRec<BF16>::kCellUnit);
             sample_aux_a<BF16>(x.aux, c, ax);
@@ -301,8 +307,9 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         float ft_lo = 0.0f, ft_hi = 0.0f;
         if (z_hi > sf.blh || prescribed) {
             constexpr uint32_t ru = WinRec<BF16>::kCellUnit;
-            const float izl = level_index(lev, nz, a.ltop, z_lo);
-            const float izh = level_index(lev, nz, a.ltop, z_hi);
+            int lp = lvl;
+            const float izl = level_index_near(lev, nz, a.ltop, z_lo, lp);
+            const float izh = level_index_near(lev, nz, a.ltop, z_hi, lp);
             const Corner3 cl = corners3(ix, iy, izl, nx, ny, nz, ru);
             const Corner3 ch = corners3(ix, iy, izh, nx, ny, nz, ru);
             const int zl0 = static_cast<int>(floorf(izl)), zh0 = static_cast<int>(floorf(izh));
