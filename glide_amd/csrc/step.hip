@@ -113,7 +113,15 @@ __device__ __forceinline__ StepScalars step_scalars(const GlideStepParams& p) {
 // offsets from the particle's cell point (rlon, rlat); every displacement
 // moves them, and the absolute coordinate rlon + lon is formed only where a
 // formula needs it (map factors, Coriolis, footprint cell, domain test).
-template <bool BF16, bool FIXED, bool CELL>
+//
+// RELOAD: the state stores and the footprint epilogue read their pointers from
+// the kernel arguments afresh (through an opaque copy of the kernarg segment
+// address), so those pointers are not held in scalar registers through the
+// whole particle update. Without it the table-row, served-slot, cell and
+// bfloat16 instantiations run out of scalar registers and spill about 200 of
+// them to vector lanes; the float32 plain launch, which holds them without
+// spilling, is 2-4 us faster without the reloads.
+template <bool BF16, bool FIXED, bool CELL, bool RELOAD>
 __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* lev, int64_t i,
                                               const StepScalars& p, const StepCtx& x, uint64_t& fkey,
                                               long long& fval) {
@@ -431,36 +439,52 @@ __device__ __forceinline__ bool step_particle(const StepArgs& a, const float* le
         rlat = __builtin_fmaf(static_cast<float>(cy), a.cdlat, a.lat0);
         sto(s.cell, i4, (cy << 16) | cx);
     }
-    sto(s.lon, ip, lon);
-    sto(s.lat, ip, lat);
-    sto(s.z, ip, z);
-    sto(s.up, i4, up);
-    sto(s.vp, i4, vp);
-    sto(s.wp, i4, wp);
-    if (meander) {
-        sto(s.um, i4, um);
-        sto(s.vm, i4, vm);
+    // The stores and the footprint epilogue, with their pointers from B: the
+    // kernel arguments as passed (a) or read afresh (RELOAD).
+#define GLIDE_STEP_FINISH(B)                                                                    \
+    do {                                                                                        \
+        sto(B.s.lon, ip, lon);                                                                  \
+        sto(B.s.lat, ip, lat);                                                                  \
+        sto(B.s.z, ip, z);                                                                      \
+        sto(B.s.up, i4, up);                                                                    \
+        sto(B.s.vp, i4, vp);                                                                    \
+        sto(B.s.wp, i4, wp);                                                                    \
+        if (meander) {                                                                          \
+            sto(B.s.um, i4, um);                                                                \
+            sto(B.s.vm, i4, vm);                                                                \
+        }                                                                                       \
+        if (emit) {                                                                             \
+            if constexpr (CELL) { /* the absolute coordinates of the footprint and domain */    \
+                lon = rlon + lon;                                                               \
+                lat = rlat + lat;                                                               \
+            }                                                                                   \
+            const int64_t tb = time_bin(ldo(B.s.we_off, i4), co, a.fp.n_t);                     \
+            const int64_t rb = x.slot >= 0 ? x.slot : (B.s.rel_idx ? ldo(B.s.rel_idx, i4) : 0); \
+            const float val = ldo(B.s.weight, i4) * dt;                                         \
+            if (fixed) {                                                                        \
+                fkey = fixed_cell(a.fp, lon, lat, z, tb, rb);                                   \
+                fval = fkey != kNoCell ? to_fixed(val, B.e.fixed_exp[rb]) : 0;                  \
+            } else {                                                                            \
+                sto(B.e.keys, i4, footprint_key(a.fp, true, lon, lat, z, tb, rb));              \
+                sto(B.e.vals, i4, val);                                                         \
+            }                                                                                   \
+            const bool esc = !within_domain(a.fp, lon, lat, z);                                 \
+            if (esc) sto(B.s.alive, i1, static_cast<uint8_t>(0));                               \
+            count_flags(esc, x.escaped);                                                        \
+        }                                                                                       \
+    } while (0)
+    if constexpr (RELOAD) {
+        // the kernel's only argument sits at offset 0 of the kernarg segment;
+        // the empty asm hides the address, so the loads happen here
+        using KArgs = const __attribute__((address_space(4))) StepArgs;
+        KArgs* kp = (KArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
+        asm volatile("" : "+s"(kp));
+        KArgs& b = *kp;
+        GLIDE_STEP_FINISH(b);
+    } else {
+        GLIDE_STEP_FINISH(a);
     }
-
-    if (emit) {
-        if constexpr (CELL) {  // the absolute coordinates the footprint and domain formulas take
-            lon = rlon + lon;
-            lat = rlat + lat;
-        }
-        const int64_t tb = time_bin(ldo(s.we_off, i4), co, a.fp.n_t);
-        const int64_t rb = x.slot >= 0 ? x.slot : (s.rel_idx ? ldo(s.rel_idx, i4) : 0);
-        const float val = ldo(s.weight, i4) * dt;
-        if (fixed) {
-            fkey = fixed_cell(a.fp, lon, lat, z, tb, rb);
-            fval = fkey != kNoCell ? to_fixed(val, a.e.fixed_exp[rb]) : 0;
-        } else {
-            sto(a.e.keys, i4, footprint_key(a.fp, true, lon, lat, z, tb, rb));
-            sto(a.e.vals, i4, val);
-        }
-        const bool esc = !within_domain(a.fp, lon, lat, z);
-        if (esc) sto(s.alive, i1, static_cast<uint8_t>(0));
-        count_flags(esc, x.escaped);
-    }
+#undef GLIDE_STEP_FINISH
     return true;
 }
 
@@ -490,13 +514,12 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // so neither variant keeps the other's pointers live (the kernel is short of
 // scalar registers; the spare ones spill to vector lanes).
 //
-// Occupancy: the bfloat16 kernel with float32 positions is held to 96 VGPRs,
-// 5 wavefronts per SIMD (a few registers spill, measured 3-4 us per launch
-// faster at 2^20 particles); the other instantiations keep the compiler's
-// allocation at 4 wavefronts per SIMD (held to 96, the float32 kernel spills
-// 48 bytes per lane and runs 6 us slower).
+// Occupancy: the bfloat16 kernels fit 96 VGPRs, 5 wavefronts per SIMD
+// (measured 1-8 us per launch faster at 2^20 particles than at 4); the
+// float32 ones keep 4 (held to 96 they spill 12-48 bytes per lane and run no
+// faster).
 template <bool BF16, int MODE, bool FIXED, bool CELL = false>
-__global__ __launch_bounds__(kStepBlockMax) __attribute__((amdgpu_waves_per_eu((BF16 && !CELL) ? 5 : 4)))
+__global__ __launch_bounds__(kStepBlockMax) __attribute__((amdgpu_waves_per_eu(BF16 ? 5 : 4)))
 void step_kernel(const StepArgs a) {
     __shared__ float lev[kLadder];
     fill_ladder(lev, a.levels, a.nz);
@@ -568,7 +591,8 @@ void step_kernel(const StepArgs a) {
     bool act = false;
     uint64_t fkey = kNoCell;
     long long fval = 0;
-    if (run) act = step_particle<BF16, FIXED, CELL>(a, lev, i, p, x, fkey, fval);
+    constexpr bool kReload = BF16 || CELL || MODE != kModePlain;
+    if (run) act = step_particle<BF16, FIXED, CELL, kReload>(a, lev, i, p, x, fkey, fval);
     if (FIXED && (p.flags & GLIDE_F_EMIT))
         wave_fixed_add(fkey, fval, fixed_copy(a.e, a.fp.acc_cells));  // every lane is here
     if constexpr (MODE == kModeSlots) {
