@@ -67,6 +67,9 @@ N_SIMD, CLOCK_HZ, VALU_CYCLES = 1024, 2.4e9, 4
 # accumulation replaces the emission with one 8 B integer atomic per run of
 # equal cells in a wavefront, at most 8 B per particle-step.)
 BYTES_PER_PARTICLE_STEP = 90
+# Positions as an int32 grid point plus float32 lon/lat offsets and z (16 B
+# instead of 12 B), read and written: 8 B more per particle-step.
+CELL_EXTRA_BYTES = 8
 
 
 def parse():
@@ -472,6 +475,10 @@ def batched_leg(args, name: str, stream, sptr: int, shared: dict, evidence: dict
 
     cfg = workload_config(name)
     bf16 = name.endswith("-bf16")
+    # the bfloat16 leg is the precision layout: positions as an int32 grid
+    # point plus float32 offsets as well (4 B more state read and written)
+    cell = bf16
+    bpp = BYTES_PER_PARTICLE_STEP + (CELL_EXTRA_BYTES if cell else 0)
     met_dtype = L.GLIDE_MET_BF16 if bf16 else L.GLIDE_MET_F32
     if "met" not in shared:
         shared["met"] = SyntheticMet(cfg["spec"])
@@ -498,7 +505,7 @@ def batched_leg(args, name: str, stream, sptr: int, shared: dict, evidence: dict
     else:
         tuned = tune(cfg, met, candidates=BATCH_CANDIDATES, use_cache=False, stream=stream, device_met=dm, **kw)
         shared["tuned"] = tuned["best"]
-    eng = BatchEngine(cfg, device="cuda", met=met, device_met=dm, **kw, **tuned["best"])
+    eng = BatchEngine(cfg, device="cuda", met=met, device_met=dm, cell_positions=cell, **kw, **tuned["best"])
     n_sched = len(eng.sched)
     K = n_sched if args.batched_steps <= 0 else min(args.batched_steps, n_sched)
     eng.run(min(20, n_sched))  # warm-up, then back to the release state
@@ -522,10 +529,12 @@ def batched_leg(args, name: str, stream, sptr: int, shared: dict, evidence: dict
         t_ms = ms[:t_act.size].astype(np.float64)
     kp = {"launches": int(len(t_ms)), "avg_us": float(t_ms.mean() * 1e3) if len(t_ms) else float("nan"),
           "particles_per_launch": float(t_act.mean()) if len(t_act) else 0.0,
-          "bytes_per_s": (BYTES_PER_PARTICLE_STEP * float(t_act.sum()) / (float(t_ms.sum()) * 1e-3)
+          "bytes_per_s": (bpp * float(t_act.sum()) / (float(t_ms.sum()) * 1e-3)
                           if len(t_ms) else float("nan"))}
-    roofline = _roofline(kp, f"glide::step_kernel<{str(bf16).lower()}, 0, true, {str(bf16).lower()}>", {})
-    roofline["whole_step_achieved"] = round(BYTES_PER_PARTICLE_STEP * psteps / elapsed / 1e9, 2)
+    roofline = _roofline(kp, f"glide::step_kernel<{str(bf16).lower()}, 0, true, {str(cell).lower()}>", {})
+    roofline["bytes_per_particle_step"] = bpp
+    roofline["bytes_per_launch"] = round(bpp * kp["particles_per_launch"])
+    roofline["whole_step_achieved"] = round(bpp * psteps / elapsed / 1e9, 2)
     g = cfg["grid"]
     tot = eng.release_total.cpu().numpy()
     if K == n_sched:
@@ -553,7 +562,7 @@ def batched_leg(args, name: str, stream, sptr: int, shared: dict, evidence: dict
         "device_gb_allocated": round(torch.cuda.max_memory_allocated() / 1e9, 1),
         "autotune": {"chosen": tuned["best"], "trials": tuned["trials"], **({"from": tuned["from"]}
                                                                             if "from" in tuned else {})},
-        "positions": "float32 lon/lat/z",
+        "positions": "int32 grid point + float32 lon/lat offsets" if cell else "float32 lon/lat/z",
         "roofline": roofline,
     }
     eng.close()
