@@ -154,8 +154,7 @@ def _timed_steps(eng, K: int, sptr: int) -> float:
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(K):
-        eng.run_step(sptr)
+    eng.advance(K, sptr)
     eng.join(sptr)
     torch.cuda.synchronize()
     return time.perf_counter() - t0

@@ -258,7 +258,7 @@ class BatchEngine:
         self.extra_flags = (L.F_RELEASE_TERMINUS if self.release_terminus else 0) | (
             L.F_FIXED if self.fixed_mode else 0)
         self.use_graph = bool(use_graph)
-        self._graphs: dict[tuple[int, int], int] = {}
+        self._graphs: dict[tuple[int, int, int], int] = {}
         self.graph_launches = 0
         self.device = device
         self.h = hanna or HannaOptions()
@@ -691,13 +691,14 @@ class BatchEngine:
                 main.wait_event(self.ev_acc[b])
                 self.acc_pending[b] = False
 
-    def _capture(self, stream: int) -> int:
-        """Capture the plain step body (table-driven step, which also advances
-        the device cursor, and the footprint accumulation) for the current
-        particle buffer. (A body whose accumulation of the previous step runs
-        on a parallel branch beside the step kernel replayed more than twice
-        as slowly as this serial one on MI355X, 0.35 against 0.15 ms per step:
-        the step kernel fills every CU, so the branches hardly overlap.)"""
+    def _capture(self, stream: int, steps: int = 1) -> int:
+        """Capture ``steps`` plain step bodies back to back (table-driven step,
+        which also advances the device cursor, and the footprint accumulation)
+        for the current particle buffer. (A body whose accumulation of the
+        previous step runs on a parallel branch beside the step kernel
+        replayed more than twice as slowly as this serial one on MI355X, 0.35
+        against 0.15 ms per step: the step kernel fills every CU, so the
+        branches hardly overlap.)"""
 
         lib = L.lib()
         if stream == 0:
@@ -706,13 +707,14 @@ class BatchEngine:
                          active=self._active.data_ptr(), active_copies=ACTIVE_COPIES, **self._fixed_emit())
         L.check(lib.glide_graph_begin(stream))
         try:
-            L.check(lib.glide_step_table(C.byref(self.step_tmpl), self.step_table.data_ptr(), len(self.sched),
-                                         self.row.data_ptr(),
-                                         C.byref(self.parts), C.byref(self.met.grid), C.byref(self.met.window),
-                                         C.byref(self.fgs[0]), C.byref(em), stream))
-            self._accumulate(stream)
-            if self.diagnostics:
-                self._position_means(stream, row=True)
+            for _ in range(steps):
+                L.check(lib.glide_step_table(C.byref(self.step_tmpl), self.step_table.data_ptr(), len(self.sched),
+                                             self.row.data_ptr(),
+                                             C.byref(self.parts), C.byref(self.met.grid), C.byref(self.met.window),
+                                             C.byref(self.fgs[0]), C.byref(em), stream))
+                self._accumulate(stream)
+                if self.diagnostics:
+                    self._position_means(stream, row=True)
         finally:
             h = C.c_void_p()
             rc = lib.glide_graph_end(stream, C.byref(h))
@@ -896,13 +898,7 @@ class BatchEngine:
         ev = self.step_events.get(self.step_i) if self.step_events is not None else None
         if self.use_graph and not convect and ev is None:
             self._wait_acc(stream)  # the captured body emits into buffer 0 and accumulates in line
-            key = (self.cur, self.n_live)
-            g = self._graphs.get(key)
-            if g is None:
-                for k in [k for k in self._graphs if k[1] != self.n_live]:  # a shrunken live set
-                    L.lib().glide_graph_destroy(self._graphs.pop(k))
-                g = self._graphs[key] = self._capture(stream)
-            L.check(lib.glide_graph_launch(g, stream))
+            L.check(lib.glide_graph_launch(self._graph(1, stream), stream))
             self.graph_launches += 1
             self.last_bracket = st.bracket_hour
             self.step_i += 1
@@ -972,10 +968,65 @@ class BatchEngine:
         self.step_events = {self.step_i + i: e for i, e in evs.items()}
         return evs
 
-    def run(self, n_steps: int | None = None) -> None:
-        end = len(self.sched) if n_steps is None else min(len(self.sched), self.step_i + n_steps)
+    # steps per captured graph: a run of replayable steps is launched as
+    # graphs of these sizes, largest first (one launch per graph, not per step)
+    GRAPH_STEPS = (16, 8, 4, 2, 1)
+
+    def _graph(self, steps: int, stream: int) -> int:
+        """The captured body of ``steps`` steps for the current buffer and live count."""
+
+        key = (self.cur, self.n_live, steps)
+        g = self._graphs.get(key)
+        if g is None:
+            for k in [k for k in self._graphs if k[1] != self.n_live]:  # a shrunken live set
+                L.lib().glide_graph_destroy(self._graphs.pop(k))
+            g = self._graphs[key] = self._capture(stream, steps)
+        return g
+
+    def _replayable_run(self, end: int) -> int:
+        """Steps from step_i (before ``end``) that replay the captured body
+        back to back with nothing between them: the bracket of the step
+        before (no window build, compaction or convection), no re-binning,
+        no timing events."""
+
+        k = 0
+        for j in range(self.step_i, end):
+            if self.sched[j].bracket_hour != self.last_bracket:
+                break
+            if self.reorder_every and j > 0 and j % self.reorder_every == 0:
+                break
+            if self.step_events is not None and j in self.step_events:
+                break
+            k += 1
+        return k
+
+    def advance(self, n_steps: int, stream: int | None = None) -> None:
+        """Run the next ``n_steps`` steps of the schedule (fewer at its end).
+        With ``use_graph``, runs of replayable steps go to the device as
+        multi-step graphs; the result is the same as stepping one by one."""
+
+        if stream is None:
+            stream = L.stream_ptr()
+        end = min(len(self.sched), self.step_i + n_steps)
+        lib = L.lib()
         while self.step_i < end:
-            self.run_step()
+            k = self._replayable_run(end) if self.use_graph else 0
+            if k < 2:
+                self.run_step(stream)
+                continue
+            self.met.build_window(self.last_bracket, stream)  # (built already: a shared DeviceMet is re-checked)
+            if self.fixed_mode:
+                self._fixed_stale = True  # these steps add to the fixed-point sums
+            self._wait_acc(stream)
+            for size in self.GRAPH_STEPS:
+                while k >= size:
+                    L.check(lib.glide_graph_launch(self._graph(size, stream), stream))
+                    self.graph_launches += 1
+                    self.step_i += size
+                    k -= size
+
+    def run(self, n_steps: int | None = None) -> None:
+        self.advance(len(self.sched) if n_steps is None else n_steps)
         self.join()
 
     def active_count(self, st) -> int:
