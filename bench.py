@@ -108,7 +108,7 @@ def parse():
     ap.add_argument("--batched-block", type=int, default=0, help="batched legs: step workgroup size (no tuning)")
     ap.add_argument("--batched-reorder", type=int, default=-1, help="batched legs: re-binning interval (no tuning)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
-    ap.add_argument("--serve-seconds", type=float, default=120.0,
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="seconds of Poisson request arrivals (20 requests/s) in the serving leg (600 = the "
                          "full 10-minute stream)")
     ap.add_argument("--serve-out", default="", help="keep the serving leg's Zarr stores in this directory")
