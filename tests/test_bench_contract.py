@@ -33,6 +33,6 @@ def test_documented_defaults():
         sys.argv = old
     assert a.gpus == 1 and a.steps == 1440 and a.accumulate == "fixed"
     assert a.reorder_every == 32 and a.kernel_steps == 288 and a.step_block == 128
-    assert a.serve_seconds == 120.0 and a.check_releases == 1 and a.torch_steps == 60
+    assert a.serve_seconds == 60.0 and a.check_releases == 1 and a.torch_steps == 60
     assert bench.BYTES_PER_PARTICLE_STEP == 90
     assert bench.BATCHED == "batched-gpu-2688r" and bench.CPU_WORKLOAD == "plain-cpu-10k-3h"
