@@ -4,7 +4,6 @@
 
 namespace glide {
 
-constexpr int kColBlock = 256;
 constexpr double kMPerDegLatD = 110540.0;
 
 // ---------------------------------------------------------------------------
@@ -61,7 +60,7 @@ struct ResampleArgs {
     const double* p_hpa;
     int32_t iu, iv, iw;
     float* dst3;
-    double* p_part;  // [n_blocks][za] per-block sums of the regridded pressure
+    double* p_part;  // [n_groups][za] per-wavefront sums of the regridded pressure (64 columns each)
     const int32_t* flipped;
 };
 
@@ -75,34 +74,63 @@ __device__ __forceinline__ float grad1(const float* __restrict__ f, int64_t stri
     return static_cast<float>(static_cast<double>(f[stride] - f[-stride]) / (2.0 * d));
 }
 
-// One thread per (column, target level): blockIdx.y is the level, so each
-// block's pressure partial covers the same 256 columns at one level.
-__global__ __launch_bounds__(kColBlock) void resample_kernel(const ResampleArgs a) {
+// Target levels per thread of the resample, and 64-column wavefronts per block.
+constexpr int kResLevels = 4;
+constexpr int kResWaves = 4;
+
+// One thread per (column, kResLevels consecutive target levels): the column's
+// source heights are read once for all of its levels, in batches of
+// independent loads, and the terrain slopes once per thread. A wavefront
+// takes 64 consecutive columns, so every load and store is coalesced along x;
+// each wavefront leaves one float64 pressure partial per target level.
+__global__ __launch_bounds__(64 * kResWaves) void resample_kernel(const ResampleArgs a) {
 #pragma clang fp contract(off)
-    __shared__ double psum[kColBlock];
     const int64_t ncol = static_cast<int64_t>(a.ny) * a.nx;
-    const int64_t col = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int64_t grp = static_cast<int64_t>(blockIdx.x) * kResWaves + (threadIdx.x >> 6);
+    const int64_t col = grp * 64 + lane;
     const bool live = col < ncol;
     const int zp = a.zp;
+    const int j0 = static_cast<int>(blockIdx.y) * kResLevels;
     const bool flip = *a.flipped != 0;
-    // ascending index -> source index
     auto src_idx = [&](int asc) { return flip ? (zp - 1 - asc) : asc; };
     auto hasc = [&](int asc) { return static_cast<double>(a.h[static_cast<int64_t>(src_idx(asc)) * ncol + col]); };
 
-    double pval = 0.0;
+    // idx = searchsorted(h, tg, side="left") on the ascending heights, i.e. the
+    // number of levels below tg, and k0 the number below ground; counting is
+    // order-free, so the source order serves. For a float h and a double tg,
+    // h < tg exactly when h < tg rounded up to a float.
+    double tg[kResLevels];
+    float tgu[kResLevels];
+#pragma unroll
+    for (int g = 0; g < kResLevels; ++g) {
+        tg[g] = a.agl[min(j0 + g, a.za - 1)];
+        tgu[g] = __double2float_ru(tg[g]);
+    }
+    double pval[kResLevels];
+#pragma unroll
+    for (int g = 0; g < kResLevels; ++g) pval[g] = 0.0;
     if (live) {
-        const double tg = a.agl[j];
-        // k0 = number of sub-surface levels; idx = searchsorted(h, tg,
-        // side="left"), which on the ascending heights is the number of levels
-        // below tg: one pass of independent loads (a scan that stops at the
-        // first level >= tg makes every load wait for the one before)
-        int k0 = 0, idx = 0;
-        for (int k = 0; k < zp; ++k) {
-            const double hk = hasc(k);
-            k0 += (hk < 0.0) ? 1 : 0;
-            idx += (hk < tg) ? 1 : 0;
+        int k0 = 0;
+        int idx[kResLevels];
+#pragma unroll
+        for (int g = 0; g < kResLevels; ++g) idx[g] = 0;
+        const float* hp = a.h + col;
+        auto count = [&](float hk) {
+            k0 += (hk < 0.0f) ? 1 : 0;
+#pragma unroll
+            for (int g = 0; g < kResLevels; ++g) idx[g] += (hk < tgu[g]) ? 1 : 0;
+        };
+        int k = 0;
+        for (; k + 8 <= zp; k += 8) {
+            float hv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) hv[u] = hp[static_cast<int64_t>(k + u) * ncol];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) count(hv[u]);
         }
+        for (; k < zp; ++k) count(hp[static_cast<int64_t>(k) * ncol]);
+
         const int y = static_cast<int>(col / a.nx);
         const int x = static_cast<int>(col % a.nx);
         const float* tc = a.terrain + col;
@@ -112,51 +140,62 @@ __global__ __launch_bounds__(kColBlock) void resample_kernel(const ResampleArgs 
         const float dhdy = dlat_g / static_cast<float>(kMPerDegLatD);  // float32, as numpy keeps it
         const double ztop = a.agl[a.za - 1];
         const double ztop_c = ztop > 1.0 ? ztop : 1.0;
-        int lower = idx - 1;
-        if (lower < 0) lower = 0;
-        if (lower < k0) lower = k0;
-        if (lower > zp - 2) lower = zp - 2;
-        const double hlo = hasc(lower), hhi = hasc(lower + 1);
-        double den = hhi - hlo;
-        if (den < 1e-6) den = 1e-6;
-        double fr = (tg - hlo) / den;
-        fr = fr < 0.0 ? 0.0 : (fr > 1.0 ? 1.0 : fr);
-        const int slo = src_idx(lower), shi = src_idx(lower + 1);
-        const float wf = static_cast<float>(fr);
-        float uu = 0.0f, vv = 0.0f, ww = 0.0f;
-        for (int c = 0; c < a.c3; ++c) {
-            const float* fc = a.src3 + static_cast<int64_t>(c) * zp * ncol;
-            const float f0 = fc[static_cast<int64_t>(slo) * ncol + col];
-            const float f1 = fc[static_cast<int64_t>(shi) * ncol + col];
-            const float r = (fabsf(wf) < 0.5f) ? f0 + wf * (f1 - f0) : f1 - (f1 - f0) * (1.0f - wf);
-            if (c == a.iu) uu = r;
-            if (c == a.iv) vv = r;
-            if (c == a.iw) ww = r;
-            if (c != a.iw) a.dst3[(static_cast<int64_t>(c) * a.za + j) * ncol + col] = r;
+#pragma unroll
+        for (int g = 0; g < kResLevels; ++g) {
+            const int j = j0 + g;
+            if (j >= a.za) break;
+            int lower = idx[g] - 1;
+            if (lower < 0) lower = 0;
+            if (lower < k0) lower = k0;
+            if (lower > zp - 2) lower = zp - 2;
+            const double hlo = hasc(lower), hhi = hasc(lower + 1);
+            double den = hhi - hlo;
+            if (den < 1e-6) den = 1e-6;
+            double fr = (tg[g] - hlo) / den;
+            fr = fr < 0.0 ? 0.0 : (fr > 1.0 ? 1.0 : fr);
+            const int slo = src_idx(lower), shi = src_idx(lower + 1);
+            const float wf = static_cast<float>(fr);
+            float uu = 0.0f, vv = 0.0f, ww = 0.0f;
+            for (int c = 0; c < a.c3; ++c) {
+                const float* fc = a.src3 + static_cast<int64_t>(c) * zp * ncol;
+                const float f0 = fc[static_cast<int64_t>(slo) * ncol + col];
+                const float f1 = fc[static_cast<int64_t>(shi) * ncol + col];
+                const float r = (fabsf(wf) < 0.5f) ? f0 + wf * (f1 - f0) : f1 - (f1 - f0) * (1.0f - wf);
+                if (c == a.iu) uu = r;
+                if (c == a.iv) vv = r;
+                if (c == a.iw) ww = r;
+                if (c != a.iw) a.dst3[(static_cast<int64_t>(c) * a.za + j) * ncol + col] = r;
+            }
+            // Slope correction of w in float64, rounded once (vertical_grid.py:296-316).
+            double taper = 1.0 - tg[g] / ztop_c;
+            taper = taper < 0.0 ? 0.0 : (taper > 1.0 ? 1.0 : taper);
+            const double slope = static_cast<double>(uu) * dhdx + static_cast<double>(vv * dhdy);
+            a.dst3[(static_cast<int64_t>(a.iw) * a.za + j) * ncol + col] =
+                static_cast<float>(static_cast<double>(ww) - taper * slope);
+            const double p0 = a.p_hpa[slo], p1 = a.p_hpa[shi];
+            pval[g] = (fabs(fr) < 0.5) ? p0 + fr * (p1 - p0) : p1 - (p1 - p0) * (1.0 - fr);
         }
-        // Slope correction of w in float64, rounded once (vertical_grid.py:296-316).
-        double taper = 1.0 - tg / ztop_c;
-        taper = taper < 0.0 ? 0.0 : (taper > 1.0 ? 1.0 : taper);
-        const double slope = static_cast<double>(uu) * dhdx + static_cast<double>(vv * dhdy);
-        a.dst3[(static_cast<int64_t>(a.iw) * a.za + j) * ncol + col] =
-            static_cast<float>(static_cast<double>(ww) - taper * slope);
-        const double p0 = a.p_hpa[slo], p1 = a.p_hpa[shi];
-        pval = (fabs(fr) < 0.5) ? p0 + fr * (p1 - p0) : p1 - (p1 - p0) * (1.0 - fr);
     }
-    psum[threadIdx.x] = pval;
-    __syncthreads();
-    for (int off = kColBlock / 2; off > 0; off >>= 1) {
-        if (static_cast<int>(threadIdx.x) < off) psum[threadIdx.x] += psum[threadIdx.x + off];
-        __syncthreads();
+    // per-wavefront pressure sums (a fixed butterfly: the same bits every run)
+#pragma unroll
+    for (int g = 0; g < kResLevels; ++g) {
+        double v = pval[g];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        pval[g] = v;
     }
-    if (threadIdx.x == 0) a.p_part[static_cast<int64_t>(blockIdx.x) * a.za + j] = psum[0];
+    const int64_t ngrp = (ncol + 63) / 64;
+    if (lane == 0 && grp < ngrp) {
+#pragma unroll
+        for (int g = 0; g < kResLevels; ++g)
+            if (j0 + g < a.za) a.p_part[grp * a.za + j0 + g] = pval[g];
+    }
 }
 
 constexpr int kMeanStage = 1024;  // partials staged in LDS per pass
 
-// One workgroup per level: the per-block partials are loaded in parallel into
-// LDS, then added in block order by one thread (the order the float64 sum
-// has always had), so the loads no longer wait on one another.
+// One workgroup per level: the per-wavefront partials are loaded in parallel
+// into LDS, then added in column order by one thread, so the float64 sum has
+// a fixed order and the loads do not wait on one another.
 __global__ __launch_bounds__(256) void pressure_mean_kernel(const double* __restrict__ part, int64_t nblk,
                                                             int32_t za, int64_t ncol, double* __restrict__ out) {
     __shared__ double stage[kMeanStage];
@@ -192,100 +231,194 @@ struct WindowArgs {
 
 __device__ __forceinline__ float safe_den(float d) { return fabsf(d) < 1e-6f ? 1e-6f : d; }
 
-// Levels per workgroup of the window build: one 128 B run of records per
-// column (bfloat16: 8 records of 16 B; float32: 8 plane-A entries of 16 B, and
-// 8 plane-B entries of 8 B), so the column-major writes go out in whole or
-// half cache lines.
-template <bool BF16>
-constexpr int window_levels() { return 8; }
+// Window build tile: a workgroup takes kWinX consecutive columns of one row
+// (the lanes of a wavefront, so the field reads coalesce) and kWinZ
+// consecutive levels (one wavefront each). A column's kWinZ records are one
+// 128 B run in the window (bfloat16: 8 records of 16 B; float32: 8 plane-A
+// entries of 16 B and 8 plane-B entries of 8 B), written whole from LDS.
+constexpr int kWinX = 64;
+constexpr int kWinZ = 8;
+constexpr int kWinT = kWinX + 2;  // stencil tile width: one column either side
 
-// One thread per (column, level); a workgroup takes kColBlock / kTz
-// consecutive columns (the fastest index, so the field reads coalesce) times
-// kTz consecutive levels (blockIdx.y). The records are staged in LDS and
-// written column by column. Level 0's threads also write the surface record.
+// local_std's 3 x 3 stencil (two-pass population std, edge-clipped) over a
+// staged tile of hourly means whose row 1, column lane + 1 is the centre;
+// the flags say which neighbouring rows and columns exist. Same values, same
+// order of the sums.
+__device__ __forceinline__ float stencil_std3(const float (&t)[3][kWinT], int lane, bool ylo, bool yhi, bool xlo,
+                                              bool xhi) {
+    float v[9];
+    bool in[9];
+    int cnt = 0;
+    float sum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int dy = q / 3, dx = q % 3;
+        in[q] = (dy != 0 || ylo) && (dy != 2 || yhi) && (dx != 0 || xlo) && (dx != 2 || xhi);
+        v[q] = t[dy][lane + dx];
+        if (in[q]) {
+            sum += v[q];
+            ++cnt;
+        }
+    }
+    const float mean = sum / static_cast<float>(cnt);
+    float ss = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        if (in[q]) {
+            const float d = v[q] - mean;
+            ss += d * d;
+        }
+    }
+    return sqrtf(ss / static_cast<float>(cnt));
+}
+
+// One thread per (column, level). Everything a thread needs from its
+// neighbours is evaluated once per workgroup into LDS first: the hourly-mean
+// u and v over the 3 x 3 stencil rows of each level (the meander spread), and
+// theta, rho, mean u and mean v of the row at the kWinZ + 2 levels the
+// vertical gradients touch. The values and the order of every sum are those
+// of the per-thread evaluation, so the records do not depend on the tiling.
 template <bool BF16>
-__global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
-    constexpr int kTz = window_levels<BF16>();
-    constexpr int kTc = kColBlock / kTz;
-    // staged [column][level]: bfloat16 records (wind, aux), or float32 plane-A
-    // entries (wind, aux) and plane-B entries (wind, aux)
-    __shared__ uint4 stage[2][kColBlock];
-    __shared__ uint2 stage_b[BF16 ? 1 : 2][BF16 ? 1 : kColBlock];
+__global__ __launch_bounds__(kWinX * kWinZ) void window_kernel(const WindowArgs a) {
+    // One LDS block serves two phases: first the neighbour tiles, then (after
+    // every thread has its values in registers) the staged records.
+    constexpr int kTileFloats = kWinZ * 2 * 3 * kWinT;  // [level][u, v][row y-1, y, y+1][column x0-1 ...]
+    constexpr int kRowFloats = (kWinZ + 2) * 4 * kWinX;  // [level z0-1 ...][theta, rho, mean u, mean v][column]
+    constexpr int kNeighbourBytes = 4 * (kTileFloats + kRowFloats);
+    constexpr int kStageBytes = kWinX * kWinZ * (BF16 ? 32 : 48);  // [column][level] records
+    constexpr int kSmemBytes = kNeighbourBytes > kStageBytes ? kNeighbourBytes : kStageBytes;
+    __shared__ uint4 smem[(kSmemBytes + 15) / 16];
+    auto tile = reinterpret_cast<float (*)[2][3][kWinT]>(smem);
+    auto rowv = reinterpret_cast<float (*)[4][kWinX]>(reinterpret_cast<float*>(smem) + kTileFloats);
+    uint4* const stage0 = smem;                       // bfloat16 records, or float32 plane-A entries
+    uint4* const stage1 = smem + kWinX * kWinZ;
+    uint2* const stage_b0 = reinterpret_cast<uint2*>(smem + 2 * kWinX * kWinZ);  // float32 plane-B entries
+    uint2* const stage_b1 = stage_b0 + kWinX * kWinZ;
     const int64_t ncol = static_cast<int64_t>(a.ny) * a.nx;
-    const int tc = static_cast<int>(threadIdx.x) % kTc;
-    const int tz = static_cast<int>(threadIdx.x) / kTc;
-    const int64_t col = static_cast<int64_t>(blockIdx.x) * kTc + tc;
-    const int nz = a.nz;
-    const int z = static_cast<int>(blockIdx.y) * kTz + tz;
-    const bool ok = col < ncol && z < nz;
-    const int y = static_cast<int>((ok ? col : 0) / a.nx);
-    const int x = static_cast<int>((ok ? col : 0) % a.nx);
-    auto at = [&](const float* f, int ch, int zz) {
-        return f[(static_cast<int64_t>(ch) * nz + zz) * ncol + col];
+    const int nz = a.nz, nx = a.nx, ny = a.ny;
+    const int lane = static_cast<int>(threadIdx.x) & 63;
+    const int wz = static_cast<int>(threadIdx.x) >> 6;
+    const int nxb = (nx + kWinX - 1) / kWinX;
+    const int y = static_cast<int>(blockIdx.x) / nxb;
+    const int x0 = (static_cast<int>(blockIdx.x) % nxb) * kWinX;
+    const int x = x0 + lane;
+    const int z0 = static_cast<int>(blockIdx.y) * kWinZ;
+    const int z = z0 + wz;
+    const bool ok = x < nx && z < nz;
+    const int64_t col = static_cast<int64_t>(y) * nx + (x < nx ? x : nx - 1);
+    auto at = [&](const float* f, int ch, int zz, int64_t c) {
+        return f[(static_cast<int64_t>(ch) * nz + zz) * ncol + c];
     };
     auto pres = [&](int zz) { return static_cast<float>(a.p_hpa[zz]) * 100.0f; };
-    auto theta = [&](int zz) {
-        const float tm = 0.5f * (at(a.s3, 3, zz) + at(a.e3, 3, zz));
-        return potential_temperature(tm, pres(zz));
-    };
-    auto umid = [&](int zz) { return 0.5f * (at(a.s3, 0, zz) + at(a.e3, 0, zz)); };
-    auto vmid = [&](int zz) { return 0.5f * (at(a.s3, 1, zz) + at(a.e3, 1, zz)); };
-    auto rho = [&](int zz) {
-        const float tm = 0.5f * (at(a.s3, 3, zz) + at(a.e3, 3, zz));
-        return pres(zz) / (287.05f * fmaxf(tm, 1.0f));
-    };
+    // the record's own winds, in flight while the neighbours are staged
+    float su = 0.0f, sv = 0.0f, sw3 = 0.0f, eu = 0.0f, ev = 0.0f, ew3 = 0.0f;
+    if (ok) {
+        su = at(a.s3, 0, z, col);
+        sv = at(a.s3, 1, z, col);
+        sw3 = at(a.s3, 2, z, col);
+        eu = at(a.e3, 0, z, col);
+        ev = at(a.e3, 1, z, col);
+        ew3 = at(a.e3, 2, z, col);
+    }
+    const bool tiled = a.radius == 1;
+    if (tiled && z < nz) {
+        // hourly-mean u and v over rows y-1..y+1, columns x0-1..x0+kWinX
+#pragma unroll
+        for (int it = 0; it < (3 * kWinT + 63) / 64; ++it) {
+            const int q = lane + 64 * it;
+            if (q < 3 * kWinT) {
+                const int r = q / kWinT, t = q % kWinT;
+                const int yy = y + r - 1, xx = x0 + t - 1;
+                float mu = 0.0f, mv = 0.0f;
+                if (yy >= 0 && yy < ny && xx >= 0 && xx < nx) {
+                    const int64_t c = static_cast<int64_t>(yy) * nx + xx;
+                    mu = 0.5f * (at(a.s3, 0, z, c) + at(a.e3, 0, z, c));
+                    mv = 0.5f * (at(a.s3, 1, z, c) + at(a.e3, 1, z, c));
+                }
+                tile[wz][0][r][t] = mu;
+                tile[wz][1][r][t] = mv;
+            }
+        }
+    }
+    // theta, rho, mean u, mean v of the row at levels z0-1 .. z0+kWinZ
+#pragma unroll
+    for (int it = 0; it < ((kWinZ + 2) * kWinX + kWinX * kWinZ - 1) / (kWinX * kWinZ); ++it) {
+        const int q = static_cast<int>(threadIdx.x) + it * kWinX * kWinZ;
+        const int lz = q / kWinX, t = q % kWinX;
+        const int zz = z0 - 1 + lz;
+        if (lz < kWinZ + 2 && zz >= 0 && zz < nz && x0 + t < nx) {
+            const int64_t c = static_cast<int64_t>(y) * nx + x0 + t;
+            const float tm = 0.5f * (at(a.s3, 3, zz, c) + at(a.e3, 3, zz, c));
+            const float pz = pres(zz);
+            rowv[lz][0][t] = potential_temperature(tm, pz);
+            rowv[lz][1][t] = pz / (287.05f * fmaxf(tm, 1.0f));
+            rowv[lz][2][t] = 0.5f * (at(a.s3, 0, zz, c) + at(a.e3, 0, zz, c));
+            rowv[lz][3][t] = 0.5f * (at(a.s3, 1, zz, c) + at(a.e3, 1, zz, c));
+        }
+    }
+    __syncthreads();
+    float sw = 0.0f, tl = 0.0f, rho_z = 0.0f, drho = 0.0f, smu = 0.0f, smv = 0.0f;
     if (ok) {
         const int zl = z == 0 ? 0 : z - 1;
         const int zh = z == nz - 1 ? nz - 1 : z + 1;
+        const int ll = zl - z0 + 1, lh = zh - z0 + 1, lc = wz + 1;
         const float dh = safe_den(a.lev[zh] - a.lev[zl]);
         // vertical gradients (central inside, one-sided at the ends)
-        const float th = theta(z);
-        const float dth = (theta(zh) - theta(zl)) / dh;
-        const float du = (umid(zh) - umid(zl)) / dh;
-        const float dv = (vmid(zh) - vmid(zl)) / dh;
-        const float drho = (rho(zh) - rho(zl)) / dh;
+        const float th = rowv[lc][0][lane];
+        const float dth = (rowv[lh][0][lane] - rowv[ll][0][lane]) / dh;
+        const float du = (rowv[lh][2][lane] - rowv[ll][2][lane]) / dh;
+        const float dv = (rowv[lh][3][lane] - rowv[ll][3][lane]) / dh;
+        rho_z = rowv[lc][1][lane];
+        drho = (rowv[lh][1][lane] - rowv[ll][1][lane]) / dh;
         const float n2 = brunt_vaisala_squared(th, dth);
         const float shear_sq = du * du + dv * dv;
         const float shear_mag = sqrtf(fmaxf(shear_sq, 1e-8f));
         const float ri = gradient_richardson(n2, shear_sq);
         const float kz = free_trop_diffusivity(a.lev[z], shear_mag, ri);
-        float sw, tl;
         free_trop_sigma_tl(kz, n2, sw, tl);
-        const int64_t zoff = static_cast<int64_t>(z) * ncol;
-        const float smu = a.coeff * local_std(a.s3 + zoff, a.e3 + zoff, y, x, a.ny, a.nx, a.radius);
-        const float smv = a.coeff * local_std(a.s3 + (static_cast<int64_t>(nz) + z) * ncol,
-                                              a.e3 + (static_cast<int64_t>(nz) + z) * ncol, y, x, a.ny, a.nx,
-                                              a.radius);
-        const int sl = tc * kTz + tz;
-        if constexpr (BF16) {
-            float wr[8] = {at(a.s3, 0, z), at(a.s3, 1, z), at(a.s3, 2, z), at(a.e3, 0, z),
-                           at(a.e3, 1, z), at(a.e3, 2, z), 0.0f, 0.0f};
-            float xr[8] = {sw, tl, rho(z), drho, smu, smv, 0.0f, 0.0f};
-            Rec8<true>::store(stage[0], sl, wr);
-            Rec8<true>::store(stage[1], sl, xr);
+        if (tiled) {
+            const bool ylo = y > 0, yhi = y < ny - 1, xlo = x > 0, xhi = x < nx - 1;
+            smu = a.coeff * stencil_std3(tile[wz][0], lane, ylo, yhi, xlo, xhi);
+            smv = a.coeff * stencil_std3(tile[wz][1], lane, ylo, yhi, xlo, xhi);
         } else {
-            stage[0][sl] = make_uint4(__float_as_uint(at(a.s3, 0, z)), __float_as_uint(at(a.s3, 1, z)),
-                                      __float_as_uint(at(a.e3, 0, z)), __float_as_uint(at(a.e3, 1, z)));
-            stage_b[0][sl] = make_uint2(__float_as_uint(at(a.s3, 2, z)), __float_as_uint(at(a.e3, 2, z)));
-            stage[1][sl] = make_uint4(__float_as_uint(sw), __float_as_uint(tl), __float_as_uint(rho(z)),
-                                      __float_as_uint(drho));
-            stage_b[1][sl] = make_uint2(__float_as_uint(smu), __float_as_uint(smv));
+            const int64_t zoff = static_cast<int64_t>(z) * ncol;
+            smu = a.coeff * local_std(a.s3 + zoff, a.e3 + zoff, y, x, ny, nx, a.radius);
+            smv = a.coeff * local_std(a.s3 + (static_cast<int64_t>(nz) + z) * ncol,
+                                      a.e3 + (static_cast<int64_t>(nz) + z) * ncol, y, x, ny, nx, a.radius);
+        }
+    }
+    __syncthreads();  // every neighbour read is done: the records take the LDS over
+    const int sl = lane * kWinZ + wz;
+    if (ok) {
+        if constexpr (BF16) {
+            float wr[8] = {su, sv, sw3, eu, ev, ew3, 0.0f, 0.0f};
+            float xr[8] = {sw, tl, rho_z, drho, smu, smv, 0.0f, 0.0f};
+            Rec8<true>::store(stage0, sl, wr);
+            Rec8<true>::store(stage1, sl, xr);
+        } else {
+            stage0[sl] = make_uint4(__float_as_uint(su), __float_as_uint(sv), __float_as_uint(eu),
+                                    __float_as_uint(ev));
+            stage_b0[sl] = make_uint2(__float_as_uint(sw3), __float_as_uint(ew3));
+            stage1[sl] = make_uint4(__float_as_uint(sw), __float_as_uint(tl), __float_as_uint(rho_z),
+                                    __float_as_uint(drho));
+            stage_b1[sl] = make_uint2(__float_as_uint(smu), __float_as_uint(smv));
         }
     }
     __syncthreads();
-    // column-major: a column's levels are contiguous, so each column's kTz
+    // column-major: a column's levels are contiguous, so each column's kWinZ
     // staged entries are one run in the window (float32: in each plane)
     const int64_t ncell = ncol * nz;
-    for (int q = static_cast<int>(threadIdx.x); q < kColBlock; q += kColBlock) {
-        const int64_t c = static_cast<int64_t>(blockIdx.x) * kTc + q / kTz;
-        const int zz = static_cast<int>(blockIdx.y) * kTz + q % kTz;
-        if (c < ncol && zz < nz) {
-            const int64_t g = c * nz + zz;
-            static_cast<uint4*>(a.wind)[g] = stage[0][q];
-            static_cast<uint4*>(a.aux)[g] = stage[1][q];
+    {
+        const int q = static_cast<int>(threadIdx.x);
+        const int xx = x0 + q / kWinZ;
+        const int zz = z0 + q % kWinZ;
+        if (xx < nx && zz < nz) {
+            const int64_t g = (static_cast<int64_t>(y) * nx + xx) * nz + zz;
+            static_cast<uint4*>(a.wind)[g] = stage0[q];
+            static_cast<uint4*>(a.aux)[g] = stage1[q];
             if constexpr (!BF16) {
-                reinterpret_cast<uint2*>(static_cast<uint4*>(a.wind) + ncell)[g] = stage_b[0][q];
-                reinterpret_cast<uint2*>(static_cast<uint4*>(a.aux) + ncell)[g] = stage_b[1][q];
+                reinterpret_cast<uint2*>(static_cast<uint4*>(a.wind) + ncell)[g] = stage_b0[q];
+                reinterpret_cast<uint2*>(static_cast<uint4*>(a.aux) + ncell)[g] = stage_b1[q];
             }
         }
     }
@@ -296,8 +429,8 @@ __global__ __launch_bounds__(kColBlock) void window_kernel(const WindowArgs a) {
         sr[c] = a.s2[static_cast<int64_t>(c) * ncol + col];
         sr[5 + c] = a.e2[static_cast<int64_t>(c) * ncol + col];
     }
-    sr[4] = at(a.s3, 3, 0);
-    sr[9] = at(a.e3, 3, 0);
+    sr[4] = at(a.s3, 3, 0, col);
+    sr[9] = at(a.e3, 3, 0, col);
     sr[10] = 0.0f;
     sr[11] = 0.0f;
     // float in both layouts (the surface scalars are not rounded to bfloat16)
@@ -323,8 +456,8 @@ extern "C" int glide_resample_hour(const float* src3, int32_t c3, int32_t zp, in
     GLIDE_REQUIRE(dlat_deg != 0.0 && dlon_deg != 0.0, "grid spacing must be non-zero");
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int64_t ncol = static_cast<int64_t>(ny) * nx;
-    const int64_t nblk = (ncol + kColBlock - 1) / kColBlock;
-    // Small device scratch: orientation flag + per-block pressure sums.
+    const int64_t nblk = (ncol + 63) / 64;  // pressure partials: one per 64-column wavefront
+    // Small device scratch: orientation flag + per-wavefront pressure sums.
     static thread_local void* scratch = nullptr;
     static thread_local size_t scratch_bytes = 0;
     const size_t need = 256 + sizeof(double) * 2 * kOrientBlocks + sizeof(double) * static_cast<size_t>(nblk) * za;
@@ -343,8 +476,9 @@ extern "C" int glide_resample_hour(const float* src3, int32_t c3, int32_t zp, in
     hipLaunchKernelGGL(level_orientation_final, dim3(1), dim3(1), 0, st, opart, flipped);
     ResampleArgs a{src3, c3, zp, ny, nx, height_agl, agl_levels, za, terrain, m_per_deg_lon, dlat_deg, dlon_deg,
                    p_level_hpa, iu, iv, iw, dst3, part, flipped};
-    hipLaunchKernelGGL(resample_kernel, dim3(static_cast<unsigned>(nblk), static_cast<unsigned>(za)), dim3(kColBlock),
-                       0, st, a);
+    const dim3 grid(static_cast<unsigned>((nblk + kResWaves - 1) / kResWaves),
+                    static_cast<unsigned>((za + kResLevels - 1) / kResLevels));
+    hipLaunchKernelGGL(resample_kernel, grid, dim3(64 * kResWaves), 0, st, a);
     hipLaunchKernelGGL(pressure_mean_kernel, dim3(static_cast<unsigned>(za)), dim3(256), 0, st, part, nblk, za, ncol,
                        p_out_hpa);
     GLIDE_HIP(hipGetLastError());
@@ -363,16 +497,15 @@ extern "C" int glide_build_window(const float* h3_s, const float* h3_e, const fl
     GLIDE_REQUIRE(dtype == GLIDE_MET_F32 || dtype == GLIDE_MET_BF16, "unknown met dtype");
     WindowArgs a{h3_s, h3_e, h2_s, h2_e, p_level_hpa, levels, nz, ny, nx, meander_radius,
                  static_cast<float>(meander_coeff), wind, aux, surf};
-    const int64_t ncol = static_cast<int64_t>(ny) * nx;
     const bool bf16 = dtype == GLIDE_MET_BF16;
-    const int tz = bf16 ? window_levels<true>() : window_levels<false>();
-    const int tc = kColBlock / tz;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const dim3 grid(static_cast<unsigned>((ncol + tc - 1) / tc), static_cast<unsigned>((nz + tz - 1) / tz));
+    const int64_t nxb = (static_cast<int64_t>(nx) + kWinX - 1) / kWinX;
+    GLIDE_REQUIRE(nxb * ny <= 0x7fffffff, "window grid too large");
+    const dim3 grid(static_cast<unsigned>(nxb * ny), static_cast<unsigned>((nz + kWinZ - 1) / kWinZ));
     if (bf16) {
-        hipLaunchKernelGGL(window_kernel<true>, grid, dim3(kColBlock), 0, st, a);
+        hipLaunchKernelGGL(window_kernel<true>, grid, dim3(kWinX * kWinZ), 0, st, a);
     } else {
-        hipLaunchKernelGGL(window_kernel<false>, grid, dim3(kColBlock), 0, st, a);
+        hipLaunchKernelGGL(window_kernel<false>, grid, dim3(kWinX * kWinZ), 0, st, a);
     }
     GLIDE_HIP(hipGetLastError());
     return GLIDE_OK;

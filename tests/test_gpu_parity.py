@@ -154,6 +154,21 @@ def test_window_fields_match_oracle(cfg0):
         close(aux[..., c], win.ft[c], 5e-2, floor, frac=0.9999)
     close(aux[..., 4], win.meander[0], 1e-3, 1e-3, frac=1.0)
     close(aux[..., 5], win.meander[1], 1e-3, 1e-3, frac=1.0)
+    # Against the oracle in float32, the reference's own arithmetic: each
+    # product and difference rounded on its own (no fused multiply-add, as
+    # separate tensor ops give), density and its gradient bit for bit, and the
+    # closure to the last bits of pow/sqrt/divide (measured: within 1.3e-3 on
+    # this grid, 1e-6 on two thirds of the cells). A fused theta or wind
+    # difference across a thin layer moves Ri and N^2 by percents on ~1 % of
+    # the cells, which these bounds reject.
+    w32 = ORUN.build_window(h0, h1, met.spec.agl_levels, met.lon, met.lat, np.float32)
+    assert np.array_equal(aux[..., 2], w32.dens[0]) and np.array_equal(aux[..., 3], w32.dens[1])
+    for c, floor in ((0, 1e-2), (1, 1.0)):
+        close(aux[..., c], w32.ft[c], 1e-6, floor, frac=0.5)
+        close(aux[..., c], w32.ft[c], 1e-3, floor, frac=0.9999)
+        close(aux[..., c], w32.ft[c], 1e-2, floor, frac=1.0)
+    close(aux[..., 4], w32.meander[0], 1e-3, 1e-3, frac=1.0)
+    close(aux[..., 5], w32.meander[1], 1e-3, 1e-3, frac=1.0)
 
 
 # ---------------------------------------------------------------------------
