@@ -228,11 +228,8 @@ __device__ __forceinline__ float cloud_w(const Cloud& c, int k) {
     if (k < c.lcl || k > c.lnb) return 0.0f;
     return 1.0f - static_cast<float>(k - c.lcl) / static_cast<float>(c.lnb - c.lcl);
 }
-__device__ __forceinline__ float cloud_e_of(const Cloud& c, float m_i, int i) {
-    return i < c.lcl ? c.mb * m_i / c.bl_total : 0.0f;
-}
 __device__ __forceinline__ float cloud_e(const Cloud& c, const float* m, int i) {
-    return i < c.lcl ? cloud_e_of(c, m[i], i) : 0.0f;
+    return i < c.lcl ? c.mb * m[i] / c.bl_total : 0.0f;
 }
 __device__ __forceinline__ float cloud_d(const Cloud& c, int k) { return c.mb * cloud_w(c, k) / c.w_total; }
 
@@ -439,114 +436,128 @@ __global__ __launch_bounds__(256) void convect_kernel(const ConvectArgs a) {
 // f[i][j] = scale (e_i d_j / M_b + [i == j+1] phi_j), summed in float64 along
 // rows and columns, give max_i |row_i - col_i| / m_i (non-divergence,
 // emanuel.py:437-446). One wavefront per column. The cloud profile is the one
-// make_cloud builds (same per-level terms, sums in the same order): level k
-// lives on lane k (and k - 64 in a second register above 64 levels), and the
-// running sums walk the levels in order with each level's value broadcast
-// from its lane, so every lane holds the sums without a trip through LDS.
+// make_cloud builds (same per-level terms, sums in the same order), with the
+// per-level terms spread over the lanes and the running sums on lane 0.
 constexpr int kColsPerBlock = 4;
-
-// Level k's value (wave-uniform k) from lane k of a one-level-per-lane register.
-__device__ __forceinline__ float lane_value(float v, int k) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
-}
-
-// Levels [a, b) of a per-lane value (level k on lane k, level k + 64 in the
-// second register), walked in order: f(k, value). Two plain loops, so the
-// walk carries no per-level choice of register.
-template <typename F>
-__device__ __forceinline__ void walk_levels(int a, int b, float lo, float hi, F&& f) {
-    const int m = b < 64 ? b : 64;
-    for (int k = a; k < m; ++k) f(k, lane_value(lo, k));
-    for (int k = a > 64 ? a : 64; k < b; ++k) f(k, lane_value(hi, k - 64));
-}
+constexpr int kEntryCap = 1024;  // cloud-block entries staged per column (rows below base x cloud levels)
 
 __global__ __launch_bounds__(64 * kColsPerBlock) void colsum_check_kernel(const int32_t* i_lcl, const int32_t* i_lnb,
                                                                          const float* m_b, const float* lm_g, int nz,
                                                                          int64_t ncol, double* err, float4* cloud_out) {
+    __shared__ float lm[kConvLevels];
     __shared__ float es[kColsPerBlock][kConvLevels];
     __shared__ float dsh[kColsPerBlock][kConvLevels];
     __shared__ float ph[kColsPerBlock][kConvLevels];
+    __shared__ float rowv[kColsPerBlock][kConvLevels];
+    __shared__ float ent[kColsPerBlock][kEntryCap];
+    __shared__ float totals[kColsPerBlock][3];  // bl_total, w_total, sum_d
+    for (int k = threadIdx.x; k < nz; k += blockDim.x) lm[k] = lm_g[k];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t col = static_cast<int64_t>(blockIdx.x) * kColsPerBlock + wave;
-    if (col >= ncol) return;  // whole wavefronts leave together; no block barrier below
-    const int lcl = i_lcl[col], lnb = i_lnb[col];
-    const float mb = m_b[col];
-    const bool cloud = lcl >= 0 && lnb > lcl && mb > 0.0f;
-    double worst = 0.0;
+    const bool have = col < ncol;
+    int lcl = -1, lnb = -1;
+    float mb = 0.0f;
+    if (have) {
+        lcl = i_lcl[col];
+        lnb = i_lnb[col];
+        mb = m_b[col];
+    }
+    const bool cloud = have && lcl >= 0 && lnb > lcl && mb > 0.0f;
     Cloud c{lcl, lnb, mb, 0.0f, 0.0f, 1.0f};
-    float sum_d = 0.0f;
+    float* e = es[wave];
+    float* d = dsh[wave];
+    float* phi = ph[wave];
+    float* rv = rowv[wave];
     if (cloud) {
-        const int k1 = lane + 64;
-        const float lm0 = lane < nz ? lm_g[lane] : 1.0f;
-        const float lm1 = k1 < nz ? lm_g[k1] : 1.0f;
+        for (int k = lane; k < nz; k += 64) d[k] = cloud_w(c, k);  // detrainment shape first
+    }
+    __syncthreads();
+    if (cloud && lane == 0) {
         float bl = 0.0f;
-        walk_levels(0, lcl, lm0, lm1, [&](int, float v) { bl += v; });
-        const float w0 = cloud_w(c, lane), w1 = cloud_w(c, k1);  // detrainment shape
+#pragma unroll 8
+        for (int i = 0; i < lcl; ++i) bl += lm[i];
         float wt = 0.0f;
-        walk_levels(lcl, lnb + 1, w0, w1, [&](int, float v) { wt += v; });
-        c.bl_total = fmaxf(bl, 1e-12f);
-        c.w_total = fmaxf(wt, 1e-12f);
-        const float e0 = lane < nz ? cloud_e_of(c, lm0, lane) : 0.0f;
-        const float e1 = k1 < nz ? cloud_e_of(c, lm1, k1) : 0.0f;
-        const float d0 = lane < nz ? cloud_d(c, lane) : 0.0f;
-        const float d1 = k1 < nz ? cloud_d(c, k1) : 0.0f;
-        walk_levels(0, nz, d0, d1, [&](int, float v) { sum_d += v; });
-        // row sums of the unscaled matrix, e_i sum(d) / M_b + phi_{i-1}, and
-        // the subsidence phi_i = max(sum e - sum d, 0) below the top level
-        float rv0 = e0 * sum_d / mb, rv1 = e1 * sum_d / mb;
-        float phi0 = 0.0f, phi1 = 0.0f;
+#pragma unroll 8
+        for (int k = lcl; k <= lnb; ++k) wt += d[k];
+        totals[wave][0] = fmaxf(bl, 1e-12f);
+        totals[wave][1] = fmaxf(wt, 1e-12f);
+    }
+    __syncthreads();
+    if (cloud) {
+        c.bl_total = totals[wave][0];
+        c.w_total = totals[wave][1];
+        for (int k = lane; k < nz; k += 64) {
+            e[k] = cloud_e(c, lm, k);
+            d[k] = cloud_d(c, k);
+        }
+    }
+    __syncthreads();
+    if (cloud && lane == 0) {
+        // d is +0 outside [lcl, lnb], and adding +0 leaves the sum as it is
+        float sum_d = 0.0f;
+#pragma unroll 8
+        for (int k = lcl; k <= lnb; ++k) sum_d += d[k];
+        totals[wave][2] = sum_d;
+    }
+    __syncthreads();
+    const float sum_d = cloud ? totals[wave][2] : 0.0f;
+    if (cloud) {
+        for (int k = lane; k < nz; k += 64) rv[k] = e[k] * sum_d / mb;
+    }
+    __syncthreads();
+    if (cloud && lane == 0) {
         float ce = 0.0f, cd = 0.0f, phi_prev = 0.0f;
-        const auto level = [&](int i, float ei, float di) {
-            const bool mine0 = i == lane, mine1 = i == k1;
-            rv0 = mine0 ? rv0 + phi_prev : rv0;
-            rv1 = mine1 ? rv1 + phi_prev : rv1;
-            ce += ei;
-            cd += di;
+#pragma unroll 8
+        for (int i = 0; i < nz; ++i) {
+            rv[i] = rv[i] + phi_prev;  // unscaled row sum of row i
+            ce += e[i];
+            cd += d[i];
             phi_prev = (i < nz - 1) ? fmaxf(ce - cd, 0.0f) : 0.0f;
-            phi0 = mine0 ? phi_prev : phi0;
-            phi1 = mine1 ? phi_prev : phi1;
-        };
-        const int m = nz < 64 ? nz : 64;
-        for (int i = 0; i < m; ++i) level(i, lane_value(e0, i), lane_value(d0, i));
-        for (int i = 64; i < nz; ++i) level(i, lane_value(e1, i - 64), lane_value(d1, i - 64));
-        // CFL cap on the per-layer outflux fraction (emanuel.py:500-504)
+            phi[i] = phi_prev;
+        }
+    }
+    __syncthreads();
+    // e_i is zero at and above cloud base (i >= lcl) and d_j outside
+    // [lcl, lnb], so f[i][j] is +0 except for i < lcl with lcl <= j <= lnb
+    // and on the sub-diagonal j = i - 1. Each lane visits only those entries,
+    // in ascending j as a full sweep would; adding +0 leaves a float64 sum of
+    // non-negative terms unchanged, so both sums are the full sweep's bit for
+    // bit. The block i < lcl, lcl <= j <= lnb is evaluated once, spread over
+    // the lanes, when it fits the staging area.
+    const int nb = lnb - lcl + 1;
+    const bool staged = cloud && lcl * nb <= kEntryCap;
+    float* en = ent[wave];
+    const auto f = [&](int r, int q) { return c.scale * (e[r] * d[q] / mb + ((r == q + 1) ? phi[q] : 0.0f)); };
+    if (cloud) {
+        // CFL cap on the per-layer outflux fraction (emanuel.py:500-504).
         float mx = 0.0f;
-        if (lane < nz) mx = fmaxf(mx, rv0 / fmaxf(lm0, 1e-12f));
-        if (k1 < nz) mx = fmaxf(mx, rv1 / fmaxf(lm1, 1e-12f));
+        for (int i = lane; i < nz; i += 64) mx = fmaxf(mx, rv[i] / fmaxf(lm[i], 1e-12f));
         for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
         if (mx > 0.9f) c.scale = 0.9f / mx;
-        float* e = es[wave];
-        float* d = dsh[wave];
-        float* phi = ph[wave];
-        if (lane < nz) {
-            e[lane] = e0;
-            d[lane] = d0;
-            phi[lane] = phi0;
+        if (staged) {
+            for (int i = 0; i < lcl; ++i)
+                for (int jj = lane; jj < nb; jj += 64) en[i * nb + jj] = f(i, lcl + jj);
         }
-        if (k1 < nz) {
-            e[k1] = e1;
-            d[k1] = d1;
-            phi[k1] = phi1;
-        }
-        // the wavefront's levels are in LDS before any lane reads another's
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        // e_i is zero at and above cloud base (i >= lcl) and d_j outside
-        // [lcl, lnb], so f[i][j] is +0 except for i < lcl with lcl <= j <= lnb
-        // and on the sub-diagonal j = i - 1. Each lane visits only those
-        // entries, in ascending j as a full sweep would; adding +0 leaves a
-        // float64 sum of non-negative terms unchanged, so both sums are the
-        // full sweep's bit for bit.
-        const auto f = [&](int r, int q) { return c.scale * (e[r] * d[q] / mb + ((r == q + 1) ? phi[q] : 0.0f)); };
+    }
+    __syncthreads();
+    double worst = 0.0;
+    if (cloud) {
         for (int i = lane; i < nz; i += 64) {
             double row = 0.0, cols = 0.0;
             if (i >= 1) row += static_cast<double>(f(i, i - 1));  // i - 1 < lcl whenever i < lcl
-            if (i < lcl)
-                for (int j = lcl; j <= lnb; ++j) row += static_cast<double>(f(i, j));
-            if (i >= lcl && i <= lnb)
-                for (int j = 0; j < lcl; ++j) cols += static_cast<double>(f(j, i));
+            if (staged) {
+                if (i < lcl)
+                    for (int jj = 0; jj < nb; ++jj) row += static_cast<double>(en[i * nb + jj]);
+                if (i >= lcl && i <= lnb)
+                    for (int j = 0; j < lcl; ++j) cols += static_cast<double>(en[j * nb + i - lcl]);
+            } else {
+                if (i < lcl)
+                    for (int j = lcl; j <= lnb; ++j) row += static_cast<double>(f(i, j));
+                if (i >= lcl && i <= lnb)
+                    for (int j = 0; j < lcl; ++j) cols += static_cast<double>(f(j, i));
+            }
             if (i + 1 < nz) cols += static_cast<double>(f(i + 1, i));  // i + 1 > lcl - 1 whenever i >= lcl
-            const double r = fabs(row - cols) / fmax(static_cast<double>(i < 64 ? lm0 : lm1), 1e-12);
+            const double r = fabs(row - cols) / fmax(static_cast<double>(lm[i]), 1e-12);
             worst = r > worst ? r : worst;
         }
     }
@@ -554,7 +565,7 @@ __global__ __launch_bounds__(64 * kColsPerBlock) void colsum_check_kernel(const 
         const double o = __shfl_xor(worst, off, 64);
         worst = o > worst ? o : worst;
     }
-    if (lane == 0) {
+    if (have && lane == 0) {
         err[col] = worst;
         if (cloud_out != nullptr)
             cloud_out[col] = cloud ? make_float4(c.bl_total, c.w_total, c.scale, sum_d)

@@ -94,7 +94,6 @@ __global__ __launch_bounds__(64 * kResWaves) void resample_kernel(const Resample
     const int j0 = static_cast<int>(blockIdx.y) * kResLevels;
     const bool flip = *a.flipped != 0;
     auto src_idx = [&](int asc) { return flip ? (zp - 1 - asc) : asc; };
-    auto hasc = [&](int asc) { return static_cast<double>(a.h[static_cast<int64_t>(src_idx(asc)) * ncol + col]); };
 
     // idx = searchsorted(h, tg, side="left") on the ascending heights, i.e. the
     // number of levels below tg, and k0 the number below ground; counting is
@@ -140,40 +139,66 @@ __global__ __launch_bounds__(64 * kResWaves) void resample_kernel(const Resample
         const float dhdy = dlat_g / static_cast<float>(kMPerDegLatD);  // float32, as numpy keeps it
         const double ztop = a.agl[a.za - 1];
         const double ztop_c = ztop > 1.0 ? ztop : 1.0;
+        // bracketing source levels and fractions of the thread's targets; every
+        // level's loads go out together, a channel at a time
+        int slo[kResLevels], shi[kResLevels];
+        double fr[kResLevels];
+        float wf[kResLevels];
 #pragma unroll
         for (int g = 0; g < kResLevels; ++g) {
-            const int j = j0 + g;
-            if (j >= a.za) break;
             int lower = idx[g] - 1;
             if (lower < 0) lower = 0;
             if (lower < k0) lower = k0;
             if (lower > zp - 2) lower = zp - 2;
-            const double hlo = hasc(lower), hhi = hasc(lower + 1);
-            double den = hhi - hlo;
+            slo[g] = src_idx(lower);
+            shi[g] = src_idx(lower + 1);
+        }
+        double hlo[kResLevels], hhi[kResLevels];
+#pragma unroll
+        for (int g = 0; g < kResLevels; ++g) {
+            hlo[g] = static_cast<double>(a.h[static_cast<int64_t>(slo[g]) * ncol + col]);
+            hhi[g] = static_cast<double>(a.h[static_cast<int64_t>(shi[g]) * ncol + col]);
+        }
+#pragma unroll
+        for (int g = 0; g < kResLevels; ++g) {
+            double den = hhi[g] - hlo[g];
             if (den < 1e-6) den = 1e-6;
-            double fr = (tg[g] - hlo) / den;
-            fr = fr < 0.0 ? 0.0 : (fr > 1.0 ? 1.0 : fr);
-            const int slo = src_idx(lower), shi = src_idx(lower + 1);
-            const float wf = static_cast<float>(fr);
-            float uu = 0.0f, vv = 0.0f, ww = 0.0f;
-            for (int c = 0; c < a.c3; ++c) {
-                const float* fc = a.src3 + static_cast<int64_t>(c) * zp * ncol;
-                const float f0 = fc[static_cast<int64_t>(slo) * ncol + col];
-                const float f1 = fc[static_cast<int64_t>(shi) * ncol + col];
-                const float r = (fabsf(wf) < 0.5f) ? f0 + wf * (f1 - f0) : f1 - (f1 - f0) * (1.0f - wf);
-                if (c == a.iu) uu = r;
-                if (c == a.iv) vv = r;
-                if (c == a.iw) ww = r;
-                if (c != a.iw) a.dst3[(static_cast<int64_t>(c) * a.za + j) * ncol + col] = r;
+            double q = (tg[g] - hlo[g]) / den;
+            fr[g] = q < 0.0 ? 0.0 : (q > 1.0 ? 1.0 : q);
+            wf[g] = static_cast<float>(fr[g]);
+        }
+        float uu[kResLevels], vv[kResLevels], ww[kResLevels];
+        for (int c = 0; c < a.c3; ++c) {
+            const float* fc = a.src3 + static_cast<int64_t>(c) * zp * ncol;
+            float f0[kResLevels], f1[kResLevels];
+#pragma unroll
+            for (int g = 0; g < kResLevels; ++g) {
+                f0[g] = fc[static_cast<int64_t>(slo[g]) * ncol + col];
+                f1[g] = fc[static_cast<int64_t>(shi[g]) * ncol + col];
             }
+#pragma unroll
+            for (int g = 0; g < kResLevels; ++g) {
+                const int j = j0 + g;
+                const float r = (fabsf(wf[g]) < 0.5f) ? f0[g] + wf[g] * (f1[g] - f0[g])
+                                                      : f1[g] - (f1[g] - f0[g]) * (1.0f - wf[g]);
+                if (c == a.iu) uu[g] = r;
+                if (c == a.iv) vv[g] = r;
+                if (c == a.iw) ww[g] = r;
+                if (c != a.iw && j < a.za) a.dst3[(static_cast<int64_t>(c) * a.za + j) * ncol + col] = r;
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < kResLevels; ++g) {
+            const int j = j0 + g;
+            if (j >= a.za) break;
             // Slope correction of w in float64, rounded once (vertical_grid.py:296-316).
             double taper = 1.0 - tg[g] / ztop_c;
             taper = taper < 0.0 ? 0.0 : (taper > 1.0 ? 1.0 : taper);
-            const double slope = static_cast<double>(uu) * dhdx + static_cast<double>(vv * dhdy);
+            const double slope = static_cast<double>(uu[g]) * dhdx + static_cast<double>(vv[g] * dhdy);
             a.dst3[(static_cast<int64_t>(a.iw) * a.za + j) * ncol + col] =
-                static_cast<float>(static_cast<double>(ww) - taper * slope);
-            const double p0 = a.p_hpa[slo], p1 = a.p_hpa[shi];
-            pval[g] = (fabs(fr) < 0.5) ? p0 + fr * (p1 - p0) : p1 - (p1 - p0) * (1.0 - fr);
+                static_cast<float>(static_cast<double>(ww[g]) - taper * slope);
+            const double p0 = a.p_hpa[slo[g]], p1 = a.p_hpa[shi[g]];
+            pval[g] = (fabs(fr[g]) < 0.5) ? p0 + fr[g] * (p1 - p0) : p1 - (p1 - p0) * (1.0 - fr[g]);
         }
     }
     // per-wavefront pressure sums (a fixed butterfly: the same bits every run)
