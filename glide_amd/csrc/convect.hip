@@ -452,7 +452,9 @@ __global__ __launch_bounds__(64 * kColsPerBlock) void colsum_check_kernel(const 
     __shared__ float ent[kColsPerBlock][kEntryCap];
     __shared__ float totals[kColsPerBlock][3];  // bl_total, w_total, sum_d
     for (int k = threadIdx.x; k < nz; k += blockDim.x) lm[k] = lm_g[k];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wavefront's column (uniform: its descriptors and loop bounds are scalar)
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
     const int64_t col = static_cast<int64_t>(blockIdx.x) * kColsPerBlock + wave;
     const bool have = col < ncol;
     int lcl = -1, lnb = -1;

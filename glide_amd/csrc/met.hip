@@ -87,7 +87,8 @@ __global__ __launch_bounds__(64 * kResWaves) void resample_kernel(const Resample
 #pragma clang fp contract(off)
     const int64_t ncol = static_cast<int64_t>(a.ny) * a.nx;
     const int lane = static_cast<int>(threadIdx.x) & 63;
-    const int64_t grp = static_cast<int64_t>(blockIdx.x) * kResWaves + (threadIdx.x >> 6);
+    const int64_t grp = static_cast<int64_t>(blockIdx.x) * kResWaves +
+                        __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);  // uniform
     const int64_t col = grp * 64 + lane;
     const bool live = col < ncol;
     const int zp = a.zp;
@@ -322,7 +323,8 @@ __global__ __launch_bounds__(kWinX * kWinZ) void window_kernel(const WindowArgs 
     const int64_t ncol = static_cast<int64_t>(a.ny) * a.nx;
     const int nz = a.nz, nx = a.nx, ny = a.ny;
     const int lane = static_cast<int>(threadIdx.x) & 63;
-    const int wz = static_cast<int>(threadIdx.x) >> 6;
+    // the wavefront's level (uniform: its plane offsets are scalar arithmetic)
+    const int wz = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     const int nxb = (nx + kWinX - 1) / kWinX;
     const int y = static_cast<int>(blockIdx.x) / nxb;
     const int x0 = (static_cast<int>(blockIdx.x) % nxb) * kWinX;
@@ -368,8 +370,7 @@ __global__ __launch_bounds__(kWinX * kWinZ) void window_kernel(const WindowArgs 
     // theta, rho, mean u, mean v of the row at levels z0-1 .. z0+kWinZ
 #pragma unroll
     for (int it = 0; it < ((kWinZ + 2) * kWinX + kWinX * kWinZ - 1) / (kWinX * kWinZ); ++it) {
-        const int q = static_cast<int>(threadIdx.x) + it * kWinX * kWinZ;
-        const int lz = q / kWinX, t = q % kWinX;
+        const int lz = wz + it * kWinZ, t = lane;  // wavefront-uniform level slot
         const int zz = z0 - 1 + lz;
         if (lz < kWinZ + 2 && zz >= 0 && zz < nz && x0 + t < nx) {
             const int64_t c = static_cast<int64_t>(y) * nx + x0 + t;
