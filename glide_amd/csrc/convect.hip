@@ -450,7 +450,7 @@ __global__ __launch_bounds__(64 * kColsPerBlock) void colsum_check_kernel(const 
     __shared__ float ph[kColsPerBlock][kConvLevels];
     __shared__ float rowv[kColsPerBlock][kConvLevels];
     __shared__ float ent[kColsPerBlock][kEntryCap];
-    __shared__ float totals[kColsPerBlock][3];  // bl_total, w_total, sum_d
+    __shared__ float totals[kColsPerBlock][4];  // bl_total, w_total, sum_d, phi above cloud top
     for (int k = threadIdx.x; k < nz; k += blockDim.x) lm[k] = lm_g[k];
     // the wavefront's column (uniform: its descriptors and loop bounds are scalar)
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
@@ -508,17 +508,30 @@ __global__ __launch_bounds__(64 * kColsPerBlock) void colsum_check_kernel(const 
     }
     __syncthreads();
     if (cloud && lane == 0) {
+        // Above cloud top e and d are +0, so the running sums stop changing and
+        // every level there has phi = max(sum e - sum d, 0) (0 on the top
+        // level): the walk ends at lnb and the lanes fill in the rest.
         float ce = 0.0f, cd = 0.0f, phi_prev = 0.0f;
 #pragma unroll 8
-        for (int i = 0; i < nz; ++i) {
+        for (int i = 0; i <= lnb; ++i) {
             rv[i] = rv[i] + phi_prev;  // unscaled row sum of row i
             ce += e[i];
             cd += d[i];
             phi_prev = (i < nz - 1) ? fmaxf(ce - cd, 0.0f) : 0.0f;
             phi[i] = phi_prev;
         }
+        totals[wave][3] = fmaxf(ce - cd, 0.0f);
     }
     __syncthreads();
+    if (cloud) {
+        const float p_top = totals[wave][3];
+        for (int i = lane; i < nz; i += 64) {
+            if (i > lnb) {
+                rv[i] = rv[i] + p_top;  // phi_{i-1}: level lnb's or the constant above it
+                phi[i] = (i < nz - 1) ? p_top : 0.0f;
+            }
+        }
+    }
     // e_i is zero at and above cloud base (i >= lcl) and d_j outside
     // [lcl, lnb], so f[i][j] is +0 except for i < lcl with lcl <= j <= lnb
     // and on the sub-diagonal j = i - 1. Each lane visits only those entries,
@@ -537,8 +550,19 @@ __global__ __launch_bounds__(64 * kColsPerBlock) void colsum_check_kernel(const 
         for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
         if (mx > 0.9f) c.scale = 0.9f / mx;
         if (staged) {
-            for (int i = 0; i < lcl; ++i)
-                for (int jj = lane; jj < nb; jj += 64) en[i * nb + jj] = f(i, lcl + jj);
+            // entry t = i * nb + jj on lane t mod 64, (i, jj) stepped without a division per entry
+            const int n_ent = lcl * nb;
+            const int qs = 64 / nb, rs = 64 % nb;
+            int i = lane / nb, jj = lane % nb;
+            for (int t = lane; t < n_ent; t += 64) {
+                en[t] = f(i, lcl + jj);
+                i += qs;
+                jj += rs;
+                if (jj >= nb) {
+                    jj -= nb;
+                    ++i;
+                }
+            }
         }
     }
     __syncthreads();
