@@ -60,8 +60,18 @@ __global__ __launch_bounds__(kFinThreads) void fixed_finalize_kernel(const long 
     long long sum = 0;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kFinThreads;
     for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * kFinThreads + threadIdx.x; c < cnt; c += stride) {
+        // the copies' loads go out eight at a time (integer sums: any order is exact)
+        const long long* q = fixed + base + c;
         long long v = 0;
-        for (int k = 0; k < copies; ++k) v += fixed[static_cast<uint64_t>(k) * L.acc_cells + base + c];
+        int k = 0;
+        for (; k + 8 <= copies; k += 8) {
+            long long t[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) t[u] = q[static_cast<uint64_t>(k + u) * L.acc_cells];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v += t[u];
+        }
+        for (; k < copies; ++k) v += q[static_cast<uint64_t>(k) * L.acc_cells];
         sum += v;
         slab[obase + c] = ldexpf(static_cast<float>(v), -e);
     }
