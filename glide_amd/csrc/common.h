@@ -9,6 +9,18 @@
 
 namespace glide {
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8), and each XCD has its own L2. This bijection of [0, nb) gives each
+// XCD a contiguous eighth of the logical blocks, so neighbouring blocks that
+// touch the same lines (particles binned by met cell and their met records,
+// a permutation's nearby sources) share one L2 instead of eight.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t xcd = b % kXcd, idx = b / kXcd;
+    const uint32_t q = nb / kXcd, r = nb % kXcd;
+    return xcd * q + (xcd < r ? xcd : r) + idx;
+}
+
 // ---------------------------------------------------------------------------
 // Host-side error plumbing (thread-local message, status codes of glide.h)
 // ---------------------------------------------------------------------------

@@ -116,7 +116,8 @@ __device__ __forceinline__ void mv(const T* s, T* d, int64_t j, int64_t i) {
 }
 
 __global__ __launch_bounds__(256) void permute_kernel(const PermuteArgs a) {
-    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    // contiguous destinations per XCD: their sources, mostly nearby, stay in one L2
+    const int64_t j = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     if (j >= a.src.n) return;
     const int64_t i = a.idx[j];
     const GlideParticles& s = a.src;

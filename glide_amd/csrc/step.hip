@@ -499,16 +499,6 @@ constexpr int kModePlain = 0;  // scalars from the launch
 constexpr int kModeTable = 1;  // scalars from the device table row (captured step body)
 constexpr int kModeSlots = 2;  // served slots: scalars and window per wavefront
 
-// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
-// b % 8), and each XCD has its own L2. This bijection gives each XCD a
-// contiguous eighth of the particles, so particles binned by met cell keep
-// their met records in one L2 instead of all eight.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-    constexpr uint32_t kXcd = 8;
-    const uint32_t xcd = b % kXcd, idx = b / kXcd;
-    const uint32_t q = nb / kXcd, r = nb % kXcd;
-    return xcd * q + (xcd < r ? xcd : r) + idx;
-}
 
 // FIXED: the fixed-point footprint (GLIDE_F_FIXED) as its own instantiation,
 // so neither variant keeps the other's pointers live (the kernel is short of
