@@ -87,12 +87,18 @@ __global__ __launch_bounds__(64 * kResWaves) void resample_kernel(const Resample
 #pragma clang fp contract(off)
     const int64_t ncol = static_cast<int64_t>(a.ny) * a.nx;
     const int lane = static_cast<int>(threadIdx.x) & 63;
-    const int64_t grp = static_cast<int64_t>(blockIdx.x) * kResWaves +
-                        __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);  // uniform
+    // logical block: column blocks in order, each with its level groups next
+    // to one another, and each XCD a contiguous range (the column block's
+    // heights and fields stay in one L2 for all its level groups)
+    const int n_lg = (a.za + kResLevels - 1) / kResLevels;
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t cb = lb / n_lg;
+    const int lg = static_cast<int>(lb % n_lg);
+    const int64_t grp = cb * kResWaves + __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);  // uniform
     const int64_t col = grp * 64 + lane;
     const bool live = col < ncol;
     const int zp = a.zp;
-    const int j0 = static_cast<int>(blockIdx.y) * kResLevels;
+    const int j0 = lg * kResLevels;
     const bool flip = *a.flipped != 0;
     auto src_idx = [&](int asc) { return flip ? (zp - 1 - asc) : asc; };
 
@@ -326,10 +332,17 @@ __global__ __launch_bounds__(kWinX * kWinZ) void window_kernel(const WindowArgs 
     // the wavefront's level (uniform: its plane offsets are scalar arithmetic)
     const int wz = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     const int nxb = (nx + kWinX - 1) / kWinX;
-    const int y = static_cast<int>(blockIdx.x) / nxb;
-    const int x0 = (static_cast<int>(blockIdx.x) % nxb) * kWinX;
+    const int nzb = (nz + kWinZ - 1) / kWinZ;
+    // logical block: rows in order, each row's segments and level groups next
+    // to one another, and each XCD a contiguous range of rows (the stencil's
+    // neighbouring rows stay in one L2)
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const int zb = static_cast<int>(lb % nzb);
+    const int rs = static_cast<int>(lb / nzb);
+    const int y = rs / nxb;
+    const int x0 = (rs % nxb) * kWinX;
     const int x = x0 + lane;
-    const int z0 = static_cast<int>(blockIdx.y) * kWinZ;
+    const int z0 = zb * kWinZ;
     const int z = z0 + wz;
     const bool ok = x < nx && z < nz;
     const int64_t col = static_cast<int64_t>(y) * nx + (x < nx ? x : nx - 1);
@@ -502,9 +515,9 @@ extern "C" int glide_resample_hour(const float* src3, int32_t c3, int32_t zp, in
     hipLaunchKernelGGL(level_orientation_final, dim3(1), dim3(1), 0, st, opart, flipped);
     ResampleArgs a{src3, c3, zp, ny, nx, height_agl, agl_levels, za, terrain, m_per_deg_lon, dlat_deg, dlon_deg,
                    p_level_hpa, iu, iv, iw, dst3, part, flipped};
-    const dim3 grid(static_cast<unsigned>((nblk + kResWaves - 1) / kResWaves),
-                    static_cast<unsigned>((za + kResLevels - 1) / kResLevels));
-    hipLaunchKernelGGL(resample_kernel, grid, dim3(64 * kResWaves), 0, st, a);
+    const int64_t n_rb = ((nblk + kResWaves - 1) / kResWaves) * ((za + kResLevels - 1) / kResLevels);
+    GLIDE_REQUIRE(n_rb <= 0x7fffffff, "resample grid too large");
+    hipLaunchKernelGGL(resample_kernel, dim3(static_cast<unsigned>(n_rb)), dim3(64 * kResWaves), 0, st, a);
     hipLaunchKernelGGL(pressure_mean_kernel, dim3(static_cast<unsigned>(za)), dim3(256), 0, st, part, nblk, za, ncol,
                        p_out_hpa);
     GLIDE_HIP(hipGetLastError());
@@ -526,8 +539,9 @@ extern "C" int glide_build_window(const float* h3_s, const float* h3_e, const fl
     const bool bf16 = dtype == GLIDE_MET_BF16;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int64_t nxb = (static_cast<int64_t>(nx) + kWinX - 1) / kWinX;
-    GLIDE_REQUIRE(nxb * ny <= 0x7fffffff, "window grid too large");
-    const dim3 grid(static_cast<unsigned>(nxb * ny), static_cast<unsigned>((nz + kWinZ - 1) / kWinZ));
+    const int64_t n_wb = nxb * ny * ((nz + kWinZ - 1) / kWinZ);
+    GLIDE_REQUIRE(n_wb <= 0x7fffffff, "window grid too large");
+    const dim3 grid(static_cast<unsigned>(n_wb));
     if (bf16) {
         hipLaunchKernelGGL(window_kernel<true>, grid, dim3(kWinX * kWinZ), 0, st, a);
     } else {
