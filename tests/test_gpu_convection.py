@@ -149,3 +149,49 @@ def test_engine_runs_with_parcel_lift():
         assert np.array_equal(a[k], b[k]), k
     assert np.array_equal(eng.footprint.cpu().numpy(), g.footprint.cpu().numpy())
     g.close()
+
+
+def test_column_sum_check_many_levels_bit_exact():
+    """The per-column float64 mass-conservation check (glide_convect with
+    colsum_err) on 128 levels and hand-picked clouds, against the oracle's
+    float32 matrix summed in float64 in index order, exactly: cloud blocks
+    small enough to be staged and too large for it (rows below base x cloud
+    levels > 1,024), levels above 64 (the second lane register), clouds from
+    the ground, one-layer clouds, the top level as cloud top, and columns that
+    do not fire (no cloud, no flux)."""
+    import ctypes as C
+
+    nz, ny, nx = 128, 2, 8
+    rng = np.random.default_rng(7)
+    p_hpa = np.linspace(1000.0, 20.0, nz)
+    lm = OC.layer_masses(p_hpa * 100.0).astype(np.float32)
+    clouds = [(0, 5), (1, 2), (10, 60), (40, 100), (30, 127), (63, 64), (64, 127), (3, 127),
+              (-1, -1), (5, 9), (2, 50), (20, 90), (100, 127), (0, 127), (12, 13), (7, 8)]
+    lcl = np.array([c[0] for c in clouds], np.int32).reshape(ny, nx)
+    lnb = np.array([c[1] for c in clouds], np.int32).reshape(ny, nx)
+    mb = rng.uniform(0.5, 40.0, (ny, nx)).astype(np.float32)
+    mb[0, 1] = 0.0  # a cloud with no cloud-base flux does not fire
+    assert any(a >= 0 and a * (b - a + 1) > 1024 for a, b in clouds)  # the unstaged path
+    assert any(0 <= a and a * (b - a + 1) <= 1024 for a, b in clouds)  # the staged path
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    lcl_d, lnb_d, mb_d, lm_d = dev(lcl), dev(lnb), dev(mb), dev(lm)
+    levels = dev(np.linspace(0.0, 20000.0, nz).astype(np.float32))
+    one = torch.zeros(1, dtype=torch.float32, device="cuda")
+    alive = torch.zeros(1, dtype=torch.uint8, device="cuda")
+    parts = L.GlideParticles(n=0, pos_stride=1, lon=one.data_ptr(), lat=one.data_ptr(), z=one.data_ptr(),
+                             r_off=one.data_ptr(), alive=alive.data_ptr())
+    grid = L.GlideGrid(nx=nx, ny=ny, nz=nz, reserved=0, lon0=0.0, lon1=1.75, lat0=0.0, lat1=0.25,
+                       levels=levels.data_ptr())
+    prm = L.GlideStepParams(dt=60.0, alpha=0.5, max_substeps=1, flags=0)
+    err = torch.full((ny * nx,), -1.0, dtype=torch.float64, device="cuda")
+    print(f"glide_convect colsum nz={nz} columns={ny * nx}", flush=True)
+    L.check(L.lib().glide_convect(C.byref(prm), C.byref(parts), C.byref(grid), lcl_d.data_ptr(), lnb_d.data_ptr(),
+                                  mb_d.data_ptr(), lm_d.data_ptr(), err.data_ptr(), None, None, None, _stream()))
+    torch.cuda.synchronize()
+    got = err.cpu().numpy().reshape(ny, nx)
+    for iy in range(ny):
+        for ix in range(nx):
+            f32 = OC.mass_flux_matrix_f32(lm, int(lcl[iy, ix]), int(lnb[iy, ix]), float(mb[iy, ix]))
+            want = OC.column_sum_error(f32, lm)
+            assert got[iy, ix] == want, (iy, ix, lcl[iy, ix], lnb[iy, ix], got[iy, ix], want)
+    assert got[0, 1] == 0.0 and got[1, 0] == 0.0  # no flux / no cloud
